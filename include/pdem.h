@@ -1,0 +1,235 @@
+/*
+ * pdem.h -- C-ABI of the MI355X explicit time-step engine for PeriDEM models.
+ *
+ * The engine owns the per-node state of a PeriDEM model on one GPU and runs
+ * the central-difference time step there: peridynamic bond forces (bond-based
+ * RNP and state-based PD, with bond breaking and damage), the cell-list
+ * contact search, the contact / damping / friction forces, external forces,
+ * displacement boundary conditions and the node update.
+ *
+ * Every entry point below replaces one virtual method of the reference's
+ * model::DEMModel (reference file src/model/dem/demModel.h:74-233); the
+ * mapping is given per function.  Signatures hold only plain C types and
+ * pointers: no HIP or torch types cross this boundary.
+ *
+ * Conventions
+ *  - Return codes are int: 0 = ok, nonzero = error; pdem_last_error() gives
+ *    the message.  No exception crosses the ABI.
+ *  - Point arrays are AoS [n][3] doubles, the memory layout of the reference's
+ *    std::vector<util::Point> (src/util/point.h), so ModelData vectors can be
+ *    passed as &v[0].d_x.  In 2D the third component is ignored on input and
+ *    written as zero on output.
+ *  - The context owns all device memory; host buffers stay owned by the caller
+ *    and are copied at create / get / set.  One host thread per context; calls
+ *    are stream ordered and every pdem_get_* call synchronises.
+ */
+#ifndef PDEM_H
+#define PDEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* material laws (reference src/material/mparticle/material.h) */
+#define PDEM_MAT_RNP 0       /* RnpMaterial      material.h:251-659   */
+#define PDEM_MAT_PDSTATE 1   /* PdState          material.h:1330-1724 */
+
+/* influence functions (reference src/material/mparticle/influenceFn.cpp:14-122) */
+#define PDEM_INF_CONST 0
+#define PDEM_INF_LINEAR 1
+#define PDEM_INF_GAUSSIAN 2
+
+/* arithmetic modes */
+#define PDEM_PREC_FP64 0   /* all FP64, no FMA contraction: parity mode */
+#define PDEM_PREC_MIXED 1  /* FP32 bond/contact math, FP64 sums, FP64 theta/m, FP64 integration */
+
+/* displacement BC time functions (reference src/loading/particleULoading.cpp:115-230) */
+#define PDEM_UFN_CONSTANT 0
+#define PDEM_UFN_LINEAR 1
+#define PDEM_UFN_QUADRATIC 2
+#define PDEM_UFN_SIN 3
+#define PDEM_UFN_ROTATION 4
+#define PDEM_UFN_NONE 5   /* unknown type: u = v = 0 on listed dofs */
+
+/* force BC time functions (reference src/loading/particleFLoading.cpp:85-173) */
+#define PDEM_FFN_CONSTANT 0
+#define PDEM_FFN_LINEAR 1
+#define PDEM_FFN_LINEAR_STEP 2
+#define PDEM_FFN_LINEAR_SLOW_FAST 3
+#define PDEM_FFN_SIN 4
+
+/* fields for pdem_get_field / pdem_set_field */
+#define PDEM_FIELD_X 0      /* double [n][3] current position  (ModelData::d_x)   */
+#define PDEM_FIELD_U 1      /* double [n][3] displacement      (d_u)              */
+#define PDEM_FIELD_V 2      /* double [n][3] velocity          (d_v)              */
+#define PDEM_FIELD_F 3      /* double [n][3] force density     (d_f)              */
+#define PDEM_FIELD_XREF 4   /* double [n][3] reference position (d_xRef)          */
+#define PDEM_FIELD_VOL 5    /* double [n]    nodal volume      (d_vol)            */
+#define PDEM_FIELD_Z 6      /* float  [n]    damage            (d_Z)              */
+#define PDEM_FIELD_THETA 7  /* double [n]    dilation          (d_thetaX)         */
+#define PDEM_FIELD_MX 8     /* double [n]    weighted volume   (d_mX)             */
+#define PDEM_FIELD_VMAG 9   /* double [n]    |v|               (d_vMag)           */
+#define PDEM_FIELD_BONDS 10 /* uint8  [nnz]  1 = broken, PD neighbor-list order    */
+
+#define PDEM_NEIGH_PD 0       /* d_neighPd: peridynamic neighbors (same particle)    */
+#define PDEM_NEIGH_CONTACT 1  /* d_neighC: contact candidates (other particles)      */
+
+/* One particle or wall.  The reference builds these in
+ * particle::BaseParticle::BaseParticle (src/particle/baseParticle.cpp:60-189). */
+typedef struct pdem_particle {
+  int64_t glob_start;        /* first node (d_globStart) */
+  int64_t glob_end;          /* one past last node (d_globEnd) */
+  int64_t center_node;       /* global id of the node closest to the geometric centre
+                                (refParticle.cpp:36-45 + baseParticle.h:825) */
+  int32_t zone;              /* zone id: row/column of the contact table */
+  int32_t type_index;        /* 0 particle, 1 wall (d_typeIndex) */
+  int32_t all_dofs_constrained;
+  int32_t compute_force;     /* d_computeForce */
+  int32_t material_type;     /* PDEM_MAT_* */
+  int32_t irreversible;      /* RNP: irreversible bond breaking */
+  double density;            /* rho */
+  double horizon;            /* delta */
+  double mesh_size;          /* h = minimum node distance in the particle */
+  double bounding_radius;    /* d_geom_p->boundingRadius(): damping reach and mass */
+  double Kn;                 /* broken-bond contact stiffness 18 K / (pi delta^5) */
+  double Rc;                 /* broken-bond contact radius 0.95 h */
+  /* material parameters: RNP   {C, beta, rbar, inv_factor, check_sc_factor}
+                          PDState {K, G, s0} */
+  double mat[6];
+} pdem_particle;
+
+/* Contact data for one zone pair after DEMModel::setupContact()
+ * (demModel.cpp:1531-1600): contact_r already multiplied by hMin when the deck
+ * gave a factor, Kn already multiplied by Kn_Factor. */
+typedef struct pdem_contact_pair {
+  double contact_r;
+  double Kn;
+  double mu;
+  double betan;
+  double kappa;
+} pdem_contact_pair;
+
+/* One displacement or force BC set.  The node selection (particle list / region
+ * tests of needToComputeDof) and the spatial amplitude are static, so they are
+ * resolved at setup into a node list and a per-node amplitude. */
+typedef struct pdem_bc_set {
+  int32_t time_fn;           /* PDEM_UFN_* or PDEM_FFN_* */
+  int32_t dir_mask;          /* bit d set: dof d (0-based) is prescribed */
+  int32_t is_zero;           /* displacement BC: zero-displacement set (applied once) */
+  int32_t reserved;
+  double time_params[4];     /* d_timeFnParams */
+  int64_t n_entries;
+  const int64_t *nodes;      /* global node ids, in the reference's traversal order */
+  const double *amp;         /* per node umax (U BC) or fmax (F BC) after the spatial fn */
+} pdem_bc_set;
+
+typedef struct pdem_desc {
+  int32_t dim;               /* 2 or 3 */
+  int32_t precision;         /* PDEM_PREC_* */
+  int32_t multi_particle;    /* Model.Particle_Sim_Type == Multi_Particle: contact active */
+  int32_t influence_fn_type; /* PDEM_INF_*: process-global, first material wins */
+  double influence_fn_params[2];
+  double dt;
+  int64_t n_nodes;
+  const double *x_ref;       /* [n][3] */
+  const double *u;           /* [n][3] initial displacement (usually 0) */
+  const double *v;           /* [n][3] initial velocity (initial condition applied) */
+  const double *vol;         /* [n] */
+  const uint8_t *fix;        /* [n] bit d = dof d fixed (ParticleULoading::setFixity) */
+  const int64_t *pt_id;      /* [n] particle id of each node */
+  int32_t n_particles;
+  int32_t n_zones;
+  const pdem_particle *particles;      /* [n_particles], walls included */
+  const pdem_contact_pair *contact;    /* [n_zones][n_zones] */
+  double gravity[3];
+  double search_factor;               /* Neighbor.Search_Factor */
+  int64_t neigh_update_interval;      /* Neighbor.Search_Interval */
+  int32_t n_ubc;
+  int32_t n_fbc;
+  const pdem_bc_set *ubc;
+  const pdem_bc_set *fbc;
+  int64_t contact_capacity;  /* max contact candidates per node, 0 = automatic */
+} pdem_desc;
+
+/* State of the contact-search cadence controller (the reference's
+ * d_contNeighUpdateInterval / Counter / SearchRadius, demModel.cpp:1808-1955). */
+typedef struct pdem_contact_state {
+  int64_t interval;
+  int64_t counter;
+  double search_radius;
+  double max_velocity;
+  double max_contact_r;
+  int64_t n_rebuilds;
+  int64_t overflow;          /* nonzero: a node had more candidates than capacity */
+  int64_t capacity;
+} pdem_contact_state;
+
+typedef struct pdem_ctx pdem_ctx;
+
+/* Build the device model: upload state, build the PD neighbor list on device
+ * (replaces DEMModel::updatePeridynamicNeighborlist, demModel.cpp:1659-1694),
+ * create the bond-state bits (geometry::Fracture ctor, fracture.cpp:20-49),
+ * compute the weighted volume m_x (material::computeStateMx,
+ * materialUtil.cpp:211-245) and run the first contact search (the
+ * updateContactNeighborlist call inside DEMModel::init, demModel.cpp:179-187). */
+int pdem_create(const pdem_desc *desc, pdem_ctx **out);
+void pdem_destroy(pdem_ctx *ctx);
+const char *pdem_last_error(const pdem_ctx *ctx);  /* ctx may be NULL: last create error */
+
+/* The pre-loop part of DEMModel::integrate (demModel.cpp:276-283):
+ * computeExternalDisplacementBC() then computeForces(). */
+int pdem_integrate_begin(pdem_ctx *ctx);
+
+/* nsteps x DEMModel::integrateCD (demModel.cpp:349-390): node update, n++,
+ * t += dt, computeExternalDisplacementBC(), computeForces(). */
+int pdem_integrate_cd(pdem_ctx *ctx, int64_t nsteps);
+
+/* DEMModel::computeForces (demModel.cpp:462-600). */
+int pdem_compute_forces(pdem_ctx *ctx);
+/* DEMModel::computePeridynamicForces (demModel.cpp:602-789); overwrites f. */
+int pdem_compute_peridynamic_forces(pdem_ctx *ctx);
+/* DEMModel::updateContactNeighborlist (demModel.cpp:1696-1806), including the
+ * cadence decision of updateContactNeighborSearchParameters (:1808-1955). */
+int pdem_update_contact_neighborlist(pdem_ctx *ctx);
+/* DEMModel::computeContactForces (demModel.cpp:820-1066); adds to f. */
+int pdem_compute_contact_forces(pdem_ctx *ctx);
+/* DEMModel::computeExternalForces (demModel.cpp:791-812); adds to f. */
+int pdem_compute_external_forces(pdem_ctx *ctx);
+/* DEMModel::computeExternalDisplacementBC (demModel.cpp:814-818). */
+int pdem_compute_external_displacement_bc(pdem_ctx *ctx);
+
+/* Capture one integrateCD step in a HIP graph and replay it nsteps times.
+ * Same results as pdem_integrate_cd; fewer launches. */
+int pdem_integrate_cd_graph(pdem_ctx *ctx, int64_t nsteps);
+
+int pdem_synchronize(pdem_ctx *ctx);
+
+int pdem_get_field(pdem_ctx *ctx, int32_t field, void *host);
+int pdem_set_field(pdem_ctx *ctx, int32_t field, const void *host);
+int pdem_get_time(pdem_ctx *ctx, int64_t *n, double *t);
+int pdem_get_neighbor_count(pdem_ctx *ctx, int32_t kind, int64_t *nnz);
+/* rowptr [n+1], cols [nnz]; candidate lists are in search order */
+int pdem_get_neighbors(pdem_ctx *ctx, int32_t kind, int64_t *rowptr, int64_t *cols);
+int pdem_get_contact_state(pdem_ctx *ctx, pdem_contact_state *st);
+
+/* DEMModel::ppCompressiveTest (demModel.cpp:2202-2250) quantities:
+ * penetration (x - xRef)[f_dir] of the wall's first node and the reaction
+ * sum_i f_i[f_dir] V_i over the wall's nodes; f_dir is 0-based. */
+int pdem_pp_compressive_test(pdem_ctx *ctx, int32_t wall_id, int32_t f_dir,
+                             double *penetration, double *reaction);
+/* number of broken bonds and of all bonds (damage curve) */
+int pdem_broken_bonds(pdem_ctx *ctx, int64_t *broken, int64_t *total);
+
+/* Timing of the last pdem_integrate_cd / _graph call's kernels: for the
+ * dominant (PD bond) kernel, average duration in ms measured with HIP events
+ * on the stream it was launched on. */
+int pdem_profile_enable(pdem_ctx *ctx, int32_t on);
+int pdem_profile_get(pdem_ctx *ctx, double *pd_kernel_ms_avg, int64_t *pd_kernel_launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDEM_H */

@@ -1,0 +1,367 @@
+"""ctypes mirror of include/pdem.h, the C-ABI of the GPU engine.
+
+The structures below must stay byte-identical to the C declarations; the
+tests check their sizes against the values the library reports.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+MAT_RNP = 0
+MAT_PDSTATE = 1
+
+INF_CONST = 0
+INF_LINEAR = 1
+INF_GAUSSIAN = 2
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+UFN_CONSTANT, UFN_LINEAR, UFN_QUADRATIC, UFN_SIN, UFN_ROTATION, UFN_NONE = range(6)
+FFN_CONSTANT, FFN_LINEAR, FFN_LINEAR_STEP, FFN_LINEAR_SLOW_FAST, FFN_SIN = range(5)
+
+FIELD_X, FIELD_U, FIELD_V, FIELD_F, FIELD_XREF, FIELD_VOL, FIELD_Z, FIELD_THETA, \
+    FIELD_MX, FIELD_VMAG, FIELD_BONDS = range(11)
+NEIGH_PD = 0
+NEIGH_CONTACT = 1
+
+
+class Particle(C.Structure):
+    _fields_ = [
+        ("glob_start", C.c_int64), ("glob_end", C.c_int64), ("center_node", C.c_int64),
+        ("zone", C.c_int32), ("type_index", C.c_int32),
+        ("all_dofs_constrained", C.c_int32), ("compute_force", C.c_int32),
+        ("material_type", C.c_int32), ("irreversible", C.c_int32),
+        ("density", C.c_double), ("horizon", C.c_double), ("mesh_size", C.c_double),
+        ("bounding_radius", C.c_double), ("Kn", C.c_double), ("Rc", C.c_double),
+        ("mat", C.c_double * 6),
+    ]
+
+
+class ContactPair(C.Structure):
+    _fields_ = [("contact_r", C.c_double), ("Kn", C.c_double), ("mu", C.c_double),
+                ("betan", C.c_double), ("kappa", C.c_double)]
+
+
+class BcSet(C.Structure):
+    _fields_ = [("time_fn", C.c_int32), ("dir_mask", C.c_int32), ("is_zero", C.c_int32),
+                ("reserved", C.c_int32), ("time_params", C.c_double * 4),
+                ("n_entries", C.c_int64), ("nodes", C.POINTER(C.c_int64)),
+                ("amp", C.POINTER(C.c_double))]
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("precision", C.c_int32), ("multi_particle", C.c_int32),
+        ("influence_fn_type", C.c_int32), ("influence_fn_params", C.c_double * 2),
+        ("dt", C.c_double), ("n_nodes", C.c_int64),
+        ("x_ref", C.POINTER(C.c_double)), ("u", C.POINTER(C.c_double)),
+        ("v", C.POINTER(C.c_double)), ("vol", C.POINTER(C.c_double)),
+        ("fix", C.POINTER(C.c_uint8)), ("pt_id", C.POINTER(C.c_int64)),
+        ("n_particles", C.c_int32), ("n_zones", C.c_int32),
+        ("particles", C.POINTER(Particle)), ("contact", C.POINTER(ContactPair)),
+        ("gravity", C.c_double * 3), ("search_factor", C.c_double),
+        ("neigh_update_interval", C.c_int64),
+        ("n_ubc", C.c_int32), ("n_fbc", C.c_int32),
+        ("ubc", C.POINTER(BcSet)), ("fbc", C.POINTER(BcSet)),
+        ("contact_capacity", C.c_int64),
+    ]
+
+
+class ContactState(C.Structure):
+    _fields_ = [("interval", C.c_int64), ("counter", C.c_int64),
+                ("search_radius", C.c_double), ("max_velocity", C.c_double),
+                ("max_contact_r", C.c_double), ("n_rebuilds", C.c_int64),
+                ("overflow", C.c_int64), ("capacity", C.c_int64)]
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class DescHolder:
+    """Keeps the numpy buffers a Desc points into alive."""
+
+    def __init__(self, spec, precision: int = PREC_FP64, contact_capacity: int = 0):
+        self._keep = []
+        s = spec
+
+        def keep(a, dtype):
+            a = np.ascontiguousarray(a, dtype=dtype)
+            self._keep.append(a)
+            return a
+
+        n = s.n_nodes
+        xr = keep(s.x_ref, np.float64)
+        u = keep(s.u, np.float64)
+        v = keep(s.v, np.float64)
+        vol = keep(s.vol, np.float64)
+        fix = keep(s.fix, np.uint8)
+        pt = keep(s.pt_id, np.int64)
+        parts = (Particle * len(s.particles))()
+        for k, p in enumerate(s.particles):
+            for name, _ in Particle._fields_:
+                if name == "mat":
+                    for q in range(6):
+                        parts[k].mat[q] = float(p["mat"][q]) if q < len(p["mat"]) else 0.0
+                else:
+                    setattr(parts[k], name, p[name])
+        self._keep.append(parts)
+        nz = s.n_zones
+        ct = (ContactPair * max(1, nz * nz))()
+        for a in range(nz):
+            for b in range(nz):
+                c = s.contact[a][b]
+                ct[a * nz + b] = ContactPair(c["contact_r"], c["Kn"], c["mu"], c["betan"],
+                                             c["kappa"])
+        self._keep.append(ct)
+
+        def bcs(sets):
+            arr = (BcSet * max(1, len(sets)))()
+            for k, b in enumerate(sets):
+                nodes = keep(b["nodes"], np.int64)
+                amp = keep(b["amp"], np.float64)
+                tp = list(b["time_params"]) + [0.0] * (4 - len(b["time_params"]))
+                arr[k] = BcSet(b["time_fn"], b["dir_mask"], int(b.get("is_zero", 0)), 0,
+                               (C.c_double * 4)(*tp[:4]), len(nodes), _ptr(nodes, C.c_int64),
+                               _ptr(amp, C.c_double))
+            self._keep.append(arr)
+            return arr
+
+        ubc = bcs(s.ubc)
+        fbc = bcs(s.fbc)
+        d = Desc()
+        d.dim = s.dim
+        d.precision = precision
+        d.multi_particle = int(s.multi_particle)
+        d.influence_fn_type = s.influence_fn_type
+        d.influence_fn_params[0] = s.influence_fn_params[0]
+        d.influence_fn_params[1] = s.influence_fn_params[1]
+        d.dt = s.dt
+        d.n_nodes = n
+        d.x_ref = _ptr(xr, C.c_double)
+        d.u = _ptr(u, C.c_double)
+        d.v = _ptr(v, C.c_double)
+        d.vol = _ptr(vol, C.c_double)
+        d.fix = _ptr(fix, C.c_uint8)
+        d.pt_id = _ptr(pt, C.c_int64)
+        d.n_particles = len(s.particles)
+        d.n_zones = nz
+        d.particles = C.cast(parts, C.POINTER(Particle))
+        d.contact = C.cast(ct, C.POINTER(ContactPair))
+        for q in range(3):
+            d.gravity[q] = s.gravity[q]
+        d.search_factor = s.search_factor
+        d.neigh_update_interval = s.neigh_update_interval
+        d.n_ubc = len(s.ubc)
+        d.n_fbc = len(s.fbc)
+        d.ubc = C.cast(ubc, C.POINTER(BcSet))
+        d.fbc = C.cast(fbc, C.POINTER(BcSet))
+        d.contact_capacity = contact_capacity
+        self.desc = d
+
+
+def declare(lib: C.CDLL, prefix: str, extra_create_int: bool = False):
+    """Declare argtypes/restype for the pdem_* family (or the oracle_* twin)."""
+    vp = C.c_void_p
+
+    def f(name, res, args):
+        fn = getattr(lib, prefix + name)
+        fn.restype = res
+        fn.argtypes = args
+        return fn
+
+    if extra_create_int:
+        f("create", C.c_int, [C.POINTER(Desc), C.c_int32, C.POINTER(vp)])
+    else:
+        f("create", C.c_int, [C.POINTER(Desc), C.POINTER(vp)])
+    f("destroy", None, [vp])
+    f("last_error", C.c_char_p, [vp])
+    for name in ("integrate_begin", "compute_forces", "compute_peridynamic_forces"):
+        f(name, C.c_int, [vp])
+    f("integrate_cd", C.c_int, [vp, C.c_int64])
+    f("get_field", C.c_int, [vp, C.c_int32, vp])
+    f("set_field", C.c_int, [vp, C.c_int32, vp])
+    f("get_time", C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_double)])
+    f("get_neighbor_count", C.c_int, [vp, C.c_int32, C.POINTER(C.c_int64)])
+    f("get_neighbors", C.c_int, [vp, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
+    f("get_contact_state", C.c_int, [vp, C.POINTER(ContactState)])
+    f("pp_compressive_test", C.c_int,
+      [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)])
+    f("broken_bonds", C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
+    return lib
+
+
+class Engine:
+    """Thin object wrapper over one context of a pdem_*-shaped library."""
+
+    def __init__(self, lib, prefix, spec, precision=PREC_FP64, nthreads=None,
+                 contact_capacity=0):
+        self.lib = lib
+        self.p = prefix
+        self.spec = spec
+        self.holder = DescHolder(spec, precision, contact_capacity)
+        self.ctx = C.c_void_p()
+        if nthreads is not None:
+            rc = self._f("create")(C.byref(self.holder.desc), nthreads, C.byref(self.ctx))
+        else:
+            rc = self._f("create")(C.byref(self.holder.desc), C.byref(self.ctx))
+        if rc != 0:
+            raise RuntimeError(f"{prefix}create failed: {self._f('last_error')(None).decode()}")
+
+    def _f(self, name):
+        return getattr(self.lib, self.p + name)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._f("last_error")(self.ctx).decode()
+            raise RuntimeError(f"{self.p}{what} failed: {msg}")
+
+    def close(self):
+        if self.ctx:
+            self._f("destroy")(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ stepping
+    def integrate_begin(self):
+        self._check(self._f("integrate_begin")(self.ctx), "integrate_begin")
+
+    def integrate_cd(self, nsteps=1):
+        self._check(self._f("integrate_cd")(self.ctx, nsteps), "integrate_cd")
+
+    def compute_forces(self):
+        self._check(self._f("compute_forces")(self.ctx), "compute_forces")
+
+    def compute_peridynamic_forces(self):
+        self._check(self._f("compute_peridynamic_forces")(self.ctx), "compute_peridynamic_forces")
+
+    # ------------------------------------------------------------ queries
+    def field(self, fid):
+        n = self.spec.n_nodes
+        if fid in (FIELD_X, FIELD_U, FIELD_V, FIELD_F, FIELD_XREF):
+            a = np.zeros((n, 3), np.float64)
+        elif fid == FIELD_Z:
+            a = np.zeros(n, np.float32)
+        elif fid == FIELD_BONDS:
+            a = np.zeros(self.neighbor_count(NEIGH_PD), np.uint8)
+        else:
+            a = np.zeros(n, np.float64)
+        self._check(self._f("get_field")(self.ctx, fid, a.ctypes.data_as(C.c_void_p)), "get_field")
+        return a
+
+    def set_field(self, fid, a):
+        a = np.ascontiguousarray(a)
+        self._check(self._f("set_field")(self.ctx, fid, a.ctypes.data_as(C.c_void_p)), "set_field")
+
+    def time(self):
+        n = C.c_int64()
+        t = C.c_double()
+        self._check(self._f("get_time")(self.ctx, C.byref(n), C.byref(t)), "get_time")
+        return n.value, t.value
+
+    def neighbor_count(self, kind):
+        m = C.c_int64()
+        self._check(self._f("get_neighbor_count")(self.ctx, kind, C.byref(m)), "get_neighbor_count")
+        return m.value
+
+    def neighbors(self, kind):
+        m = self.neighbor_count(kind)
+        rp = np.zeros(self.spec.n_nodes + 1, np.int64)
+        cols = np.zeros(max(1, m), np.int64)
+        self._check(self._f("get_neighbors")(self.ctx, kind, rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                             cols.ctypes.data_as(C.POINTER(C.c_int64))),
+                    "get_neighbors")
+        return rp, cols[:m]
+
+    def contact_state(self):
+        st = ContactState()
+        self._check(self._f("get_contact_state")(self.ctx, C.byref(st)), "get_contact_state")
+        return {k: getattr(st, k) for k, _ in ContactState._fields_}
+
+    def pp_compressive_test(self, wall_id, f_dir):
+        pen = C.c_double()
+        reac = C.c_double()
+        self._check(self._f("pp_compressive_test")(self.ctx, wall_id, f_dir, C.byref(pen),
+                                                   C.byref(reac)), "pp_compressive_test")
+        return pen.value, reac.value
+
+    def broken_bonds(self):
+        b = C.c_int64()
+        t = C.c_int64()
+        self._check(self._f("broken_bonds")(self.ctx, C.byref(b), C.byref(t)), "broken_bonds")
+        return b.value, t.value
+
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libpdem_hip.so")
+
+_engine_lib = None
+
+
+def engine_library():
+    """Load the HIP engine library; raises if it has not been built."""
+    global _engine_lib
+    if _engine_lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP engine library missing: {LIB_PATH} (run __graft_entry__.build())")
+        lib = C.CDLL(LIB_PATH)
+        declare(lib, "pdem_")
+        lib.pdem_integrate_cd_graph.restype = C.c_int
+        lib.pdem_integrate_cd_graph.argtypes = [C.c_void_p, C.c_int64]
+        lib.pdem_synchronize.restype = C.c_int
+        lib.pdem_synchronize.argtypes = [C.c_void_p]
+        for nm in ("update_contact_neighborlist", "compute_contact_forces",
+                   "compute_external_forces", "compute_external_displacement_bc"):
+            fn = getattr(lib, "pdem_" + nm)
+            fn.restype = C.c_int
+            fn.argtypes = [C.c_void_p]
+        lib.pdem_profile_enable.restype = C.c_int
+        lib.pdem_profile_enable.argtypes = [C.c_void_p, C.c_int32]
+        lib.pdem_profile_get.restype = C.c_int
+        lib.pdem_profile_get.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        _engine_lib = lib
+    return _engine_lib
+
+
+class GpuEngine(Engine):
+    """The MI355X engine: one pdem_ctx on the current HIP device."""
+
+    def __init__(self, spec, precision=PREC_FP64, contact_capacity=0):
+        super().__init__(engine_library(), "pdem_", spec, precision, None, contact_capacity)
+
+    def integrate_cd_graph(self, nsteps):
+        self._check(self.lib.pdem_integrate_cd_graph(self.ctx, nsteps), "integrate_cd_graph")
+
+    def synchronize(self):
+        self._check(self.lib.pdem_synchronize(self.ctx), "synchronize")
+
+    def profile_enable(self, on=True):
+        self._check(self.lib.pdem_profile_enable(self.ctx, int(on)), "profile_enable")
+
+    def profile_get(self):
+        ms = C.c_double()
+        n = C.c_int64()
+        self._check(self.lib.pdem_profile_get(self.ctx, C.byref(ms), C.byref(n)), "profile_get")
+        return ms.value, n.value
+
+    def update_contact_neighborlist(self):
+        self._check(self.lib.pdem_update_contact_neighborlist(self.ctx), "update_contact_neighborlist")
+
+    def compute_contact_forces(self):
+        self._check(self.lib.pdem_compute_contact_forces(self.ctx), "compute_contact_forces")
+
+    def compute_external_forces(self):
+        self._check(self.lib.pdem_compute_external_forces(self.ctx), "compute_external_forces")
+
+    def compute_external_displacement_bc(self):
+        self._check(self.lib.pdem_compute_external_displacement_bc(self.ctx),
+                    "compute_external_displacement_bc")

@@ -1,0 +1,217 @@
+"""The synthetic workloads: seeded generators for the five configurations.
+
+Every generator returns a ModelSpec; sizes are parameters so the tests can
+build reduced versions of the same decks.  Deck deviations from the
+configuration text (contact stiffness factor, box size) are stated in each
+generator's docstring and applied identically to the oracle and the GPU.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import capi
+from .geometry import ball_mesh, box_mesh, center_node, disk_mesh, hexagon_mesh, transform
+from .material import MaterialDeck
+from .model import BcDeck, ContactPairDeck, ModelBuilder, ZoneDeck
+
+
+def rnp_material(E=2.0e9, nu=0.25, rho=1200.0, Gc=500.0, ratio=3.0, inf=capi.INF_CONST):
+    K = E / (3.0 * (1.0 - 2.0 * nu))
+    return MaterialDeck(type="RNPBond", E=E, K=K, Gc=Gc, density=rho, horizon_mesh_ratio=ratio,
+                        influence_fn_type=inf)
+
+
+def pdstate_material(K=2.16e7, G=1.296e7, rho=1200.0, Gc=50.0, ratio=3.0, inf=capi.INF_LINEAR):
+    return MaterialDeck(type="PDState", K=K, G=G, Gc=Gc, density=rho, horizon_mesh_ratio=ratio,
+                        influence_fn_type=inf)
+
+
+def _kn(mat: MaterialDeck, horizon: float) -> float:
+    """Deck Kn = 18 K / (pi delta^5), as the reference's deck generators write it
+    (test/test_data/peridem/twop_circ/inp/problem_setup.py:195-197)."""
+    return 18.0 * mat.K / (math.pi * horizon ** 5)
+
+
+def two_particle(R=1.0e-3, h=1.0e-4, gap=1.0e-4, speed=0.1, dt=1.0e-7, n_steps=20000,
+                 kn_factor=1.0e-3, precision_note=""):
+    """Config 0: two R = 1 mm circles, h = 0.1 mm, RNP, particle 2 at 0.1 m/s
+    toward particle 1 across a 0.1 mm gap; no damping, no friction.
+
+    Deviation: Kn_Factor = 1e-3 on the deck Kn = 18K/(pi delta^5).  With the
+    full value the explicit step is unstable at dt = 1e-7 (per-pair contact
+    stiffness Kn V / rho gives dt_crit ~ 1e-8 s)."""
+    mat = rnp_material()
+    zones = [ZoneDeck(mat)]
+    horizon = 3.0 * h
+    cp = ContactPairDeck(contact_radius_factor=0.95, Kn=_kn(mat, horizon), Kn_factor=kn_factor,
+                         damping_on=False, friction_on=False)
+    b = ModelBuilder(2, dt, zones, {(0, 0): cp}, search_factor=5.0, neigh_update_interval=40)
+    ref, vol = disk_mesh(R, h)
+    cl = center_node(ref, (0, 0, 0), R)
+    b.add_particle(0, transform(ref, (0.0, 0.0, 0.0), 0.0, 1.0), vol, cl, R)
+    b.add_particle(0, transform(ref, (0.0, 2 * R + gap, 0.0), math.pi, 1.0), vol, cl, R)
+    b.ic.append(([1], (0.0, -speed, 0.0)))
+    return b.build(name="two_particle_rnp_2d", n_steps=n_steps)
+
+
+def compressive_2d(n_particles=500, R_lo=0.8e-3, R_hi=1.2e-3, nodes_per_particle=400,
+                   box=None, seed=0, dt=5.0e-8, n_steps=50000, wall_speed=0.1,
+                   kn_factor=1.0e-3, epsilon=0.7, mu=0.5, interval=40):
+    """Config 1: 2D compressive test, 50/50 circles and hexagons, R ~ U[0.8, 1.2] mm,
+    random orientation, seeded jittered grid; ~400 nodes per particle; RNP
+    (E = 2 GPa, nu = 0.25, rho = 1200, Gc = 500), horizon 3h; contact with
+    restitution 0.7 (Beta_n_Factor 1) and friction 0.5; the top wall moves down
+    at 0.1 m/s (linear Displacement_BC); compressive_test observables on it.
+
+    Deviations: the box is enlarged from 40 x 40 mm to the grid the particles
+    need without overlap (the stated box would be 90% full); Kn_Factor = 1e-3
+    for explicit stability at dt = 5e-8 (see two_particle)."""
+    rng = np.random.default_rng(seed)
+    mat = rnp_material()
+    wall_mat = rnp_material()
+    zones = [ZoneDeck(mat), ZoneDeck(mat), ZoneDeck(wall_mat, is_wall=True, all_dofs_constrained=True),
+             ZoneDeck(wall_mat, is_wall=True)]
+    # reference meshes at R = 1 mm with ~nodes_per_particle nodes
+    h_ref = math.sqrt(math.pi / (nodes_per_particle * math.sqrt(3) / 2)) * 1.0e-3
+    circ, cvol = disk_mesh(1.0e-3, h_ref)
+    hexa, hvol = hexagon_mesh(1.0e-3, h_ref)
+    c_cl = center_node(circ, (0, 0, 0), 1.0e-3)
+    h_cl = center_node(hexa, (0, 0, 0), 1.0e-3)
+    pitch = 2.0 * R_hi * 1.08
+    ncol = int(math.ceil(math.sqrt(n_particles)))
+    nrow = int(math.ceil(n_particles / ncol))
+    W = ncol * pitch
+    H = nrow * pitch
+    if box is not None:
+        W, H = max(W, box[0]), max(H, box[1])
+    h_wall = 0.8 * h_ref * R_lo / 1.0e-3
+    t_wall = 3.5 * h_wall
+    delta_ref = 3.0 * h_ref
+    cp = ContactPairDeck(contact_radius_factor=0.95, Kn=_kn(mat, delta_ref * R_lo / 1e-3),
+                         Kn_factor=kn_factor, epsilon=epsilon, beta_n_factor=1.0,
+                         friction_coeff=mu, damping_on=True, friction_on=True)
+    contact = {(a, b): cp for a in range(4) for b in range(a, 4)}
+    b = ModelBuilder(2, dt, zones, contact, search_factor=5.0, neigh_update_interval=interval,
+                     gravity=(0.0, -10.0, 0.0))
+    k = 0
+    for r in range(nrow):
+        for c in range(ncol):
+            if k >= n_particles:
+                break
+            rad = rng.uniform(R_lo, R_hi)
+            jit = (pitch / 2 - rad) * 0.9
+            cx = (c + 0.5) * pitch + rng.uniform(-jit, jit)
+            cy = (r + 0.5) * pitch + rng.uniform(-jit, jit)
+            th = rng.uniform(0, 2 * math.pi)
+            s = rad / 1.0e-3
+            if k % 2 == 0:
+                b.add_particle(0, transform(circ, (cx, cy, 0.0), th, s), cvol * s * s, c_cl, rad)
+            else:
+                b.add_particle(1, transform(hexa, (cx, cy, 0.0), th, s), hvol * s * s, h_cl, rad)
+            k += 1
+    gap = 0.5 * h_wall
+    # fixed container: bottom, left and right strips (one wall particle)
+    parts = []
+    for lo, hi in (((-t_wall, -t_wall - gap), (W + t_wall, -gap)),
+                   ((-t_wall - gap, 0.0), (-gap, H + 2 * pitch)),
+                   ((W + gap, 0.0), (W + gap + t_wall, H + 2 * pitch))):
+        p, vv = box_mesh(lo, hi, h_wall, 2)
+        parts.append((p, vv))
+    wx = np.concatenate([p for p, _ in parts])
+    wv = np.concatenate([v for _, v in parts])
+    fixed_id = b.add_particle(2, wx, wv, 0, 0.5 * math.hypot(W, H))
+    top, tv = box_mesh((0.0, H + gap), (W, H + gap + t_wall), h_wall, 2)
+    top_id = b.add_particle(3, top, tv, 0, 0.5 * W)
+    b.ubc.append(BcDeck(particles=[fixed_id], directions=[1, 2], time_fn=capi.UFN_CONSTANT,
+                        time_params=[0.0], is_zero=True))
+    b.ubc.append(BcDeck(particles=[top_id], directions=[2], time_fn=capi.UFN_LINEAR,
+                        time_params=[-wall_speed]))
+    spec = b.build(name=f"compressive_2d_n{n_particles}", n_steps=n_steps, wall_id=top_id, f_dir=1)
+    spec.info.update(box=(W, H), seed=seed)
+    return spec
+
+
+def packing_3d(n_spheres=2000, R_lo=0.9e-3, R_hi=1.1e-3, nodes_per_sphere=1000, L=30.0e-3,
+               seed=0, dt=2.0e-8, n_steps=10000, wall_speed=0.05, kn_factor=1.0e-2,
+               max_tries=200000):
+    """Config 2: 3D packing, state-based PD: spheres R ~ U[0.9, 1.1] mm placed by
+    seeded sequential random insertion in an L^3 cube, ~1000 nodes per sphere,
+    horizon 3h, gravity, a fixed bottom wall and a top wall moving down at
+    0.05 m/s.  Material (not given by the configuration): the reference's own
+    PDState deck values K = 2.16e7, G = 1.296e7, Gc = 50, rho = 1200, linear
+    influence function; dt chosen for stability with Kn_Factor 1e-2."""
+    rng = np.random.default_rng(seed)
+    mat = pdstate_material()
+    zones = [ZoneDeck(mat), ZoneDeck(mat, is_wall=True, all_dofs_constrained=True),
+             ZoneDeck(mat, is_wall=True)]
+    h_ref = 1.0e-3 * (4.0 / 3.0 * math.pi / nodes_per_sphere) ** (1.0 / 3.0)
+    ball, bvol = ball_mesh(1.0e-3, h_ref)
+    b_cl = center_node(ball, (0, 0, 0), 1.0e-3)
+    delta = 3.0 * h_ref * R_lo / 1e-3
+    cp = ContactPairDeck(contact_radius_factor=0.95, Kn=_kn(mat, delta), Kn_factor=kn_factor,
+                         epsilon=0.7, beta_n_factor=1.0, friction_coeff=0.5)
+    contact = {(a, b): cp for a in range(3) for b in range(a, 3)}
+    b = ModelBuilder(3, dt, zones, contact, search_factor=5.0, neigh_update_interval=40,
+                     gravity=(0.0, 0.0, -10.0))
+    centers, radii = [], []
+    margin = R_hi * 1.02
+    tries = 0
+    from scipy.spatial import cKDTree
+    grid = {}
+    cell = 2.0 * R_hi * 1.05
+    while len(centers) < n_spheres and tries < max_tries:
+        tries += 1
+        r = rng.uniform(R_lo, R_hi)
+        c = rng.uniform(margin, L - margin, size=3)
+        key = tuple((c // cell).astype(int))
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for j in grid.get((key[0] + dx, key[1] + dy, key[2] + dz), ()):
+                        if np.linalg.norm(c - centers[j]) < (r + radii[j]) * 1.04:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if ok:
+            grid.setdefault(key, []).append(len(centers))
+            centers.append(c)
+            radii.append(r)
+    if len(centers) < n_spheres:
+        raise RuntimeError(f"could only place {len(centers)} spheres")
+    for c, r in zip(centers, radii):
+        s = r / 1.0e-3
+        th = rng.uniform(0, 2 * math.pi)
+        b.add_particle(0, transform(ball, c, th, s), bvol * s ** 3, b_cl, r)
+    h_wall = 2.0 * h_ref
+    t_wall = 1.5 * h_wall
+    bot, bv = box_mesh((0.0, 0.0, -t_wall - 0.5 * h_wall), (L, L, -0.5 * h_wall), h_wall, 3)
+    top, tv = box_mesh((0.0, 0.0, L + 0.5 * h_wall), (L, L, L + 0.5 * h_wall + t_wall), h_wall, 3)
+    bid = b.add_particle(1, bot, bv, 0, L)
+    tid = b.add_particle(2, top, tv, 0, L)
+    b.ubc.append(BcDeck(particles=[bid], directions=[1, 2, 3], time_fn=capi.UFN_CONSTANT,
+                        time_params=[0.0], is_zero=True))
+    b.ubc.append(BcDeck(particles=[tid], directions=[3], time_fn=capi.UFN_LINEAR,
+                        time_params=[-wall_speed]))
+    spec = b.build(name=f"packing_3d_n{n_spheres}", n_steps=n_steps, wall_id=tid, f_dir=2)
+    spec.info.update(seed=seed)
+    return spec
+
+
+def single_particle_3d(R=1.0e-3, nodes=1000, state=True, stretch=0.0):
+    """One sphere, no contact: used by the closed-form tests."""
+    mat = pdstate_material() if state else rnp_material()
+    zones = [ZoneDeck(mat)]
+    h = 1.0e-3 * (4.0 / 3.0 * math.pi / nodes) ** (1.0 / 3.0)
+    ball, bvol = ball_mesh(R, h)
+    cl = center_node(ball, (0, 0, 0), R)
+    b = ModelBuilder(3, 1e-8, zones, {}, multi_particle=False)
+    b.add_particle(0, ball, bvol, cl, R)
+    return b.build(name="single_3d")

@@ -1,0 +1,843 @@
+// pdem_engine.hip -- host side of the MI355X PeriDEM step engine and the
+// C-ABI declared in include/pdem.h.
+//
+// One context = one model on the current HIP device, all state resident in
+// HBM.  Every step runs as a fixed sequence of kernels on one stream with no
+// host decision inside it: the contact-search cadence controller and the
+// simulation clock live in device memory (struct Ctrl), so a step can be
+// captured once in a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pdem.h"
+#include "pdem_kernels.hpp"
+
+using namespace pdem;
+
+namespace {
+thread_local std::string g_create_err;
+
+struct HipError {
+  std::string msg;
+};
+
+#define HIPCHK(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      throw HipError{std::string(#call) + ": " + hipGetErrorString(e_)};              \
+  } while (0)
+
+constexpr int TPB = 256;
+
+inline unsigned blocks(int64_t n, int tpb = TPB) { return (unsigned)((n + tpb - 1) / tpb); }
+
+}  // namespace
+
+struct pdem_ctx {
+  int dim = 2, precision = 0, multi = 0;
+  int64_t N = 0;
+  int np = 0, nz = 0;
+  hipStream_t stream = nullptr;
+  DS ds{};
+  std::vector<void*> allocs;
+  std::vector<pdem_particle> hp;
+  std::vector<DevBc> ubc, fbc;
+  std::vector<int> ubc_zero;
+  std::vector<char> zero_done;
+  int64_t nslices = 0, nnz_padded = 0, nbitwords = 0, nnz = 0;
+  std::vector<int64_t> h_sptr, h_bptr;
+  std::vector<int> h_deg;
+  bool state_any = false, rnp_any = false;
+  int64_t cap = 0;
+  bool graph_ready = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool profiling = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
+  double prof_ms = 0.;
+  int64_t prof_n = 0;
+  std::string err;
+
+  template <class T>
+  T* dalloc(size_t n, bool zero = true) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc(&p, n * sizeof(T)));
+    allocs.push_back(p);
+    if (zero) HIPCHK(hipMemset(p, 0, n * sizeof(T)));
+    return (T*)p;
+  }
+  template <class T>
+  T* upload(const T* h, size_t n) {
+    T* d = dalloc<T>(n, false);
+    if (n) HIPCHK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+  }
+  ~pdem_ctx() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto& e : ev_used) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    for (auto& e : ev_free) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : allocs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+template <int DIM>
+struct Launch {
+  static void pd_forces(pdem_ctx* c) {
+    hipStream_t st = c->stream;
+    const unsigned nb = blocks(c->N);
+    std::pair<hipEvent_t, hipEvent_t> ev{};
+    if (c->profiling) {
+      if (c->ev_free.empty()) {
+        hipEvent_t a, b;
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        c->ev_free.push_back({a, b});
+      }
+      ev = c->ev_free.back();
+      c->ev_free.pop_back();
+    }
+    if (c->ds.state_global) hipLaunchKernelGGL(k_pd_dilation<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+    if (c->profiling) HIPCHK(hipEventRecord(ev.first, st));
+    if (c->rnp_any) hipLaunchKernelGGL(k_pd_rnp<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+    if (c->state_any) hipLaunchKernelGGL(k_pd_state<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+    if (!c->rnp_any && !c->state_any) hipLaunchKernelGGL(k_pd_rnp<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+    if (c->profiling) {
+      HIPCHK(hipEventRecord(ev.second, st));
+      c->ev_used.push_back(ev);
+    }
+  }
+  static void contact_search(pdem_ctx* c) {
+    hipStream_t st = c->stream;
+    const unsigned nb = blocks(c->N);
+    hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(64), 0, st, c->ds);
+    hipLaunchKernelGGL(k_cell_clear, dim3(blocks(c->ds.ncells + 1)), dim3(TPB), 0, st, c->ds);
+    hipLaunchKernelGGL(k_cell_count<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, st, c->ds);
+    hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(TPB), 0, st, c->ds);
+    hipLaunchKernelGGL(k_cell_sort, dim3(blocks(c->ds.ncells)), dim3(TPB), 0, st, c->ds);
+    hipLaunchKernelGGL(k_contact_clear, dim3(nb), dim3(TPB), 0, st, c->ds);
+    hipLaunchKernelGGL(k_contact_query<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+  }
+  static void contact_forces(pdem_ctx* c, bool apply_damping) {
+    hipStream_t st = c->stream;
+    const unsigned nb = blocks(c->N);
+    hipLaunchKernelGGL(k_contact_force<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+    if (c->ds.n_nonwall > 0)
+      hipLaunchKernelGGL(k_damping<DIM>, dim3(c->ds.n_nonwall), dim3(64), 0, st, c->ds);
+    if (apply_damping) hipLaunchKernelGGL(k_apply_damping<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+  }
+  static void external_forces(pdem_ctx* c, bool with_damping) {
+    hipStream_t st = c->stream;
+    if (c->ds.has_gravity || with_damping)
+      hipLaunchKernelGGL(k_finalize<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, st, c->ds,
+                         with_damping ? 1 : 0);
+    for (auto& b : c->fbc)
+      if (b.n > 0) hipLaunchKernelGGL(k_fbc<DIM>, dim3(blocks(b.n)), dim3(TPB), 0, st, c->ds, b);
+  }
+  static void displacement_bc(pdem_ctx* c) {
+    for (size_t s = 0; s < c->ubc.size(); s++) {
+      if (c->ubc_zero[s]) {
+        if (c->zero_done[s]) continue;
+        c->zero_done[s] = 1;
+      }
+      const auto& b = c->ubc[s];
+      if (b.n > 0) hipLaunchKernelGGL(k_ubc<DIM>, dim3(blocks(b.n)), dim3(TPB), 0, c->stream, c->ds, b);
+    }
+  }
+  static void compute_forces(pdem_ctx* c) {
+    pd_forces(c);
+    if (c->multi) {
+      contact_search(c);
+      contact_forces(c, false);
+    }
+    external_forces(c, c->multi != 0);
+  }
+  static void integrate_step(pdem_ctx* c) {
+    hipLaunchKernelGGL(k_integrate<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds);
+    displacement_bc(c);
+    compute_forces(c);
+  }
+};
+
+template <class F2, class F3>
+void dispatch(pdem_ctx* c, F2 f2, F3 f3) {
+  if (c->dim == 2)
+    f2();
+  else
+    f3();
+}
+
+int set_err(pdem_ctx* c, const std::string& m) {
+  if (c) c->err = m;
+  g_create_err = m;
+  return 1;
+}
+
+void check_launch(pdem_ctx* c) { HIPCHK(hipGetLastError()); }
+
+Ctrl read_ctrl(pdem_ctx* c) {
+  Ctrl h;
+  HIPCHK(hipMemcpyAsync(&h, c->ds.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return h;
+}
+
+void check_overflow(pdem_ctx* c) {
+  Ctrl h = read_ctrl(c);
+  if (h.overflow)
+    throw HipError{"contact candidate capacity exceeded (" + std::to_string(c->cap) +
+                   " per node); create the model with a larger contact_capacity"};
+}
+
+void build(pdem_ctx* c, const pdem_desc* d) {
+  const int64_t N = d->n_nodes;
+  const int dim = d->dim;
+  c->N = N;
+  c->dim = dim;
+  c->precision = d->precision;
+  c->multi = d->multi_particle;
+  c->np = d->n_particles;
+  c->nz = d->n_zones;
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  DS& s = c->ds;
+  s.N = N;
+  s.np = c->np;
+  s.nz = c->nz;
+  // ---- node state, AoS host -> SoA device
+  std::vector<double> tmp(N);
+  auto split = [&](const double* aos, double** out, bool add_ref) {
+    for (int k = 0; k < dim; k++) {
+      for (int64_t i = 0; i < N; i++) tmp[i] = aos ? aos[3 * i + k] : 0.;
+      if (add_ref)
+        for (int64_t i = 0; i < N; i++) tmp[i] = d->x_ref[3 * i + k] + tmp[i];
+      out[k] = c->upload(tmp.data(), N);
+    }
+    for (int k = dim; k < 3; k++) out[k] = nullptr;
+  };
+  split(d->x_ref, s.xr, false);
+  split(d->u, s.u, false);
+  split(d->v, s.v, false);
+  split(d->u, s.x, true);
+  for (int k = 0; k < dim; k++) s.f[k] = c->dalloc<double>(N);
+  for (int k = dim; k < 3; k++) s.f[k] = nullptr;
+  s.vol = c->upload(d->vol, N);
+  s.mx = c->dalloc<double>(N);
+  s.theta = c->dalloc<double>(N);
+  s.vmag = c->dalloc<double>(N);
+  s.Z = c->dalloc<float>(N);
+  s.fix = c->upload(d->fix, N);
+  std::vector<int> pt32(N);
+  for (int64_t i = 0; i < N; i++) pt32[i] = (int)d->pt_id[i];
+  s.ptid = c->upload(pt32.data(), N);
+  // ---- particles
+  c->hp.assign(d->particles, d->particles + c->np);
+  std::vector<DevParticle> dp(c->np);
+  std::vector<int> nonwall;
+  for (int p = 0; p < c->np; p++) {
+    const auto& q = c->hp[p];
+    DevParticle& o = dp[p];
+    o.glob_start = q.glob_start;
+    o.glob_end = q.glob_end;
+    o.center = q.center_node;
+    o.zone = q.zone;
+    o.type_index = q.type_index;
+    o.compute_force = q.compute_force;
+    o.material_type = q.material_type;
+    o.irreversible = q.irreversible;
+    o.searches = (q.type_index != 1 && !q.all_dofs_constrained && q.compute_force) ? 1 : 0;
+    o.nonwall_index = -1;
+    if (q.type_index == 0) {
+      o.nonwall_index = (int)nonwall.size();
+      nonwall.push_back(p);
+    }
+    o.rho = q.density;
+    o.horizon = q.horizon;
+    o.h = q.mesh_size;
+    o.Rb = q.bounding_radius;
+    o.Kn = q.Kn;
+    o.Rc = q.Rc;
+    for (int k = 0; k < 6; k++) o.mat[k] = q.mat[k];
+    if (q.material_type == PDEM_MAT_PDSTATE) c->state_any = true;
+    if (q.material_type == PDEM_MAT_RNP) c->rnp_any = true;
+  }
+  s.P = c->upload(dp.data(), dp.size());
+  std::vector<DevContact> dc((size_t)c->nz * c->nz);
+  for (size_t k = 0; k < dc.size(); k++) {
+    const auto& q = d->contact[k];
+    dc[k] = DevContact{q.contact_r, q.Kn, q.mu, q.betan, q.kappa};
+  }
+  s.CT = c->upload(dc.data(), dc.size());
+  s.inf_type = d->influence_fn_type;
+  s.inf_p0 = d->influence_fn_params[0];
+  s.inf_p1 = d->influence_fn_params[1];
+  s.state_global = c->hp[0].material_type == PDEM_MAT_PDSTATE ? 1 : 0;
+  double glen = std::sqrt(d->gravity[0] * d->gravity[0] + d->gravity[1] * d->gravity[1] +
+                          (dim == 3 ? d->gravity[2] * d->gravity[2] : 0.));
+  s.has_gravity = glen > 1.0E-8 ? 1 : 0;
+  for (int k = 0; k < 3; k++) s.gravity[k] = k < dim ? d->gravity[k] : 0.;
+  // ---- BC sets
+  for (int k = 0; k < d->n_ubc; k++) {
+    const auto& b = d->ubc[k];
+    DevBc o{};
+    o.time_fn = b.time_fn;
+    o.dir_mask = b.dir_mask;
+    for (int q = 0; q < 4; q++) o.tp[q] = b.time_params[q];
+    o.n = b.n_entries;
+    o.nodes = c->upload(b.nodes, b.n_entries);
+    o.amp = c->upload(b.amp, b.n_entries);
+    c->ubc.push_back(o);
+    c->ubc_zero.push_back(b.is_zero ? 1 : 0);
+  }
+  c->zero_done.assign(c->ubc.size(), 0);
+  for (int k = 0; k < d->n_fbc; k++) {
+    const auto& b = d->fbc[k];
+    DevBc o{};
+    o.time_fn = b.time_fn;
+    o.dir_mask = b.dir_mask;
+    for (int q = 0; q < 4; q++) o.tp[q] = b.time_params[q];
+    o.n = b.n_entries;
+    o.nodes = c->upload(b.nodes, b.n_entries);
+    o.amp = c->upload(b.amp, b.n_entries);
+    c->fbc.push_back(o);
+  }
+  // ---- control block
+  Ctrl h{};
+  h.dt = d->dt;
+  h.s_factor = d->search_factor;
+  h.deck_interval = d->neigh_update_interval;
+  double maxR = 0.;
+  if (c->multi)
+    for (auto& q : dc)
+      if (maxR < q.contact_r) maxR = q.contact_r;
+  h.max_contact_r = maxR;
+  s.ctrl = c->upload(&h, 1);
+
+  // ---- PD neighbor list, sliced ELL built on device
+  s.deg = c->dalloc<int>(N);
+  dispatch(
+      c, [&] { hipLaunchKernelGGL(k_pd_count<2>, dim3(blocks(N)), dim3(TPB), 0, c->stream, s); },
+      [&] { hipLaunchKernelGGL(k_pd_count<3>, dim3(blocks(N)), dim3(TPB), 0, c->stream, s); });
+  check_launch(c);
+  c->h_deg.resize(N);
+  HIPCHK(hipMemcpyAsync(c->h_deg.data(), s.deg, N * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->nslices = (N + WAVE - 1) / WAVE;
+  c->h_sptr.assign(c->nslices + 1, 0);
+  c->h_bptr.assign(c->nslices + 1, 0);
+  c->nnz = 0;
+  for (int64_t sl = 0; sl < c->nslices; sl++) {
+    int w = 0;
+    for (int64_t i = sl * WAVE; i < std::min(N, (sl + 1) * WAVE); i++) {
+      w = std::max(w, c->h_deg[i]);
+      c->nnz += c->h_deg[i];
+    }
+    c->h_sptr[sl + 1] = c->h_sptr[sl] + (int64_t)w * WAVE;
+    c->h_bptr[sl + 1] = c->h_bptr[sl] + (int64_t)((w + 31) / 32) * WAVE;
+  }
+  c->nnz_padded = c->h_sptr[c->nslices];
+  c->nbitwords = c->h_bptr[c->nslices];
+  s.sptr = c->upload(c->h_sptr.data(), c->h_sptr.size());
+  s.bptr = c->upload(c->h_bptr.data(), c->h_bptr.size());
+  s.col = c->dalloc<int>(c->nnz_padded);
+  s.bits = c->dalloc<unsigned>(c->nbitwords);
+  dispatch(
+      c, [&] { hipLaunchKernelGGL(k_pd_fill<2>, dim3(blocks(N)), dim3(TPB), 0, c->stream, s); },
+      [&] { hipLaunchKernelGGL(k_pd_fill<3>, dim3(blocks(N)), dim3(TPB), 0, c->stream, s); });
+  check_launch(c);
+
+  // ---- contact structures
+  if (c->multi) {
+    // uniform grid: cell >= every search radius the controller can pick
+    double cell = std::max(1.0, d->search_factor) * maxR;
+    if (!(cell > 0.)) cell = 1.;
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+      lo[k] = 1e300;
+      hi[k] = -1e300;
+    }
+    for (int64_t i = 0; i < N; i++)
+      for (int k = 0; k < dim; k++) {
+        lo[k] = std::min(lo[k], d->x_ref[3 * i + k]);
+        hi[k] = std::max(hi[k], d->x_ref[3 * i + k]);
+      }
+    int64_t ncells = 1;
+    for (;;) {
+      ncells = 1;
+      for (int k = 0; k < dim; k++) {
+        double ext = hi[k] - lo[k];
+        double margin = 0.05 * ext + 2 * cell;
+        s.glo[k] = lo[k] - margin;
+        s.gn[k] = (int)std::ceil((ext + 2 * margin) / cell) + 1;
+        ncells *= s.gn[k];
+      }
+      if (ncells <= (int64_t)1 << 25) break;
+      cell *= 1.25;
+    }
+    for (int k = dim; k < 3; k++) {
+      s.glo[k] = 0.;
+      s.gn[k] = 1;
+    }
+    s.cell = cell;
+    s.ncells = ncells;
+    s.cell_count = c->dalloc<int>(ncells + 1);
+    s.cell_start = c->dalloc<int>(ncells + 1);
+    s.cell_nodes = c->dalloc<int>(N);
+    s.node_cell = c->dalloc<int>(N);
+    s.node_slot = c->dalloc<int>(N);
+    // candidate capacity: nodes of a ball of the largest search radius at the
+    // finest mesh spacing
+    double hmin = 1e300;
+    for (auto& q : c->hp) hmin = std::min(hmin, q.mesh_size);
+    double rr = std::max(1.0, d->search_factor) * maxR / hmin;
+    double est = dim == 2 ? 1.25 * M_PI * rr * rr : 0.75 * 4.18879 * rr * rr * rr;
+    int64_t cap = d->contact_capacity > 0 ? d->contact_capacity : (int64_t)std::ceil(est);
+    cap = std::max<int64_t>(32, (cap + 31) / 32 * 32);
+    c->cap = cap;
+    s.cap = (int)cap;
+    s.ccount = c->dalloc<int>(N);
+    s.cand = c->dalloc<int>((size_t)cap * N, false);
+    // damping bookkeeping: wall node compact ids, per-particle wall bitmaps
+    std::vector<int> widx(N, -1), wgid;
+    for (int p = 0; p < c->np; p++)
+      if (c->hp[p].type_index == 1)
+        for (int64_t i = c->hp[p].glob_start; i < c->hp[p].glob_end; i++) {
+          widx[i] = (int)wgid.size();
+          wgid.push_back((int)i);
+        }
+    s.nwall = (int)wgid.size();
+    s.nwords = (s.nwall + 31) / 32;
+    s.widx = c->upload(widx.data(), N);
+    s.wall_gid = c->upload(wgid.data(), std::max<size_t>(1, wgid.size()));
+    s.n_nonwall = (int)nonwall.size();
+    s.nonwall_pid = c->upload(nonwall.data(), std::max<size_t>(1, nonwall.size()));
+    s.wbits = c->dalloc<unsigned>((size_t)std::max(1, s.n_nonwall) * std::max(1, s.nwords));
+    std::vector<int> lo_init(std::max(1, s.n_nonwall), INT_MAX), hi_init(std::max(1, s.n_nonwall), -1);
+    s.wlo = c->upload(lo_init.data(), lo_init.size());
+    s.whi = c->upload(hi_init.data(), hi_init.size());
+    s.fdamp = c->dalloc<double>((size_t)c->np * 3);
+  }
+  // weighted volume for every node (computeStateMx), then the init-time
+  // contact search (DEMModel::init calls updateContactNeighborlist once)
+  dispatch(
+      c, [&] { hipLaunchKernelGGL(k_mx<2>, dim3(blocks(N)), dim3(TPB), 0, c->stream, s); },
+      [&] { hipLaunchKernelGGL(k_mx<3>, dim3(blocks(N)), dim3(TPB), 0, c->stream, s); });
+  check_launch(c);
+  if (c->multi) {
+    Ctrl h2 = h;
+    h2.interval = d->neigh_update_interval;
+    h2.radius = d->search_factor * maxR;
+    HIPCHK(hipMemcpyAsync(s.ctrl, &h2, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
+    dispatch(c, [&] { Launch<2>::contact_search(c); }, [&] { Launch<3>::contact_search(c); });
+    check_launch(c);
+    check_overflow(c);
+  }
+  std::vector<double> mx(N);
+  HIPCHK(hipMemcpyAsync(mx.data(), s.mx, N * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int64_t i = 0; i < N; i++) {
+    double m = mx[i];
+    double fa = std::fabs(m), fb = 1.0E-18;
+    if ((fb - m) > ((fa < fb ? fb : fa) * 1e-5))
+      throw HipError{"weighted nodal volume m_x too close to zero at node " + std::to_string(i)};
+  }
+}
+
+template <class F>
+int guarded(pdem_ctx* c, F f) {
+  if (!c) return set_err(nullptr, "null context");
+  try {
+    f();
+    return 0;
+  } catch (const HipError& e) {
+    return set_err(c, e.msg);
+  } catch (const std::exception& e) {
+    return set_err(c, e.what());
+  }
+}
+
+void ensure_graph(pdem_ctx* c) {
+  if (c->graph_ready) return;
+  // zero-displacement sets are applied once, before any capture
+  bool pending = false;
+  for (size_t s = 0; s < c->ubc.size(); s++)
+    if (c->ubc_zero[s] && !c->zero_done[s]) pending = true;
+  if (pending) throw HipError{"call pdem_integrate_begin before pdem_integrate_cd_graph"};
+  bool prof = c->profiling;
+  c->profiling = false;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  dispatch(c, [&] { Launch<2>::integrate_step(c); }, [&] { Launch<3>::integrate_step(c); });
+  HIPCHK(hipStreamEndCapture(c->stream, &c->graph));
+  c->profiling = prof;
+  HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  c->graph_ready = true;
+}
+
+}  // namespace
+
+// ======================================================================= ABI
+extern "C" {
+
+const char* pdem_last_error(const pdem_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+int pdem_create(const pdem_desc* d, pdem_ctx** out) {
+  if (!out) return set_err(nullptr, "null output pointer");
+  *out = nullptr;
+  if (!d) return set_err(nullptr, "null descriptor");
+  if (d->dim != 2 && d->dim != 3) return set_err(nullptr, "dim must be 2 or 3");
+  if (d->n_nodes <= 0 || d->n_nodes >= INT_MAX) return set_err(nullptr, "n_nodes out of range");
+  if (d->n_particles <= 0 || !d->particles) return set_err(nullptr, "no particles");
+  if (!d->x_ref || !d->vol || !d->fix || !d->pt_id) return set_err(nullptr, "missing node arrays");
+  if (d->precision != PDEM_PREC_FP64 && d->precision != PDEM_PREC_MIXED)
+    return set_err(nullptr, "unknown precision mode");
+  if (d->multi_particle && (!d->contact || d->n_zones <= 0))
+    return set_err(nullptr, "multi-particle model without a contact table");
+  int64_t expect = 0;
+  for (int p = 0; p < d->n_particles; p++) {
+    const auto& q = d->particles[p];
+    if (q.glob_start != expect || q.glob_end <= q.glob_start || q.glob_end > d->n_nodes)
+      return set_err(nullptr, "particle node ranges must be contiguous and ordered");
+    if (q.center_node < q.glob_start || q.center_node >= q.glob_end)
+      return set_err(nullptr, "center node outside its particle");
+    if (q.zone < 0 || (d->multi_particle && q.zone >= d->n_zones)) return set_err(nullptr, "bad zone id");
+    if (q.material_type != PDEM_MAT_RNP && q.material_type != PDEM_MAT_PDSTATE)
+      return set_err(nullptr, "material type not supported on the GPU path");
+    if (q.glob_end - q.glob_start > INT_MAX / 2) return set_err(nullptr, "particle too large");
+    expect = q.glob_end;
+  }
+  if (expect != d->n_nodes) return set_err(nullptr, "particles do not cover all nodes");
+  for (int64_t i = 0; i < d->n_nodes; i++) {
+    int64_t p = d->pt_id[i];
+    if (p < 0 || p >= d->n_particles || i < d->particles[p].glob_start || i >= d->particles[p].glob_end)
+      return set_err(nullptr, "pt_id inconsistent with particle node ranges");
+  }
+  for (int k = 0; k < d->n_ubc + d->n_fbc; k++) {
+    const pdem_bc_set& b = k < d->n_ubc ? d->ubc[k] : d->fbc[k - d->n_ubc];
+    for (int64_t e = 0; e < b.n_entries; e++)
+      if (b.nodes[e] < 0 || b.nodes[e] >= d->n_nodes) return set_err(nullptr, "BC node out of range");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(nullptr, "no HIP device");
+  auto* c = new pdem_ctx();
+  try {
+    build(c, d);
+  } catch (const HipError& e) {
+    g_create_err = e.msg;
+    delete c;
+    return 1;
+  }
+  *out = c;
+  return 0;
+}
+
+void pdem_destroy(pdem_ctx* c) { delete c; }
+
+int pdem_integrate_begin(pdem_ctx* c) {
+  return guarded(c, [&] {
+    dispatch(c, [&] { Launch<2>::displacement_bc(c); }, [&] { Launch<3>::displacement_bc(c); });
+    dispatch(c, [&] { Launch<2>::compute_forces(c); }, [&] { Launch<3>::compute_forces(c); });
+    check_launch(c);
+    if (c->multi) check_overflow(c);
+  });
+}
+
+int pdem_integrate_cd(pdem_ctx* c, int64_t nsteps) {
+  return guarded(c, [&] {
+    for (int64_t k = 0; k < nsteps; k++)
+      dispatch(c, [&] { Launch<2>::integrate_step(c); }, [&] { Launch<3>::integrate_step(c); });
+    check_launch(c);
+    if (c->multi) check_overflow(c);
+  });
+}
+
+int pdem_integrate_cd_graph(pdem_ctx* c, int64_t nsteps) {
+  return guarded(c, [&] {
+    ensure_graph(c);
+    for (int64_t k = 0; k < nsteps; k++) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+    if (c->multi) check_overflow(c);
+  });
+}
+
+int pdem_compute_forces(pdem_ctx* c) {
+  return guarded(c, [&] {
+    dispatch(c, [&] { Launch<2>::compute_forces(c); }, [&] { Launch<3>::compute_forces(c); });
+    check_launch(c);
+    if (c->multi) check_overflow(c);
+  });
+}
+
+int pdem_compute_peridynamic_forces(pdem_ctx* c) {
+  return guarded(c, [&] {
+    dispatch(c, [&] { Launch<2>::pd_forces(c); }, [&] { Launch<3>::pd_forces(c); });
+    check_launch(c);
+  });
+}
+
+int pdem_update_contact_neighborlist(pdem_ctx* c) {
+  return guarded(c, [&] {
+    if (!c->multi) return;
+    dispatch(c, [&] { Launch<2>::contact_search(c); }, [&] { Launch<3>::contact_search(c); });
+    check_launch(c);
+    check_overflow(c);
+  });
+}
+
+int pdem_compute_contact_forces(pdem_ctx* c) {
+  return guarded(c, [&] {
+    if (!c->multi) return;
+    dispatch(c, [&] { Launch<2>::contact_forces(c, true); }, [&] { Launch<3>::contact_forces(c, true); });
+    check_launch(c);
+  });
+}
+
+int pdem_compute_external_forces(pdem_ctx* c) {
+  return guarded(c, [&] {
+    dispatch(c, [&] { Launch<2>::external_forces(c, false); }, [&] { Launch<3>::external_forces(c, false); });
+    check_launch(c);
+  });
+}
+
+int pdem_compute_external_displacement_bc(pdem_ctx* c) {
+  return guarded(c, [&] {
+    dispatch(c, [&] { Launch<2>::displacement_bc(c); }, [&] { Launch<3>::displacement_bc(c); });
+    check_launch(c);
+  });
+}
+
+int pdem_synchronize(pdem_ctx* c) {
+  return guarded(c, [&] { HIPCHK(hipStreamSynchronize(c->stream)); });
+}
+
+int pdem_get_field(pdem_ctx* c, int32_t field, void* host) {
+  return guarded(c, [&] {
+    const int64_t N = c->N;
+    DS& s = c->ds;
+    auto getp = [&](double* const* comp) {
+      std::vector<double> t(N);
+      double* o = (double*)host;
+      for (int k = 0; k < 3; k++) {
+        if (k < c->dim) {
+          HIPCHK(hipMemcpyAsync(t.data(), comp[k], N * 8, hipMemcpyDeviceToHost, c->stream));
+          HIPCHK(hipStreamSynchronize(c->stream));
+        } else {
+          std::fill(t.begin(), t.end(), 0.);
+        }
+        for (int64_t i = 0; i < N; i++) o[3 * i + k] = t[i];
+      }
+    };
+    auto get1 = [&](const void* d, size_t bytes) {
+      HIPCHK(hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    };
+    switch (field) {
+      case PDEM_FIELD_X: getp(s.x); break;
+      case PDEM_FIELD_U: getp(s.u); break;
+      case PDEM_FIELD_V: getp(s.v); break;
+      case PDEM_FIELD_F: getp(s.f); break;
+      case PDEM_FIELD_XREF: getp(s.xr); break;
+      case PDEM_FIELD_VOL: get1(s.vol, N * 8); break;
+      case PDEM_FIELD_Z: get1(s.Z, N * 4); break;
+      case PDEM_FIELD_THETA: get1(s.theta, N * 8); break;
+      case PDEM_FIELD_MX: get1(s.mx, N * 8); break;
+      case PDEM_FIELD_VMAG: get1(s.vmag, N * 8); break;
+      case PDEM_FIELD_BONDS: {
+        std::vector<unsigned> w(c->nbitwords);
+        HIPCHK(hipMemcpyAsync(w.data(), s.bits, w.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        uint8_t* o = (uint8_t*)host;
+        int64_t pos = 0;
+        for (int64_t i = 0; i < N; i++) {
+          int64_t sl = i / WAVE, lane = i % WAVE;
+          for (int k = 0; k < c->h_deg[i]; k++)
+            o[pos++] = (w[c->h_bptr[sl] + (k >> 5) * WAVE + lane] >> (k & 31)) & 1u;
+        }
+        break;
+      }
+      default: throw HipError{"unknown field id"};
+    }
+  });
+}
+
+int pdem_set_field(pdem_ctx* c, int32_t field, const void* host) {
+  return guarded(c, [&] {
+    const int64_t N = c->N;
+    DS& s = c->ds;
+    auto setp = [&](double* const* comp) {
+      std::vector<double> t(N);
+      const double* o = (const double*)host;
+      for (int k = 0; k < c->dim; k++) {
+        for (int64_t i = 0; i < N; i++) t[i] = o[3 * i + k];
+        HIPCHK(hipMemcpyAsync(comp[k], t.data(), N * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+      }
+    };
+    switch (field) {
+      case PDEM_FIELD_X: setp(s.x); break;
+      case PDEM_FIELD_U: setp(s.u); break;
+      case PDEM_FIELD_V: setp(s.v); break;
+      case PDEM_FIELD_F: setp(s.f); break;
+      case PDEM_FIELD_THETA:
+        HIPCHK(hipMemcpyAsync(s.theta, host, N * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        break;
+      case PDEM_FIELD_BONDS: {
+        std::vector<unsigned> w(c->nbitwords, 0u);
+        const uint8_t* o = (const uint8_t*)host;
+        int64_t pos = 0;
+        for (int64_t i = 0; i < N; i++) {
+          int64_t sl = i / WAVE, lane = i % WAVE;
+          for (int k = 0; k < c->h_deg[i]; k++)
+            if (o[pos++]) w[c->h_bptr[sl] + (k >> 5) * WAVE + lane] |= 1u << (k & 31);
+        }
+        HIPCHK(hipMemcpyAsync(s.bits, w.data(), w.size() * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        break;
+      }
+      default: throw HipError{"field is read-only"};
+    }
+  });
+}
+
+int pdem_get_time(pdem_ctx* c, int64_t* n, double* t) {
+  return guarded(c, [&] {
+    Ctrl h = read_ctrl(c);
+    *n = h.n;
+    *t = h.time;
+  });
+}
+
+int pdem_get_neighbor_count(pdem_ctx* c, int32_t kind, int64_t* nnz) {
+  return guarded(c, [&] {
+    if (kind == PDEM_NEIGH_PD) {
+      *nnz = c->nnz;
+      return;
+    }
+    if (!c->multi) {
+      *nnz = 0;
+      return;
+    }
+    std::vector<int> cnt(c->N);
+    HIPCHK(hipMemcpyAsync(cnt.data(), c->ds.ccount, c->N * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int64_t sum = 0;
+    for (int v : cnt) sum += v;
+    *nnz = sum;
+  });
+}
+
+int pdem_get_neighbors(pdem_ctx* c, int32_t kind, int64_t* rowptr, int64_t* cols) {
+  return guarded(c, [&] {
+    const int64_t N = c->N;
+    if (kind == PDEM_NEIGH_PD) {
+      std::vector<int> col(c->nnz_padded);
+      HIPCHK(hipMemcpyAsync(col.data(), c->ds.col, col.size() * 4, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      rowptr[0] = 0;
+      for (int64_t i = 0; i < N; i++) {
+        int64_t sl = i / WAVE, lane = i % WAVE;
+        for (int k = 0; k < c->h_deg[i]; k++) cols[rowptr[i] + k] = col[c->h_sptr[sl] + (int64_t)k * WAVE + lane];
+        rowptr[i + 1] = rowptr[i] + c->h_deg[i];
+      }
+      return;
+    }
+    rowptr[0] = 0;
+    if (!c->multi) {
+      for (int64_t i = 0; i < N; i++) rowptr[i + 1] = 0;
+      return;
+    }
+    std::vector<int> cnt(N);
+    HIPCHK(hipMemcpyAsync(cnt.data(), c->ds.ccount, N * 4, hipMemcpyDeviceToHost, c->stream));
+    std::vector<int> cand((size_t)c->cap * N);
+    HIPCHK(hipMemcpyAsync(cand.data(), c->ds.cand, cand.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < N; i++) {
+      for (int k = 0; k < cnt[i]; k++) cols[rowptr[i] + k] = cand[(size_t)k * N + i];
+      rowptr[i + 1] = rowptr[i] + cnt[i];
+    }
+  });
+}
+
+int pdem_get_contact_state(pdem_ctx* c, pdem_contact_state* st) {
+  return guarded(c, [&] {
+    Ctrl h = read_ctrl(c);
+    st->interval = h.interval;
+    st->counter = h.counter;
+    st->search_radius = h.radius;
+    st->max_velocity = h.max_velocity;
+    st->max_contact_r = h.max_contact_r;
+    st->n_rebuilds = h.n_rebuilds;
+    st->overflow = h.overflow;
+    st->capacity = c->cap;
+  });
+}
+
+int pdem_pp_compressive_test(pdem_ctx* c, int32_t wall_id, int32_t f_dir, double* pen, double* reac) {
+  return guarded(c, [&] {
+    if (wall_id < 0 || wall_id >= c->np || f_dir < 0 || f_dir >= c->dim)
+      throw HipError{"bad wall id or direction"};
+    dispatch(
+        c, [&] { hipLaunchKernelGGL(k_pp_compressive<2>, dim3(1), dim3(64), 0, c->stream, c->ds, wall_id, f_dir); },
+        [&] { hipLaunchKernelGGL(k_pp_compressive<3>, dim3(1), dim3(64), 0, c->stream, c->ds, wall_id, f_dir); });
+    check_launch(c);
+    Ctrl h = read_ctrl(c);
+    *pen = h.pp_pen;
+    *reac = h.pp_reac;
+  });
+}
+
+int pdem_broken_bonds(pdem_ctx* c, int64_t* broken, int64_t* total) {
+  return guarded(c, [&] {
+    unsigned long long zero = 0;
+    HIPCHK(hipMemcpyAsync(&c->ds.ctrl->broken, &zero, 8, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_count_broken, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds);
+    check_launch(c);
+    Ctrl h = read_ctrl(c);
+    *broken = (int64_t)h.broken;
+    *total = c->nnz;
+  });
+}
+
+int pdem_profile_enable(pdem_ctx* c, int32_t on) {
+  return guarded(c, [&] {
+    c->profiling = on != 0;
+    c->prof_ms = 0.;
+    c->prof_n = 0;
+  });
+}
+
+int pdem_profile_get(pdem_ctx* c, double* ms, int64_t* n) {
+  return guarded(c, [&] {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& e : c->ev_used) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, e.first, e.second));
+      c->prof_ms += t;
+      c->prof_n++;
+      c->ev_free.push_back(e);
+    }
+    c->ev_used.clear();
+    *ms = c->prof_n ? c->prof_ms / (double)c->prof_n : 0.;
+    *n = c->prof_n;
+  });
+}
+
+}  // extern "C"

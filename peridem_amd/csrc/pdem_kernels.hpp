@@ -1,0 +1,927 @@
+// pdem_kernels.hpp -- gfx950 kernels of the explicit PeriDEM step.
+//
+// Data layout in HBM (structure of arrays, FP64 state):
+//   xr[d], x[d], u[d], v[d], f[d]   one array per component, N doubles each
+//   vol, mx, theta, vmag            N doubles;  Z: N floats;  fix: N bytes;  ptid: N ints
+//   PD neighbor list: sliced ELL, slices of 64 consecutive nodes (one wavefront):
+//     col[sptr[s] + k*64 + lane]    neighbor k of node 64 s + lane, ascending node id
+//     bits[bptr[s] + (k/32)*64 + lane] bit k%32 = bond broken
+//   so every load of a wavefront's k-th neighbor id or bond word is one
+//   contiguous 256-byte (ids) / 256-byte (words) access, and a lane owns its
+//   bond words (no atomics on the parity path).
+//   contact candidates: column-major ELL cand[k*N + i], count ccount[i].
+//
+// FP64 kernels are compiled with -ffp-contract=off and evaluate every
+// expression in the reference's operation order, so they reproduce the
+// reference's x86-64 FP64 arithmetic (exp() may differ in the last ulp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pdem {
+
+constexpr int WAVE = 64;
+constexpr double COMPARE_EPS = 1e-5;
+
+struct DevParticle {
+  int64_t glob_start, glob_end, center;
+  int zone, type_index, compute_force, material_type;
+  int irreversible, searches, nonwall_index, pad;
+  double rho, horizon, h, Rb, Kn, Rc;
+  double mat[6];
+};
+
+struct DevContact {
+  double contact_r, Kn, mu, betan, kappa;
+};
+
+// device-resident step state: time, cadence controller of the contact search
+struct Ctrl {
+  int64_t n;
+  double time;
+  double dt;
+  double current_dt;
+  int64_t interval;
+  int64_t counter;
+  double radius;
+  double max_velocity;
+  double max_contact_r;
+  double s_factor;
+  int64_t deck_interval;
+  int64_t n_rebuilds;
+  int64_t init_calls;
+  int32_t rebuild;
+  int32_t overflow;
+  int32_t vdirty;
+  int32_t pad;
+  unsigned long long vmax_acc;  // bit pattern of max |v| (non-negative doubles order as integers)
+  double vmax_last;
+  // scratch results
+  double pp_pen, pp_reac;
+  unsigned long long broken;
+};
+
+struct DS {
+  int64_t N;
+  int np, nz;
+  double* xr[3];
+  double* x[3];
+  double* u[3];
+  double* v[3];
+  double* f[3];
+  double *vol, *mx, *theta, *vmag;
+  float* Z;
+  uint8_t* fix;
+  int* ptid;
+  const DevParticle* P;
+  const DevContact* CT;
+  int inf_type;
+  double inf_p0, inf_p1;
+  int state_global;  // particle 0 is state-based: run the dilation pass
+  // PD neighbors, sliced ELL
+  const int64_t* sptr;
+  const int64_t* bptr;
+  int* deg;
+  int* col;
+  unsigned* bits;
+  // contact candidates
+  int cap;
+  int* ccount;
+  int* cand;
+  // cell grid
+  double glo[3];
+  double cell;
+  int gn[3];
+  int64_t ncells;
+  int* cell_count;
+  int* cell_start;
+  int* cell_nodes;
+  int* node_cell;
+  int* node_slot;
+  // damping
+  int n_nonwall;
+  const int* nonwall_pid;
+  int nwall, nwords;
+  const int* widx;
+  const int* wall_gid;
+  unsigned* wbits;
+  int* wlo;
+  int* whi;
+  double* fdamp;  // [np][3]
+  double gravity[3];
+  int has_gravity;
+  Ctrl* ctrl;
+};
+
+// --------------------------------------------------------------- helpers
+__device__ __forceinline__ bool isGreater(double a, double b) {
+  double fa = fabs(a), fb = fabs(b);
+  return (a - b) > ((fa < fb ? fb : fa) * COMPARE_EPS);
+}
+__device__ __forceinline__ bool isLess(double a, double b) {
+  double fa = fabs(a), fb = fabs(b);
+  return (b - a) > ((fa < fb ? fb : fa) * COMPARE_EPS);
+}
+__device__ __forceinline__ double infFn(const DS& s, double r) {
+  if (s.inf_type == 0) return s.inf_p0;
+  if (s.inf_type == 1) return s.inf_p0 + s.inf_p1 * r;
+  return s.inf_p0 * exp(-r * r / s.inf_p1);
+}
+template <int DIM>
+__device__ __forceinline__ double len3(const double* a) {
+  return DIM == 3 ? sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]) : sqrt(a[0] * a[0] + a[1] * a[1]);
+}
+template <int DIM>
+__device__ __forceinline__ double dot3(const double* a, const double* b) {
+  return DIM == 3 ? a[0] * b[0] + a[1] * b[1] + a[2] * b[2] : a[0] * b[0] + a[1] * b[1];
+}
+
+__device__ __forceinline__ int64_t gid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+
+// ------------------------------------------------- PD neighbor list build
+// j in the same particle, j != i, 0 + dx^2 + dy^2 + dz^2 < horizon^2
+// (nanoflann L2_Simple_Adaptor, strict <), scanned in ascending j.
+template <int DIM>
+__global__ void k_pd_count(DS s) {
+  int64_t i = gid();
+  if (i >= s.N) return;
+  const DevParticle& p = s.P[s.ptid[i]];
+  const double r2 = p.horizon * p.horizon;
+  double xi[3];
+  for (int d = 0; d < DIM; d++) xi[d] = s.xr[d][i];
+  int c = 0;
+  for (int64_t j = p.glob_start; j < p.glob_end; j++) {
+    if (j == i) continue;
+    double d2 = 0.;
+    for (int d = 0; d < DIM; d++) {
+      double t = xi[d] - s.xr[d][j];
+      d2 += t * t;
+    }
+    if (d2 < r2) c++;
+  }
+  s.deg[i] = c;
+}
+
+template <int DIM>
+__global__ void k_pd_fill(DS s) {
+  int64_t i = gid();
+  if (i >= s.N) return;
+  const DevParticle& p = s.P[s.ptid[i]];
+  const double r2 = p.horizon * p.horizon;
+  double xi[3];
+  for (int d = 0; d < DIM; d++) xi[d] = s.xr[d][i];
+  const int64_t sl = i / WAVE, lane = i % WAVE;
+  int* out = s.col + s.sptr[sl] + lane;
+  int k = 0;
+  for (int64_t j = p.glob_start; j < p.glob_end; j++) {
+    if (j == i) continue;
+    double d2 = 0.;
+    for (int d = 0; d < DIM; d++) {
+      double t = xi[d] - s.xr[d][j];
+      d2 += t * t;
+    }
+    if (d2 < r2) {
+      out[(int64_t)k * WAVE] = (int)j;
+      k++;
+    }
+  }
+}
+
+// m_x = sum_j r^2 J(r/delta) V_j^c  (materialUtil.cpp:21-73)
+template <int DIM>
+__global__ void k_mx(DS s) {
+  int64_t i = gid();
+  if (i >= s.N) return;
+  const DevParticle& p = s.P[s.ptid[i]];
+  const double horizon = p.horizon, h = p.h;
+  const double check_up = horizon + 0.5 * h, check_low = horizon - 0.5 * h;
+  const int64_t sl = i / WAVE, lane = i % WAVE;
+  const int* c = s.col + s.sptr[sl] + lane;
+  const int n = s.deg[i];
+  double xi[3];
+  for (int d = 0; d < DIM; d++) xi[d] = s.xr[d][i];
+  double m = 0.;
+  for (int k = 0; k < n; k++) {
+    int j = c[(int64_t)k * WAVE];
+    double dx[3];
+    for (int d = 0; d < DIM; d++) dx[d] = s.xr[d][j] - xi[d];
+    double rji = len3<DIM>(dx);
+    if (isGreater(rji, horizon)) continue;
+    double volj = s.vol[j];
+    if (isGreater(rji, check_low)) volj *= (check_up - rji) / h;
+    m += rji * rji * infFn(s, rji / horizon) * volj;
+  }
+  s.mx[i] = m;
+}
+
+// ------------------------------------------------------------ node update
+// integrateCD (demModel.cpp:349-390): semi-implicit Euler on free dofs.
+template <int DIM>
+__global__ void k_integrate(DS s) {
+  int64_t i = gid();
+  Ctrl* ct = s.ctrl;
+  const double dt = ct->dt;
+  if (i == 0) {
+    ct->current_dt = dt;
+    ct->n += 1;
+    ct->time += dt;
+    ct->vdirty = 1;
+  }
+  double vm = 0.;
+  if (i < s.N) {
+    const DevParticle& p = s.P[s.ptid[i]];
+    if (p.compute_force) {
+      const double rho = p.rho;
+      const uint8_t fx = s.fix[i];
+      double vv[3] = {0., 0., 0.};
+      for (int d = 0; d < DIM; d++) {
+        double vd = s.v[d][i];
+        if (!(fx >> d & 1)) {
+          vd += (dt / rho) * s.f[d][i];
+          s.v[d][i] = vd;
+          s.u[d][i] += dt * vd;
+          s.x[d][i] += dt * vd;
+        }
+        vv[d] = vd;
+      }
+      vm = len3<DIM>(vv);
+      s.vmag[i] = vm;
+    }
+  }
+  // global max |v| for the search cadence: wave max, one atomic per wave;
+  // every lane takes part in the shuffles
+  unsigned long long b = (unsigned long long)__double_as_longlong(vm);
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(b, off);
+    b = o > b ? o : b;
+  }
+  if ((threadIdx.x & (WAVE - 1)) == 0 && b > 0) atomicMax(&ct->vmax_acc, b);
+}
+
+// ---------------------------------------------------------- boundary data
+struct DevBc {
+  int time_fn, dir_mask;
+  double tp[4];
+  int64_t n;
+  const int64_t* nodes;
+  const double* amp;
+};
+
+// ParticleULoading::apply (particleULoading.cpp:115-230)
+template <int DIM>
+__global__ void k_ubc(DS s, DevBc bc) {
+  int64_t k = gid();
+  if (k >= bc.n) return;
+  const double time = s.ctrl->time;
+  const int64_t i = bc.nodes[k];
+  double umax = bc.amp[k], du = 0., dv = 0.;
+  if (bc.time_fn == 0)
+    du = umax;
+  else if (bc.time_fn == 1) {
+    du = umax * time;
+    dv = umax;
+  } else if (bc.time_fn == 2) {
+    du = umax * time + bc.tp[1] * time * time;
+    dv = umax + bc.tp[1] * time;
+  } else if (bc.time_fn == 3) {
+    double a = M_PI * bc.tp[1];
+    du = umax * sin(a * time);
+    dv = umax * a * cos(a * time);
+  }
+  double ui[3] = {0., 0., 0.}, vi[3] = {0., 0., 0.};
+  for (int d = 0; d < 3; d++)
+    if (bc.dir_mask >> d & 1) {
+      ui[d] = du;
+      vi[d] = dv;
+    }
+  if (bc.time_fn == 4) {
+    double dx0 = s.xr[0][i] - bc.tp[1], dx1 = s.xr[1][i] - bc.tp[2];
+    double dx2 = (DIM == 3 ? s.xr[2][i] : 0.) - bc.tp[3];
+    double th = bc.tp[0] * time;
+    double rx0 = dx0 * cos(th) - dx1 * sin(th), rx1 = dx0 * sin(th) + dx1 * cos(th);
+    double d0 = -dx0 * sin(th) - dx1 * cos(th), d1 = dx0 * cos(th) - dx1 * sin(th);
+    ui[0] += rx0 - dx0;
+    ui[1] += rx1 - dx1;
+    ui[2] += 0.0 - dx2;
+    vi[0] += bc.tp[0] * d0;
+    vi[1] += bc.tp[0] * d1;
+    vi[2] += bc.tp[0] * 0.0;
+  }
+  for (int d = 0; d < DIM; d++)
+    if (bc.dir_mask >> d & 1) {
+      s.u[d][i] = ui[d];
+      s.v[d][i] = vi[d];
+      s.x[d][i] = ui[d] + s.xr[d][i];
+    }
+}
+
+// ParticleFLoading::apply (particleFLoading.cpp:85-173)
+template <int DIM>
+__global__ void k_fbc(DS s, DevBc bc) {
+  int64_t k = gid();
+  if (k >= bc.n) return;
+  const double time = s.ctrl->time;
+  double fmax = bc.amp[k];
+  if (bc.time_fn == 1)
+    fmax *= time;
+  else if (bc.time_fn == 2) {
+    double x1 = bc.tp[1], x2 = bc.tp[2];
+    double period = floor(time / (x1 + x2));
+    fmax *= isLess(time, period * (x1 + x2) + x1) ? time - period * x2 : (period + 1.) * x1;
+  } else if (bc.time_fn == 3) {
+    if (isGreater(time, bc.tp[1]))
+      fmax *= bc.tp[3] * time;
+    else
+      fmax *= bc.tp[2] * time;
+  } else if (bc.time_fn == 4) {
+    double a = M_PI * bc.tp[1];
+    fmax *= sin(a * time);
+  }
+  fmax *= bc.tp[0];
+  const int64_t i = bc.nodes[k];
+  for (int d = 0; d < DIM; d++) {
+    double fi = (bc.dir_mask >> d & 1) ? fmax : 0.;
+    s.f[d][i] += fi;
+  }
+}
+
+// -------------------------------------------------------- PD bond forces
+// Bond-based RNP: computePeridynamicForces force pass (demModel.cpp:682-788)
+// with RnpMaterial::getBondEF (material.h:342-368).  One lane per node; the
+// force sum runs over the neighbor list in order, as the reference's does.
+template <int DIM>
+__global__ void __launch_bounds__(256) k_pd_rnp(DS s) {
+  int64_t i = gid();
+  if (i >= s.N) return;
+  const DevParticle& p = s.P[s.ptid[i]];
+  if (!p.compute_force) {
+    for (int d = 0; d < DIM; d++) s.f[d][i] = 0.;
+    return;
+  }
+  if (p.material_type != 0) return;
+  const double horizon = p.horizon, h = p.h;
+  const double check_up = horizon + 0.5 * h, check_low = horizon - 0.5 * h;
+  const double Cc = p.mat[0], beta = p.mat[1], rbar = p.mat[2], invF = p.mat[3], fsc = p.mat[4];
+  const bool irrev = p.irreversible != 0;
+  const double Kn = p.Kn, Rc = p.Rc;
+  const int64_t sl = i / WAVE, lane = i % WAVE;
+  const int* c = s.col + s.sptr[sl] + lane;
+  unsigned* bw = s.bits + s.bptr[sl] + lane;
+  const int n = s.deg[i];
+  double xi[3], ui[3];
+  for (int d = 0; d < DIM; d++) {
+    xi[d] = s.xr[d][i];
+    ui[d] = s.u[d][i];
+  }
+  double force[3] = {0., 0., 0.};
+  float Zi = 0.f;
+  unsigned word = 0, wnew = 0;
+  for (int k = 0; k < n; k++) {
+    if ((k & 31) == 0) {
+      if (k > 0 && wnew != word) bw[(int64_t)((k >> 5) - 1) * WAVE] = wnew;
+      word = bw[(int64_t)(k >> 5) * WAVE];
+      wnew = word;
+    }
+    const int j = c[(int64_t)k * WAVE];
+    bool fs = (word >> (k & 31)) & 1u;
+    double dx[3], du[3];
+    for (int d = 0; d < DIM; d++) {
+      dx[d] = s.xr[d][j] - xi[d];
+      du[d] = s.u[d][j] - ui[d];
+    }
+    double volj = s.vol[j];
+    const double rji = len3<DIM>(dx);
+    const double Sji = dot3<DIM>(dx, du) / dot3<DIM>(dx, dx);
+    if (!fs) {
+      if (isGreater(rji, check_low)) volj *= (check_up - rji) / h;
+      const double J = infFn(s, rji / horizon);
+      if (irrev && isGreater(fabs(Sji), fsc * (rbar / sqrt(rji)))) fs = true;
+      double ef = 0.;
+      if (!fs) ef = J * 4. * Sji * Cc * beta * exp(-beta * rji * Sji * Sji) / invF;
+      if (fs) wnew |= 1u << (k & 31);
+      const double scalar_f = ef * volj;
+      const double l = len3<DIM>(dx);
+      for (int d = 0; d < DIM; d++) force[d] += scalar_f * (dx[d] / l);
+    } else {
+      double y[3];
+      for (int d = 0; d < DIM; d++) y[d] = (s.xr[d][j] + s.u[d][j]) - (xi[d] + ui[d]);
+      const double R = len3<DIM>(y);
+      double scalar_f = Kn * volj * (R - Rc) / R;
+      if (scalar_f > 0.) scalar_f = 0.;
+      for (int d = 0; d < DIM; d++) force[d] += scalar_f * y[d];
+    }
+    const double Sc = rbar / sqrt(rji);
+    if (isGreater(fabs(Sji / Sc), (double)Zi)) Zi = (float)fabs(Sji / Sc);
+  }
+  if (n > 0 && wnew != word) bw[(int64_t)((n - 1) >> 5) * WAVE] = wnew;
+  for (int d = 0; d < DIM; d++) s.f[d][i] = force[d];
+  s.Z[i] = Zi;
+}
+
+// State-based dilation pass (demModel.cpp:610-679): bond breaking on the
+// actual stretch, theta_i = 3 sum r (|y|-r) J V^c / m_i.
+template <int DIM>
+__global__ void __launch_bounds__(256) k_pd_dilation(DS s) {
+  int64_t i = gid();
+  if (i >= s.N) return;
+  const DevParticle& p = s.P[s.ptid[i]];
+  if (!p.compute_force || p.material_type != 1) return;
+  const double horizon = p.horizon, h = p.h;
+  const double check_up = horizon + 0.5 * h, check_low = horizon - 0.5 * h;
+  const double sc = p.mat[2];
+  const int64_t sl = i / WAVE, lane = i % WAVE;
+  const int* c = s.col + s.sptr[sl] + lane;
+  unsigned* bw = s.bits + s.bptr[sl] + lane;
+  const int n = s.deg[i];
+  double xi[3], ui[3];
+  for (int d = 0; d < DIM; d++) {
+    xi[d] = s.xr[d][i];
+    ui[d] = s.u[d][i];
+  }
+  const double m = s.mx[i];
+  double theta = 0.;
+  unsigned word = 0, wnew = 0;
+  for (int k = 0; k < n; k++) {
+    if ((k & 31) == 0) {
+      if (k > 0 && wnew != word) bw[(int64_t)((k >> 5) - 1) * WAVE] = wnew;
+      word = bw[(int64_t)(k >> 5) * WAVE];
+      wnew = word;
+    }
+    const int j = c[(int64_t)k * WAVE];
+    double dx[3], dy[3];
+    for (int d = 0; d < DIM; d++) {
+      dx[d] = s.xr[d][j] - xi[d];
+      dy[d] = (dx[d] + s.u[d][j]) - ui[d];
+    }
+    const double rji = len3<DIM>(dx);
+    const double change_length = len3<DIM>(dy) - rji;
+    const double st = change_length / rji;
+    bool fs = (word >> (k & 31)) & 1u;
+    if (!fs && isGreater(fabs(st), sc + 1.0e-10)) {
+      fs = true;
+      wnew |= 1u << (k & 31);
+    }
+    if (!fs) {
+      double volj = s.vol[j];
+      if (isGreater(rji, check_low)) volj *= (check_up - rji) / h;
+      theta += rji * change_length * infFn(s, rji / horizon) * volj;
+    }
+  }
+  if (n > 0 && wnew != word) bw[(int64_t)((n - 1) >> 5) * WAVE] = wnew;
+  s.theta[i] = 3. * theta / m;
+}
+
+// State-based force pass with PdState::getBondEF for both ends (material.h:1415-1429).
+template <int DIM>
+__global__ void __launch_bounds__(256) k_pd_state(DS s) {
+  int64_t i = gid();
+  if (i >= s.N) return;
+  const DevParticle& p = s.P[s.ptid[i]];
+  if (!p.compute_force) {
+    for (int d = 0; d < DIM; d++) s.f[d][i] = 0.;
+    return;
+  }
+  if (p.material_type != 1) return;
+  const double horizon = p.horizon, h = p.h;
+  const double check_up = horizon + 0.5 * h, check_low = horizon - 0.5 * h;
+  const double K = p.mat[0], G = p.mat[1], s0 = p.mat[2];
+  const double Kn = p.Kn, Rc = p.Rc;
+  const int64_t sl = i / WAVE, lane = i % WAVE;
+  const int* c = s.col + s.sptr[sl] + lane;
+  const unsigned* bw = s.bits + s.bptr[sl] + lane;
+  const int n = s.deg[i];
+  double xi[3], ui[3];
+  for (int d = 0; d < DIM; d++) {
+    xi[d] = s.xr[d][i];
+    ui[d] = s.u[d][i];
+  }
+  const double mi = s.mx[i], thetai = s.theta[i];
+  const double alpha_i = 15. * G / mi;
+  const double factor_i = (3. * K / mi) - alpha_i / 3.;
+  double force[3] = {0., 0., 0.};
+  float Zi = 0.f;
+  unsigned word = 0;
+  for (int k = 0; k < n; k++) {
+    if ((k & 31) == 0) word = bw[(int64_t)(k >> 5) * WAVE];
+    const int j = c[(int64_t)k * WAVE];
+    const bool fs = (word >> (k & 31)) & 1u;
+    double dx[3], du[3], dy[3];
+    for (int d = 0; d < DIM; d++) {
+      dx[d] = s.xr[d][j] - xi[d];
+      du[d] = s.u[d][j] - ui[d];
+      dy[d] = dx[d] + du[d];
+    }
+    double volj = s.vol[j];
+    const double rji = len3<DIM>(dx);
+    const double ly = len3<DIM>(dy);
+    const double Sji = (ly - len3<DIM>(dx)) / len3<DIM>(dx);
+    if (!fs) {
+      if (isGreater(rji, check_low)) volj *= (check_up - rji) / h;
+      const double J = infFn(s, rji / horizon);
+      const double mj = s.mx[j], thetaj = s.theta[j];
+      const double cl = Sji * rji;
+      const double ef_i = J * (rji * thetai * factor_i + cl * alpha_i);
+      const double alpha_j = 15. * G / mj;
+      const double factor_j = (3. * K / mj) - alpha_j / 3.;
+      const double ef_j = J * (rji * thetaj * factor_j + cl * alpha_j);
+      const double scalar_f = (ef_i + ef_j) * volj;
+      for (int d = 0; d < DIM; d++) force[d] += scalar_f * (dy[d] / ly);
+    } else {
+      double y[3];
+      for (int d = 0; d < DIM; d++) y[d] = (s.xr[d][j] + s.u[d][j]) - (xi[d] + ui[d]);
+      const double R = len3<DIM>(y);
+      double scalar_f = Kn * volj * (R - Rc) / R;
+      if (scalar_f > 0.) scalar_f = 0.;
+      for (int d = 0; d < DIM; d++) force[d] += scalar_f * y[d];
+    }
+    if (isGreater(fabs(Sji / s0), (double)Zi)) Zi = (float)fabs(Sji / s0);
+  }
+  for (int d = 0; d < DIM; d++) s.f[d][i] = force[d];
+  s.Z[i] = Zi;
+}
+
+// ---------------------------------------------- contact search controller
+// DEMModel::updateContactNeighborSearchParameters (demModel.cpp:1808-1955),
+// one thread; decides whether this step rebuilds the candidate lists.
+__global__ void k_ctrl(DS s) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Ctrl* c = s.ctrl;
+  if (c->vdirty) {
+    c->vmax_last = __longlong_as_double((long long)c->vmax_acc);
+    c->vmax_acc = 0ull;
+    c->vdirty = 0;
+  }
+  bool upd;
+  const int64_t deckI = c->deck_interval;
+  if (c->interval == 0 && isLess(c->radius, 1.e-16)) {
+    c->interval = deckI;
+    c->counter = c->n % c->interval;
+    c->radius = c->max_contact_r * c->s_factor;
+  }
+  if (c->n == 0 && (c->init_calls + 1) <= 2) {
+    c->init_calls++;
+    if (c->init_calls == 1) {
+      upd = true;
+    } else {
+      c->counter++;
+      upd = (c->counter - 1) % c->interval == 0;
+    }
+  } else {
+    if (c->n == 0) c->init_calls++;
+    if (c->interval == 1) {
+      c->radius = c->max_contact_r;
+      c->counter++;
+      upd = (c->counter - 1) % c->interval == 0;
+    } else {
+      int64_t upi = c->interval > 5 ? (int64_t)(0.2 * (double)c->interval) : 1;
+      if (c->counter > 0 && c->counter % upi != 0) {
+        c->counter++;
+        upd = (c->counter - 1) % c->interval == 0;
+      } else {
+        c->max_velocity = c->vmax_last;
+        double safety = c->s_factor > 5 ? c->s_factor : 10;
+        double r_contact = c->s_factor * c->max_contact_r;
+        double r_vel = c->max_velocity * c->current_dt * (double)deckI * safety;
+        if (isGreater(r_vel, r_contact)) {
+          c->interval = (int64_t)(c->max_contact_r / (c->max_velocity * c->current_dt));
+          c->radius = r_contact;
+          c->counter = 0;
+          if (c->interval < 1) {
+            c->interval = 1;
+            c->radius = c->max_contact_r;
+          }
+        } else {
+          c->radius = c->interval < 2 ? c->max_contact_r : r_contact;
+        }
+        c->counter++;
+        upd = (c->counter - 1) % c->interval == 0;
+      }
+    }
+  }
+  c->rebuild = upd ? 1 : 0;
+  if (upd) c->n_rebuilds++;
+}
+
+// ---------------------------------------------------------- cell list
+__device__ __forceinline__ int cellc(double a, double lo, double cell, int n) {
+  double k = floor((a - lo) / cell);
+  int ki = k < 0. ? 0 : (k > (double)(n - 1) ? n - 1 : (int)k);
+  return ki;
+}
+
+__global__ void k_cell_clear(DS s) {
+  if (!s.ctrl->rebuild) return;
+  for (int64_t c = gid(); c <= s.ncells; c += (int64_t)gridDim.x * blockDim.x) s.cell_count[c] = 0;
+}
+
+template <int DIM>
+__global__ void k_cell_count(DS s) {
+  if (!s.ctrl->rebuild) return;
+  int64_t i = gid();
+  if (i >= s.N) return;
+  int cx = cellc(s.x[0][i], s.glo[0], s.cell, s.gn[0]);
+  int cy = cellc(s.x[1][i], s.glo[1], s.cell, s.gn[1]);
+  int cz = DIM == 3 ? cellc(s.x[2][i], s.glo[2], s.cell, s.gn[2]) : 0;
+  int c = (cz * s.gn[1] + cy) * s.gn[0] + cx;
+  s.node_cell[i] = c;
+  s.node_slot[i] = atomicAdd(&s.cell_count[c], 1);
+}
+
+// exclusive scan of ncells counts, one workgroup of 1024 threads
+__global__ void __launch_bounds__(1024) k_cell_scan(DS s) {
+  if (!s.ctrl->rebuild) return;
+  __shared__ int part[1024];
+  const int64_t n = s.ncells;
+  const int t = threadIdx.x;
+  const int64_t chunk = (n + 1023) / 1024;
+  const int64_t b = t * chunk, e = b + chunk < n ? b + chunk : n;
+  int sum = 0;
+  for (int64_t c = b; c < e; c++) sum += s.cell_count[c];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = t > 0 ? part[t - 1] : 0;
+  for (int64_t c = b; c < e; c++) {
+    int v = s.cell_count[c];
+    s.cell_start[c] = run;
+    run += v;
+  }
+  if (t == 1023) s.cell_start[n] = part[1023];
+}
+
+__global__ void k_cell_scatter(DS s) {
+  if (!s.ctrl->rebuild) return;
+  int64_t i = gid();
+  if (i >= s.N) return;
+  s.cell_nodes[s.cell_start[s.node_cell[i]] + s.node_slot[i]] = (int)i;
+}
+
+// ascending node order inside each cell (deterministic candidate order)
+__global__ void k_cell_sort(DS s) {
+  if (!s.ctrl->rebuild) return;
+  int64_t c = gid();
+  if (c >= s.ncells) return;
+  int b = s.cell_start[c], e = s.cell_start[c + 1];
+  for (int a = b + 1; a < e; a++) {
+    int key = s.cell_nodes[a];
+    int q = a - 1;
+    while (q >= b && s.cell_nodes[q] > key) {
+      s.cell_nodes[q + 1] = s.cell_nodes[q];
+      q--;
+    }
+    s.cell_nodes[q + 1] = key;
+  }
+}
+
+// radiusSearchExcludeTag for searching nodes (demModel.cpp:1717-1756):
+// other particles, j != i, 0 + dx^2 + dy^2 + dz^2 < r^2.
+template <int DIM>
+__global__ void __launch_bounds__(256) k_contact_query(DS s) {
+  if (!s.ctrl->rebuild) return;
+  int64_t i = gid();
+  if (i >= s.N) return;
+  const int pi = s.ptid[i];
+  if (!s.P[pi].searches) return;
+  const double r = s.ctrl->radius, r2 = r * r;
+  double xi[3] = {0., 0., 0.};
+  for (int d = 0; d < DIM; d++) xi[d] = s.x[d][i];
+  const int cx = cellc(xi[0], s.glo[0], s.cell, s.gn[0]);
+  const int cy = cellc(xi[1], s.glo[1], s.cell, s.gn[1]);
+  const int cz = DIM == 3 ? cellc(xi[2], s.glo[2], s.cell, s.gn[2]) : 0;
+  int cnt = 0;
+  const int cap = s.cap;
+  const int z0 = DIM == 3 ? (cz > 0 ? cz - 1 : 0) : 0;
+  const int z1 = DIM == 3 ? (cz < s.gn[2] - 1 ? cz + 1 : cz) : 0;
+  for (int z = z0; z <= z1; z++)
+    for (int y = (cy > 0 ? cy - 1 : 0); y <= (cy < s.gn[1] - 1 ? cy + 1 : cy); y++)
+      for (int x = (cx > 0 ? cx - 1 : 0); x <= (cx < s.gn[0] - 1 ? cx + 1 : cx); x++) {
+        const int cc = (z * s.gn[1] + y) * s.gn[0] + x;
+        const int b = s.cell_start[cc], e = s.cell_start[cc + 1];
+        for (int q = b; q < e; q++) {
+          const int j = s.cell_nodes[q];
+          if (j == i || s.ptid[j] == pi) continue;
+          double d2 = 0.;
+          for (int d = 0; d < DIM; d++) {
+            double t = xi[d] - s.x[d][j];
+            d2 += t * t;
+          }
+          if (d2 < r2) {
+            if (cnt < cap) s.cand[(int64_t)cnt * s.N + i] = j;
+            cnt++;
+          }
+        }
+      }
+  if (cnt > cap) {
+    s.ctrl->overflow = 1;
+    cnt = cap;
+  }
+  s.ccount[i] = cnt;
+}
+
+// clear candidate counts of nodes that do not search (only at a rebuild)
+__global__ void k_contact_clear(DS s) {
+  if (!s.ctrl->rebuild) return;
+  int64_t i = gid();
+  if (i >= s.N) return;
+  if (!s.P[s.ptid[i]].searches) s.ccount[i] = 0;
+}
+
+// ------------------------------------------------------- contact forces
+// computeContactForces node loop (demModel.cpp:832-934).  Also records the
+// wall nodes within contact distance of each particle for the damping pass
+// (the condensed wall list, demModel.cpp:998-1022).
+template <int DIM>
+__global__ void __launch_bounds__(256) k_contact_force(DS s) {
+  int64_t i = gid();
+  if (i >= s.N) return;
+  const int n = s.ccount[i];
+  if (n == 0) return;
+  const int pid = s.ptid[i];
+  const DevParticle& pi = s.P[pid];
+  if (!pi.compute_force) return;
+  double yi[3], vi[3];
+  for (int d = 0; d < DIM; d++) {
+    yi[d] = s.x[d][i];
+    vi[d] = s.v[d][i];
+  }
+  double force[3] = {0., 0., 0.};
+  const int nwi = pi.nonwall_index;
+  for (int k = 0; k < n; k++) {
+    const int j = s.cand[(int64_t)k * s.N + i];
+    const int pjid = s.ptid[j];
+    const DevParticle& pj = s.P[pjid];
+    double yji[3];
+    for (int d = 0; d < DIM; d++) yji[d] = s.x[d][j] - yi[d];
+    const double Rji = len3<DIM>(yji);
+    const bool both_walls = pi.type_index == 1 && pj.type_index == 1;
+    if (j == i || pjid == pid || both_walls) continue;
+    const DevContact ct = s.CT[pi.zone * s.nz + pj.zone];
+    if (!isLess(Rji, ct.contact_r)) continue;
+    if (pj.type_index == 1 && nwi >= 0) {
+      const int w = s.widx[j];
+      atomicOr(&s.wbits[(int64_t)nwi * s.nwords + (w >> 5)], 1u << (w & 31));
+      atomicMin(&s.wlo[nwi], w >> 5);
+      atomicMax(&s.whi[nwi], w >> 5);
+    }
+    const double volj = s.vol[j];
+    double vji[3], en[3], et[3];
+    for (int d = 0; d < DIM; d++) {
+      vji[d] = s.v[d][j] - vi[d];
+      en[d] = yji[d] / Rji;
+    }
+    const double vn_mag = dot3<DIM>(vji, en);
+    for (int d = 0; d < DIM; d++) et[d] = vji[d] - vn_mag * en[d];
+    const double etl = len3<DIM>(et);
+    if (isGreater(etl, 0.)) {
+      for (int d = 0; d < DIM; d++) et[d] = et[d] / etl;
+    } else {
+      for (int d = 0; d < DIM; d++) et[d] = 0.;
+    }
+    double scalar_f = ct.Kn * (Rji - ct.contact_r) * volj;
+    if (scalar_f > 0.) scalar_f = 0.;
+    for (int d = 0; d < DIM; d++) force[d] += scalar_f * en[d];
+    const double mf = ct.mu * scalar_f;
+    for (int d = 0; d < DIM; d++) force[d] += mf * et[d];
+  }
+  for (int d = 0; d < DIM; d++) s.f[d][i] += force[d];
+}
+
+// Damping (demModel.cpp:937-1052): one wavefront per non-wall particle.
+// Particle-particle terms are evaluated by all lanes and accumulated in
+// ascending particle order; condensed wall nodes in ascending node order.
+template <int DIM>
+__global__ void __launch_bounds__(64) k_damping(DS s) {
+  const int q = blockIdx.x;
+  if (q >= s.n_nonwall) return;
+  const int lane = threadIdx.x;
+  const int pid = s.nonwall_pid[q];
+  const DevParticle& pi = s.P[pid];
+  const double Ri = pi.Rb;
+  const double vol_pi = M_PI * Ri * Ri;
+  const double rhoi = pi.rho;
+  double xc[3], vc[3];
+  for (int d = 0; d < DIM; d++) {
+    xc[d] = s.x[d][pi.center];
+    vc[d] = s.v[d][pi.center];
+  }
+  double force[3] = {0., 0., 0.};
+  for (int base = 0; base < s.n_nonwall; base += WAVE) {
+    const int qq = base + lane;
+    double term[3] = {0., 0., 0.};
+    bool on = false;
+    if (qq < s.n_nonwall && qq != q) {
+      const DevParticle& pj = s.P[s.nonwall_pid[qq]];
+      const double Rj = pj.Rb;
+      double xcj[3];
+      for (int d = 0; d < DIM; d++) xcj[d] = s.x[d][pj.center] - xc[d];
+      const double dist = len3<DIM>(xcj);
+      const DevContact ct = s.CT[pi.zone * s.nz + pj.zone];
+      if (isLess(dist, Rj + Ri + 1.01 * ct.contact_r)) {
+        on = true;
+        const double vol_pj = M_PI * Rj * Rj;
+        const double m1 = rhoi * vol_pi, m2 = pj.rho * vol_pj;
+        const double meq = 2. * m1 * m2 / (m1 + m2);
+        const double beta_n = ct.betan * sqrt(ct.kappa * ct.contact_r * meq);
+        double hat[3] = {0., 0., 0.};
+        if (isGreater(dist, 0.))
+          for (int d = 0; d < DIM; d++) hat[d] = xcj[d] / dist;
+        double vcj[3];
+        for (int d = 0; d < DIM; d++) vcj[d] = s.v[d][pj.center] - vc[d];
+        double vc_mag = dot3<DIM>(vcj, hat);
+        if (vc_mag > 0.) vc_mag = 0.;
+        for (int d = 0; d < DIM; d++) term[d] = ((beta_n * vc_mag) * hat[d]) / vol_pi;
+      }
+    }
+    unsigned long long mask = __ballot(on);
+    while (mask) {
+      const int b = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      for (int d = 0; d < DIM; d++) force[d] += __shfl(term[d], b);
+    }
+  }
+  // wall nodes within contact distance, ascending; bits are cleared as read
+  if (lane == 0) {
+    const int lo = s.wlo[q], hi = s.whi[q];
+    for (int w = lo; w <= hi; w++) {
+      unsigned word = s.wbits[(int64_t)q * s.nwords + w];
+      if (!word) continue;
+      s.wbits[(int64_t)q * s.nwords + w] = 0u;
+      while (word) {
+        const int b = __ffs(word) - 1;
+        word &= word - 1;
+        const int j = s.wall_gid[w * 32 + b];
+        const DevParticle& pj = s.P[s.ptid[j]];
+        const double meq = rhoi * vol_pi;
+        const DevContact ct = s.CT[pi.zone * s.nz + pj.zone];
+        const double beta_n = ct.betan * sqrt(ct.kappa * ct.contact_r * meq);
+        double xcj[3], hat[3] = {0., 0., 0.}, vcj[3];
+        for (int d = 0; d < DIM; d++) xcj[d] = s.x[d][j] - xc[d];
+        const double l = len3<DIM>(xcj);
+        if (isGreater(l, 0.))
+          for (int d = 0; d < DIM; d++) hat[d] = xcj[d] / l;
+        for (int d = 0; d < DIM; d++) vcj[d] = s.v[d][j] - vc[d];
+        double vc_mag = dot3<DIM>(vcj, hat);
+        if (vc_mag > 0.) vc_mag = 0.;
+        for (int d = 0; d < DIM; d++) force[d] += ((beta_n * vc_mag) * hat[d]) / vol_pi;
+      }
+    }
+    s.wlo[q] = 0x7fffffff;
+    s.whi[q] = -1;
+    for (int d = 0; d < 3; d++) s.fdamp[3 * pid + d] = d < DIM ? force[d] : 0.;
+  }
+}
+
+// f += damping force of the node's particle (non-wall), then f += rho g
+// (computeExternalForces, demModel.cpp:791-812).
+template <int DIM>
+__global__ void k_finalize(DS s, int with_damping) {
+  int64_t i = gid();
+  if (i >= s.N) return;
+  const int pid = s.ptid[i];
+  const DevParticle& p = s.P[pid];
+  if (with_damping && p.type_index == 0)
+    for (int d = 0; d < DIM; d++) s.f[d][i] += s.fdamp[3 * pid + d];
+  if (s.has_gravity)
+    for (int d = 0; d < DIM; d++) s.f[d][i] += p.rho * s.gravity[d];
+}
+
+template <int DIM>
+__global__ void k_apply_damping(DS s) {
+  int64_t i = gid();
+  if (i >= s.N) return;
+  const int pid = s.ptid[i];
+  if (s.P[pid].type_index == 0)
+    for (int d = 0; d < DIM; d++) s.f[d][i] += s.fdamp[3 * pid + d];
+}
+
+// -------------------------------------------------------- observables
+template <int DIM>
+__global__ void k_pp_compressive(DS s, int wall, int fdir) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const DevParticle& w = s.P[wall];
+  const int64_t i0 = w.glob_start;
+  s.ctrl->pp_pen = s.x[fdir][i0] - s.xr[fdir][i0];
+  double r = 0.;
+  for (int64_t i = w.glob_start; i < w.glob_end; i++) r += s.f[fdir][i] * s.vol[i];
+  s.ctrl->pp_reac = r;
+}
+
+__global__ void k_count_broken(DS s) {
+  int64_t i = gid();
+  unsigned long long c = 0;
+  if (i < s.N) {
+    const int64_t sl = i / WAVE, lane = i % WAVE;
+    const int n = s.deg[i];
+    const unsigned* bw = s.bits + s.bptr[sl] + lane;
+    for (int w = 0; w < (n + 31) / 32; w++) c += __popc(bw[(int64_t)w * WAVE]);
+  }
+  for (int off = WAVE / 2; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & (WAVE - 1)) == 0 && c) atomicAdd(&s.ctrl->broken, c);
+}
+
+}  // namespace pdem
