@@ -1,0 +1,189 @@
+"""Benchmark: mesh-node updates per second of the explicit PeriDEM step on one MI355X.
+
+A step is one central-difference time step (node update, displacement BC,
+peridynamic forces, contact search at its natural cadence, contact and
+damping forces, external forces) over the whole synthetic model, with all
+state resident in HBM.  Default workload: the 2D compressive test with 500
+particles (~400 nodes each, ~200k nodes), FP64.
+
+Prints one JSON line.  value = (nodes integrated per step) x steps / seconds
+of the timed region (HIP-graph replay of the step).  The roofline entry is
+for the peridynamic bond-force kernel: algorithmic bytes per launch divided
+by its average duration, timed with HIP events on the engine's stream over a
+second, eagerly launched timed region of the same step.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "mesh-node updates/sec at 1 GPU, % HBM roofline; closure vs plain at fixed error"
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(name: str):
+    from peridem_amd import configs
+
+    if name == "compressive_2d":
+        return configs.compressive_2d(n_particles=500, nodes_per_particle=400, seed=0)
+    if name == "packing_3d":
+        return configs.packing_3d(n_spheres=2000, nodes_per_sphere=1000, seed=0)
+    if name == "two_particle":
+        return configs.two_particle()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def pd_bytes_per_launch(spec, eng, precision):
+    """Algorithmic HBM bytes of one bond-force launch: every array element the
+    kernel must touch, counted once (canonical layout: FP64 state, 4-byte
+    neighbor ids, 1 bit per bond read and written).
+      bond-based:  24 d + 12 + 4.25 N_i   (xRef, u, f: 8d each; V 8; Z 4)
+      state-based force pass: 24 d + 28 + 4.125 N_i (adds m_x, theta 16; bits read only)
+    """
+    import numpy as np
+    from peridem_amd import capi
+
+    d = spec.dim
+    rp, _ = eng.neighbors(capi.NEIGH_PD)
+    deg = np.diff(rp)
+    cf = np.array([p["compute_force"] for p in spec.particles], bool)[spec.pt_id]
+    state = np.array([p["material_type"] == capi.MAT_PDSTATE for p in spec.particles], bool)[spec.pt_id]
+    nb = deg[cf].astype(np.float64)
+    st = state[cf]
+    per = np.where(st, 24 * d + 28 + 4.125 * nb, 24 * d + 12 + 4.25 * nb)
+    return float(per.sum()), float(nb.mean()) if len(nb) else 0.0
+
+
+def read_traffic(kernel_tag: str):
+    path = os.path.join(ROOT, "profiles", f"{kernel_tag}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
+
+
+def cpu_baseline(spec, nthreads, budget_s=12.0, max_steps=-1):
+    """The FP64 oracle on the host cores: the same workload, timed over a bounded
+    sample of steps (chunks of steps until about budget_s seconds have run)."""
+    import oracle
+
+    t0 = time.perf_counter()
+    o = oracle.OracleEngine(spec, nthreads)
+    o.integrate_begin()
+    o.integrate_cd(2)  # warm-up
+    t1 = time.perf_counter()
+    steps, chunk = 0, 2
+    while True:
+        o.integrate_cd(chunk)
+        steps += chunk
+        el = time.perf_counter() - t1
+        if el >= budget_s or (max_steps > 0 and steps >= max_steps):
+            break
+        chunk = max(1, min(chunk * 2, int(chunk * (budget_s - el) / max(el, 1e-3)) + 1))
+    t2 = time.perf_counter()
+    o.close()
+    nodes = spec.compute_nodes()
+    return {"value": round(nodes * steps / (t2 - t1), 1), "unit": "node-updates/s",
+            "cores": nthreads, "kind": "port",
+            "sample": f"{steps} steps ({t2 - t1:.1f} s) of the same workload after setup "
+                      f"({t1 - t0:.1f} s) and 2 warm-up steps; FP64 oracle, {nthreads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", default="compressive_2d")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--cpu-steps", type=int, default=-1)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--prof-steps", type=int, default=300)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; the multi-GPU (particle-sharded) "
+              "form is a design in DESIGN.md and is not part of this benchmark.", file=sys.stderr)
+        raise SystemExit(2)
+
+    import torch
+    from peridem_amd import capi
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    t_setup = time.perf_counter()
+    spec = workload(args.workload)
+    prec = capi.PREC_FP64 if args.precision == "fp64" else capi.PREC_MIXED
+    eng = capi.GpuEngine(spec, prec)
+    print(f"# workload {spec.name}: {spec.n_nodes} nodes, {spec.n_particles} particles, "
+          f"setup {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
+    eng.integrate_begin()
+    eng.integrate_cd_graph(args.warmup)
+    eng.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.integrate_cd_graph(args.steps)
+    eng.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    nodes = spec.compute_nodes()
+    value = nodes * args.steps / secs
+    print(f"# graph region: {args.steps} steps in {secs:.3f} s", file=sys.stderr, flush=True)
+
+    # eager region with HIP events around the bond-force kernel
+    eng.profile_enable(True)
+    eng.synchronize()
+    t2 = time.perf_counter()
+    eng.integrate_cd(args.prof_steps)
+    eng.synchronize()
+    t3 = time.perf_counter()
+    kms, nlaunch = eng.profile_get()
+    eng.profile_enable(False)
+    bytes_launch, mean_nbr = pd_bytes_per_launch(spec, eng, prec)
+    achieved = bytes_launch / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    state = any(p["material_type"] == capi.MAT_PDSTATE for p in spec.particles)
+    ktag = ("pd_state" if state else "pd_rnp") + "_" + spec.name
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(ktag),
+            "kernel": "k_pd_state" if state else "k_pd_rnp",
+            "kernel_ms_avg": round(kms, 5), "launches_timed": nlaunch,
+            "bytes_per_launch": bytes_launch, "mean_neighbors": round(mean_nbr, 2),
+            "timing": f"HIP events on the engine stream over {args.prof_steps} eager steps "
+                      f"({t3 - t2:.3f} s)"}
+    n, t = eng.time()
+    st = eng.contact_state()
+    broken, total = eng.broken_bonds()
+    eng.close()
+
+    cpu = None
+    if not args.no_cpu:
+        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(spec, nthreads, args.cpu_seconds, args.cpu_steps)
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "node-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(secs / args.steps * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == capi.PREC_FP64 else "f32/f64",
+        "data": "synthetic (seeded generator, reference deck semantics)",
+        "config": {"workload": spec.name, "nodes": spec.n_nodes, "nodes_integrated": nodes,
+                   "particles": spec.n_particles, "dim": spec.dim, "dt": spec.dt,
+                   "precision": args.precision, "step_mode": "hip graph replay",
+                   "sim_time_end": t, "contact_rebuilds": st["n_rebuilds"],
+                   "contact_interval": st["interval"], "broken_bond_fraction": broken / max(1, total)},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -58,7 +58,8 @@ def two_particle(R=1.0e-3, h=1.0e-4, gap=1.0e-4, speed=0.1, dt=1.0e-7, n_steps=2
 
 def compressive_2d(n_particles=500, R_lo=0.8e-3, R_hi=1.2e-3, nodes_per_particle=400,
                    box=None, seed=0, dt=5.0e-8, n_steps=50000, wall_speed=0.1,
-                   kn_factor=1.0e-3, epsilon=0.7, mu=0.5, interval=40):
+                   kn_factor=1.0e-3, epsilon=0.7, mu=0.5, interval=40, pitch_factor=1.08,
+                   init_speed=0.0):
     """Config 1: 2D compressive test, 50/50 circles and hexagons, R ~ U[0.8, 1.2] mm,
     random orientation, seeded jittered grid; ~400 nodes per particle; RNP
     (E = 2 GPa, nu = 0.25, rho = 1200, Gc = 500), horizon 3h; contact with
@@ -79,7 +80,7 @@ def compressive_2d(n_particles=500, R_lo=0.8e-3, R_hi=1.2e-3, nodes_per_particle
     hexa, hvol = hexagon_mesh(1.0e-3, h_ref)
     c_cl = center_node(circ, (0, 0, 0), 1.0e-3)
     h_cl = center_node(hexa, (0, 0, 0), 1.0e-3)
-    pitch = 2.0 * R_hi * 1.08
+    pitch = 2.0 * R_hi * pitch_factor
     ncol = int(math.ceil(math.sqrt(n_particles)))
     nrow = int(math.ceil(n_particles / ncol))
     W = ncol * pitch
@@ -110,6 +111,9 @@ def compressive_2d(n_particles=500, R_lo=0.8e-3, R_hi=1.2e-3, nodes_per_particle
                 b.add_particle(0, transform(circ, (cx, cy, 0.0), th, s), cvol * s * s, c_cl, rad)
             else:
                 b.add_particle(1, transform(hexa, (cx, cy, 0.0), th, s), hvol * s * s, h_cl, rad)
+            if init_speed > 0.0:
+                a = rng.uniform(0, 2 * math.pi)
+                b.ic.append(([k], (init_speed * math.cos(a), init_speed * math.sin(a), 0.0)))
             k += 1
     gap = 0.5 * h_wall
     # fixed container: bottom, left and right strips (one wall particle)
@@ -135,7 +139,7 @@ def compressive_2d(n_particles=500, R_lo=0.8e-3, R_hi=1.2e-3, nodes_per_particle
 
 def packing_3d(n_spheres=2000, R_lo=0.9e-3, R_hi=1.1e-3, nodes_per_sphere=1000, L=30.0e-3,
                seed=0, dt=2.0e-8, n_steps=10000, wall_speed=0.05, kn_factor=1.0e-2,
-               max_tries=200000):
+               max_tries=200000, gap_factor=1.04, init_speed=0.0):
     """Config 2: 3D packing, state-based PD: spheres R ~ U[0.9, 1.1] mm placed by
     seeded sequential random insertion in an L^3 cube, ~1000 nodes per sphere,
     horizon 3h, gravity, a fixed bottom wall and a top wall moving down at
@@ -171,7 +175,7 @@ def packing_3d(n_spheres=2000, R_lo=0.9e-3, R_hi=1.1e-3, nodes_per_sphere=1000, 
             for dy in (-1, 0, 1):
                 for dz in (-1, 0, 1):
                     for j in grid.get((key[0] + dx, key[1] + dy, key[2] + dz), ()):
-                        if np.linalg.norm(c - centers[j]) < (r + radii[j]) * 1.04:
+                        if np.linalg.norm(c - centers[j]) < (r + radii[j]) * gap_factor:
                             ok = False
                             break
                     if not ok:
@@ -189,7 +193,11 @@ def packing_3d(n_spheres=2000, R_lo=0.9e-3, R_hi=1.1e-3, nodes_per_sphere=1000, 
     for c, r in zip(centers, radii):
         s = r / 1.0e-3
         th = rng.uniform(0, 2 * math.pi)
-        b.add_particle(0, transform(ball, c, th, s), bvol * s ** 3, b_cl, r)
+        pid = b.add_particle(0, transform(ball, c, th, s), bvol * s ** 3, b_cl, r)
+        if init_speed > 0.0:
+            d = rng.normal(size=3)
+            d *= init_speed / np.linalg.norm(d)
+            b.ic.append(([pid], tuple(d)))
     h_wall = 2.0 * h_ref
     t_wall = 1.5 * h_wall
     bot, bv = box_mesh((0.0, 0.0, -t_wall - 0.5 * h_wall), (L, L, -0.5 * h_wall), h_wall, 3)

@@ -427,9 +427,11 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     s.nwall = (int)wgid.size();
     s.nwords = (s.nwall + 31) / 32;
     s.widx = c->upload(widx.data(), N);
-    s.wall_gid = c->upload(wgid.data(), std::max<size_t>(1, wgid.size()));
     s.n_nonwall = (int)nonwall.size();
-    s.nonwall_pid = c->upload(nonwall.data(), std::max<size_t>(1, nonwall.size()));
+    if (wgid.empty()) wgid.push_back(0);
+    if (nonwall.empty()) nonwall.push_back(0);
+    s.wall_gid = c->upload(wgid.data(), wgid.size());
+    s.nonwall_pid = c->upload(nonwall.data(), nonwall.size());
     s.wbits = c->dalloc<unsigned>((size_t)std::max(1, s.n_nonwall) * std::max(1, s.nwords));
     std::vector<int> lo_init(std::max(1, s.n_nonwall), INT_MAX), hi_init(std::max(1, s.n_nonwall), -1);
     s.wlo = c->upload(lo_init.data(), lo_init.size());

@@ -52,32 +52,48 @@ def _run_compare(spec, chunks, chunk_steps, tag):
 
 
 def test_two_particle_rnp_fp64():
+    # config-0 geometry at reduced size: the collision starts inside the run
     spec = configs.two_particle(R=0.5e-3, h=1.0e-4, gap=2.0e-5, speed=1.0)
-    _run_compare(spec, chunks=6, chunk_steps=50, tag="two_particle")
+    g, r = _run_compare(spec, chunks=8, chunk_steps=50, tag="two_particle")
+    assert np.abs(r.field(capi.FIELD_F)).max() > 0.0
 
 
 def test_two_particle_bond_breaking():
     # fast impact with full contact stiffness: bonds break near the contact
-    spec = configs.two_particle(R=0.5e-3, h=1.0e-4, gap=1.0e-5, speed=40.0, dt=2e-9,
+    spec = configs.two_particle(R=0.5e-3, h=1.0e-4, gap=-2.0e-5, speed=100.0, dt=1e-9,
                                 kn_factor=1.0)
     g, r = _run_compare(spec, chunks=4, chunk_steps=100, tag="breaking")
     b, t = r.broken_bonds()
     print("broken bonds", b, "of", t)
+    assert b > 100
 
 
 def test_compressive_small_fp64():
     spec = configs.compressive_2d(n_particles=12, nodes_per_particle=120, seed=3, interval=10,
-                                  wall_speed=2.0)
+                                  wall_speed=2.0, R_lo=1e-3, R_hi=1e-3, pitch_factor=1.03,
+                                  init_speed=2.0, dt=2e-8)
     g, r = _run_compare(spec, chunks=5, chunk_steps=40, tag="compressive12")
+    assert r.neighbor_count(capi.NEIGH_CONTACT) > 0
     pg = g.pp_compressive_test(spec.wall_id, spec.f_dir)
     pr = r.pp_compressive_test(spec.wall_id, spec.f_dir)
     assert abs(pg[0] - pr[0]) <= 1e-9 * max(abs(pr[0]), 1e-30)
     assert abs(pg[1] - pr[1]) <= 1e-9 * max(abs(pr[1]), 1e-30)
 
 
+def test_compressive_rebuild_every_step():
+    # interval 1: the controller pins the radius to the contact radius and
+    # rebuilds every step
+    spec = configs.compressive_2d(n_particles=9, nodes_per_particle=100, seed=4, interval=1,
+                                  wall_speed=1.0, R_lo=1e-3, R_hi=1e-3, pitch_factor=1.03,
+                                  init_speed=1.0, dt=2e-8)
+    _run_compare(spec, chunks=3, chunk_steps=30, tag="interval1")
+
+
 def test_packing_3d_state_fp64():
-    spec = configs.packing_3d(n_spheres=6, nodes_per_sphere=150, L=6.0e-3, seed=1)
-    _run_compare(spec, chunks=3, chunk_steps=20, tag="packing3d")
+    spec = configs.packing_3d(n_spheres=6, nodes_per_sphere=150, L=5.0e-3, seed=1,
+                              gap_factor=1.0, init_speed=5.0)
+    g, r = _run_compare(spec, chunks=3, chunk_steps=20, tag="packing3d")
+    assert r.neighbor_count(capi.NEIGH_CONTACT) > 0
 
 
 def test_graph_replay_matches_eager():
