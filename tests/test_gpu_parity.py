@@ -53,7 +53,7 @@ def _run_compare(spec, chunks, chunk_steps, tag):
 
 def test_two_particle_rnp_fp64():
     # config-0 geometry at reduced size: the collision starts inside the run
-    spec = configs.two_particle(R=0.5e-3, h=1.0e-4, gap=2.0e-5, speed=1.0)
+    spec = configs.two_particle(R=0.5e-3, h=1.0e-4, gap=-1.0e-4, speed=1.0)
     g, r = _run_compare(spec, chunks=8, chunk_steps=50, tag="two_particle")
     assert np.abs(r.field(capi.FIELD_F)).max() > 0.0
 
