@@ -424,6 +424,24 @@ void updateContactNeighborlist(oracle_ctx *c) {
   if (searchParameters(c)) contactSearch(c);
 }
 
+// ------------------------------------------------------------- bond laws
+// RnpMaterial::getBondEF(r, s, fs, break_bonds = true), force part
+// (material.h:342-368); fs is updated in place.
+inline double rnpBondF(double J, double r, double S, const double *mat, int irrev, bool &fs) {
+  const double Cc = mat[0], beta = mat[1], rbar = mat[2], invF = mat[3], fsc = mat[4];
+  if (irrev && !fs && isGreater(std::abs(S), fsc * (rbar / std::sqrt(r)))) fs = true;
+  if (fs) return 0.;
+  return J * 4. * S * Cc * beta * std::exp(-beta * r * S * S) / invF;
+}
+// PdState::getBondEF(r, s, fs, mx, thetax), force part (material.h:1415-1429)
+inline double pdstateBondF(double J, double r, double S, double K, double G, double mx,
+                           double thetax) {
+  double cl = S * r;
+  double alpha = 15. * G / mx;
+  double factor = (3. * K / mx) - alpha / 3.;
+  return J * (r * thetax * factor + cl * alpha);
+}
+
 // ---------------------------------------------------------------- PD forces
 void computePeridynamicForces(oracle_ctx *c) {
   const int64_t N = c->n;
@@ -492,24 +510,14 @@ void computePeridynamicForces(oracle_ctx *c) {
             // PdState::getBondEF, material.h:1415-1429 (evaluated for i and j)
             const double K = p.mat[0], G = p.mat[1];
             const double mj = c->mX[j], thetaj = c->theta[j];
-            double cl = Sji * rji;
-            double alpha_i = 15. * G / mi;
-            double factor_i = (3. * K / mi) - alpha_i / 3.;
-            double ef_i = J * (rji * thetai * factor_i + cl * alpha_i);
-            double alpha_j = 15. * G / mj;
-            double factor_j = (3. * K / mj) - alpha_j / 3.;
-            double ef_j = J * (rji * thetaj * factor_j + cl * alpha_j);
+            double ef_i = pdstateBondF(J, rji, Sji, K, G, mi, thetai);
+            double ef_j = pdstateBondF(J, rji, Sji, K, G, mj, thetaj);
             double scalar_f = (ef_i + ef_j) * volj;
             const P3 dy = add(dx, du);
             force = add(force, scale(scalar_f, divs(dy, len(dy))));
           } else {
             // RnpMaterial::getBondEF with break_bonds = true, material.h:342-368
-            const double Cc = p.mat[0], beta = p.mat[1], rbar = p.mat[2], invF = p.mat[3],
-                         fsc = p.mat[4];
-            if (p.irreversible && !fs && isGreater(std::abs(Sji), fsc * (rbar / std::sqrt(rji))))
-              fs = true;
-            double ef = 0.;
-            if (!fs) ef = J * 4. * Sji * Cc * beta * std::exp(-beta * rji * Sji * Sji) / invF;
+            double ef = rnpBondF(J, rji, Sji, p.mat, p.irreversible, fs);
             c->bits[k] = fs;
             double scalar_f = ef * volj;
             force = add(force, scale(scalar_f, divs(dx, len(dx))));
@@ -988,5 +996,23 @@ int oracle_broken_bonds(oracle_ctx *c, int64_t *broken, int64_t *total) {
 // pieces used by the known-answer tests
 int oracle_is_greater(double a, double b) { return isGreater(a, b); }
 int oracle_is_less(double a, double b) { return isLess(a, b); }
+/* returns fs after the call; force in *f */
+int oracle_rnp_bond_f(double J, double r, double S, const double *mat, int irrev, int fs_in,
+                      double *f) {
+  bool fs = fs_in != 0;
+  *f = rnpBondF(J, r, S, mat, irrev, fs);
+  return fs ? 1 : 0;
+}
+double oracle_pdstate_bond_f(double J, double r, double S, double K, double G, double mx,
+                             double thetax) {
+  return pdstateBondF(J, r, S, K, G, mx, thetax);
+}
+double oracle_influence(int type, double p0, double p1, double r) {
+  oracle_ctx c;
+  c.inf_type = type;
+  c.inf_p[0] = p0;
+  c.inf_p[1] = p1;
+  return c.infFn(r);
+}
 
 }  // extern "C"

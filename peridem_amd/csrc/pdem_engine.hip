@@ -499,6 +499,17 @@ void ensure_graph(pdem_ctx* c) {
 // ======================================================================= ABI
 extern "C" {
 
+int64_t pdem_abi_sizeof(int32_t which) {
+  switch (which) {
+    case 0: return (int64_t)sizeof(pdem_desc);
+    case 1: return (int64_t)sizeof(pdem_particle);
+    case 2: return (int64_t)sizeof(pdem_contact_pair);
+    case 3: return (int64_t)sizeof(pdem_bc_set);
+    case 4: return (int64_t)sizeof(pdem_contact_state);
+    default: return -1;
+  }
+}
+
 const char* pdem_last_error(const pdem_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
 
 int pdem_create(const pdem_desc* d, pdem_ctx** out) {
