@@ -41,10 +41,15 @@ def workload(name: str):
 
 def pd_bytes_per_launch(spec, eng, precision):
     """Algorithmic HBM bytes of one bond-force launch: every array element the
-    kernel must touch, counted once (canonical layout: FP64 state, 4-byte
-    neighbor ids, 1 bit per bond read and written).
-      bond-based:  24 d + 12 + 4.25 N_i   (xRef, u, f: 8d each; V 8; Z 4)
-      state-based force pass: 24 d + 28 + 4.125 N_i (adds m_x, theta 16; bits read only)
+    kernel must touch, counted once per launch (neighbor data assumed reused
+    on chip).  Returns (bytes in this build's layout, canonical bytes).
+      canonical (FP64 state, 4-byte ids, 1-bit bonds):
+        bond-based RNP            24 d + 12 + 4.25 N_i   (xRef, u, f: 8d each; V 8; Z 4)
+        state-based force pass    24 d + 28 + 4.125 N_i  (adds m_x, theta; bonds read)
+      this build:
+        FP64 RNP kernel           24 d + 16 + 4.25 N_i   (adds the 4-byte degree)
+        mixed state force kernel  24 d + 24 + 2.125 N_i  (2-byte ids; alpha, b terms
+                                                           FP32 instead of m_x, theta)
     """
     import numpy as np
     from peridem_amd import capi
@@ -56,8 +61,12 @@ def pd_bytes_per_launch(spec, eng, precision):
     state = np.array([p["material_type"] == capi.MAT_PDSTATE for p in spec.particles], bool)[spec.pt_id]
     nb = deg[cf].astype(np.float64)
     st = state[cf]
-    per = np.where(st, 24 * d + 28 + 4.125 * nb, 24 * d + 12 + 4.25 * nb)
-    return float(per.sum()), float(nb.mean()) if len(nb) else 0.0
+    canon = np.where(st, 24 * d + 28 + 4.125 * nb, 24 * d + 12 + 4.25 * nb)
+    if precision == capi.PREC_MIXED:
+        layout = np.where(st, 24 * d + 24 + 2.125 * nb, 24 * d + 16 + 4.25 * nb)
+    else:
+        layout = np.where(st, 24 * d + 32 + 4.125 * nb, 24 * d + 16 + 4.25 * nb)
+    return float(layout.sum()), float(canon.sum()), float(nb.mean()) if len(nb) else 0.0
 
 
 def read_traffic(kernel_tag: str):
@@ -149,19 +158,26 @@ def main():
     t3 = time.perf_counter()
     kms, nlaunch = eng.profile_get()
     eng.profile_enable(False)
-    bytes_launch, mean_nbr = pd_bytes_per_launch(spec, eng, prec)
+    bytes_launch, bytes_canon, mean_nbr = pd_bytes_per_launch(spec, eng, prec)
     achieved = bytes_launch / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    achieved_canon = bytes_canon / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     state = any(p["material_type"] == capi.MAT_PDSTATE for p in spec.particles)
-    ktag = ("pd_state" if state else "pd_rnp") + "_" + spec.name
+    mixed = prec == capi.PREC_MIXED and state
+    kname = "k_mixed_force" if mixed else ("k_pd_state" if state else "k_pd_rnp")
+    ktag = kname[2:] + "_" + spec.name
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(ktag),
-            "kernel": "k_pd_state" if state else "k_pd_rnp",
+            "kernel": kname,
             "kernel_ms_avg": round(kms, 5), "launches_timed": nlaunch,
-            "bytes_per_launch": bytes_launch, "mean_neighbors": round(mean_nbr, 2),
+            "bytes_per_launch": bytes_launch, "bytes_per_launch_canonical": bytes_canon,
+            "achieved_canonical": round(achieved_canon, 2),
+            "frac_canonical": round(achieved_canon / HBM_PEAK_GBS, 4),
+            "mean_neighbors": round(mean_nbr, 2),
             "timing": f"HIP events on the engine stream over {args.prof_steps} eager steps "
                       f"({t3 - t2:.3f} s)"}
     n, t = eng.time()
     st = eng.contact_state()
+    n_cand = eng.neighbor_count(capi.NEIGH_CONTACT)
     broken, total = eng.broken_bonds()
     eng.close()
 
@@ -179,7 +195,8 @@ def main():
                    "particles": spec.n_particles, "dim": spec.dim, "dt": spec.dt,
                    "precision": args.precision, "step_mode": "hip graph replay",
                    "sim_time_end": t, "contact_rebuilds": st["n_rebuilds"],
-                   "contact_interval": st["interval"], "broken_bond_fraction": broken / max(1, total)},
+                   "contact_interval": st["interval"], "contact_candidates": n_cand,
+                   "broken_bond_fraction": broken / max(1, total)},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

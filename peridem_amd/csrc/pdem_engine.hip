@@ -18,6 +18,7 @@
 
 #include "pdem.h"
 #include "pdem_kernels.hpp"
+#include "pdem_mixed.hpp"
 
 using namespace pdem;
 
@@ -57,6 +58,8 @@ struct pdem_ctx {
   std::vector<int> h_deg;
   bool state_any = false, rnp_any = false;
   int64_t cap = 0;
+  MixedArgs mixed{};
+  int ntiles = 0;
   bool graph_ready = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -102,9 +105,35 @@ namespace {
 
 template <int DIM>
 struct Launch {
-  static void pd_forces(pdem_ctx* c) {
+  template <int INF>
+  static void pd_inf(pdem_ctx* c, int bin) {
     hipStream_t st = c->stream;
     const unsigned nb = blocks(c->N);
+    if (c->precision == PDEM_PREC_MIXED && c->ntiles > 0) {
+      hipLaunchKernelGGL(k_bin_zero<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+      if (c->rnp_any) hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(nb), dim3(TPB), 0, st, c->ds, 0);
+      hipLaunchKernelGGL((k_mixed_dilation<DIM, INF>), dim3(c->ntiles), dim3(MIXED_TPB), 0, st, c->ds,
+                         c->mixed);
+      std::pair<hipEvent_t, hipEvent_t> ev{};
+      if (c->profiling) {
+        if (c->ev_free.empty()) {
+          hipEvent_t a, b;
+          HIPCHK(hipEventCreate(&a));
+          HIPCHK(hipEventCreate(&b));
+          c->ev_free.push_back({a, b});
+        }
+        ev = c->ev_free.back();
+        c->ev_free.pop_back();
+        HIPCHK(hipEventRecord(ev.first, st));
+      }
+      hipLaunchKernelGGL((k_mixed_force<DIM, INF>), dim3(c->ntiles), dim3(MIXED_TPB), 0, st, c->ds,
+                         c->mixed);
+      if (c->profiling) {
+        HIPCHK(hipEventRecord(ev.second, st));
+        c->ev_used.push_back(ev);
+      }
+      return;
+    }
     std::pair<hipEvent_t, hipEvent_t> ev{};
     if (c->profiling) {
       if (c->ev_free.empty()) {
@@ -116,34 +145,51 @@ struct Launch {
       ev = c->ev_free.back();
       c->ev_free.pop_back();
     }
-    if (c->ds.state_global) hipLaunchKernelGGL(k_pd_dilation<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+    if (c->ds.state_global) {
+      hipLaunchKernelGGL((k_pd_dilation<DIM, INF>), dim3(nb), dim3(TPB), 0, st, c->ds, bin);
+      bin = 0;
+    }
     if (c->profiling) HIPCHK(hipEventRecord(ev.first, st));
-    if (c->rnp_any) hipLaunchKernelGGL(k_pd_rnp<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
-    if (c->state_any) hipLaunchKernelGGL(k_pd_state<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
-    if (!c->rnp_any && !c->state_any) hipLaunchKernelGGL(k_pd_rnp<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+    if (c->rnp_any || !c->state_any) {
+      hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(nb), dim3(TPB), 0, st, c->ds, bin);
+      bin = 0;
+    }
+    if (c->state_any) hipLaunchKernelGGL((k_pd_state<DIM, INF>), dim3(nb), dim3(TPB), 0, st, c->ds, bin);
     if (c->profiling) {
       HIPCHK(hipEventRecord(ev.second, st));
       c->ev_used.push_back(ev);
     }
   }
+  static void pd_forces(pdem_ctx* c, int bin) {
+    if (c->ds.inf_type == 0)
+      pd_inf<0>(c, bin);
+    else if (c->ds.inf_type == 1)
+      pd_inf<1>(c, bin);
+    else
+      pd_inf<2>(c, bin);
+  }
+  // standalone search: controller, binning, candidate query
   static void contact_search(pdem_ctx* c) {
     hipStream_t st = c->stream;
     const unsigned nb = blocks(c->N);
-    hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(64), 0, st, c->ds);
-    hipLaunchKernelGGL(k_cell_clear, dim3(blocks(c->ds.ncells + 1)), dim3(TPB), 0, st, c->ds);
-    hipLaunchKernelGGL(k_cell_count<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
-    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, st, c->ds);
-    hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(TPB), 0, st, c->ds);
-    hipLaunchKernelGGL(k_cell_sort, dim3(blocks(c->ds.ncells)), dim3(TPB), 0, st, c->ds);
-    hipLaunchKernelGGL(k_contact_clear, dim3(nb), dim3(TPB), 0, st, c->ds);
-    hipLaunchKernelGGL(k_contact_query<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+    hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, st, c->ds);
+    hipLaunchKernelGGL(k_bin<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+    hipLaunchKernelGGL(k_query<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+  }
+  static void contact_force_kernel(pdem_ctx* c) {
+    const int64_t waves = std::min<int64_t>(c->N, 8192);
+    hipLaunchKernelGGL(k_contact_force<DIM>, dim3((unsigned)((waves + CWAVES - 1) / CWAVES)),
+                       dim3(CWAVES * WAVE), 0, c->stream, c->ds);
+  }
+  static void damping(pdem_ctx* c) {
+    if (c->ds.n_nonwall > 0)
+      hipLaunchKernelGGL(k_damping<DIM>, dim3(c->ds.n_nonwall), dim3(64), 0, c->stream, c->ds);
   }
   static void contact_forces(pdem_ctx* c, bool apply_damping) {
     hipStream_t st = c->stream;
     const unsigned nb = blocks(c->N);
-    hipLaunchKernelGGL(k_contact_force<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
-    if (c->ds.n_nonwall > 0)
-      hipLaunchKernelGGL(k_damping<DIM>, dim3(c->ds.n_nonwall), dim3(64), 0, st, c->ds);
+    contact_force_kernel(c);
+    damping(c);
     if (apply_damping) hipLaunchKernelGGL(k_apply_damping<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
   }
   static void external_forces(pdem_ctx* c, bool with_damping) {
@@ -164,18 +210,26 @@ struct Launch {
       if (b.n > 0) hipLaunchKernelGGL(k_ubc<DIM>, dim3(blocks(b.n)), dim3(TPB), 0, c->stream, c->ds, b);
     }
   }
-  static void compute_forces(pdem_ctx* c) {
-    pd_forces(c);
+  // forces after the controller has run: PD (binning on a rebuild), contact
+  // (query on a rebuild, then forces), damping, external forces
+  static void forces_after_ctrl(pdem_ctx* c) {
+    pd_forces(c, c->multi ? 1 : 0);
     if (c->multi) {
-      contact_search(c);
-      contact_forces(c, false);
+      hipLaunchKernelGGL(k_query<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds);
+      contact_force_kernel(c);
+      damping(c);
     }
     external_forces(c, c->multi != 0);
   }
+  static void compute_forces(pdem_ctx* c) {
+    if (c->multi) hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, c->stream, c->ds);
+    forces_after_ctrl(c);
+  }
   static void integrate_step(pdem_ctx* c) {
-    hipLaunchKernelGGL(k_integrate<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds);
+    hipLaunchKernelGGL(k_integrate<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds, c->multi);
     displacement_bc(c);
-    compute_forces(c);
+    if (c->multi) hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, c->stream, c->ds);
+    forces_after_ctrl(c);
   }
 };
 
@@ -204,9 +258,14 @@ Ctrl read_ctrl(pdem_ctx* c) {
 
 void check_overflow(pdem_ctx* c) {
   Ctrl h = read_ctrl(c);
-  if (h.overflow)
+  if (h.overflow == 1)
     throw HipError{"contact candidate capacity exceeded (" + std::to_string(c->cap) +
                    " per node); create the model with a larger contact_capacity"};
+  if (h.overflow == 3)
+    throw HipError{"more than 64 active contact pairs at one node"};
+  if (h.overflow == 2)
+    throw HipError{"cell capacity exceeded (" + std::to_string(c->ds.cell_cap) +
+                   " nodes per search cell): nodes crowded far beyond their initial packing"};
 }
 
 void build(pdem_ctx* c, const pdem_desc* d) {
@@ -365,6 +424,45 @@ void build(pdem_ctx* c, const pdem_desc* d) {
       [&] { hipLaunchKernelGGL(k_pd_fill<3>, dim3(blocks(N)), dim3(TPB), 0, c->stream, s); });
   check_launch(c);
 
+  // ---- mixed precision: 16-bit neighbor ids and particle tiles
+  if (c->precision == PDEM_PREC_MIXED && c->state_any) {
+    if (!s.state_global) throw HipError{"mixed precision needs a state-based first particle"};
+    for (auto& q : c->hp)
+      if (q.glob_end - q.glob_start > 65536)
+        throw HipError{"mixed precision needs particles of at most 65536 nodes"};
+    uint16_t* col16 = c->dalloc<uint16_t>(c->nnz_padded);
+    hipLaunchKernelGGL(k_col16, dim3(blocks(N)), dim3(TPB), 0, c->stream, s, col16);
+    int* dlo = c->dalloc<int>(N);
+    int* dhi = c->dalloc<int>(N);
+    hipLaunchKernelGGL(k_nbr_range, dim3(blocks(N)), dim3(TPB), 0, c->stream, s, dlo, dhi);
+    check_launch(c);
+    std::vector<int> lo(N), hi(N);
+    HIPCHK(hipMemcpyAsync(lo.data(), dlo, N * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(hi.data(), dhi, N * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<DevTile> tiles;
+    for (int p = 0; p < c->np; p++) {
+      const auto& q = c->hp[p];
+      if (!q.compute_force || q.material_type != PDEM_MAT_PDSTATE) continue;
+      for (int64_t ts = q.glob_start; ts < q.glob_end; ts += TILE_NODES) {
+        const int64_t te = std::min<int64_t>(q.glob_end, ts + TILE_NODES);
+        int ws = (int)ts, we = (int)te;
+        for (int64_t i = ts; i < te; i++) {
+          ws = std::min(ws, lo[i]);
+          we = std::max(we, hi[i] + 1);
+        }
+        tiles.push_back(DevTile{(int)ts, (int)te, ws, we, p, (we - ws) <= LDS_NODES ? 1 : 0});
+      }
+    }
+    c->ntiles = (int)tiles.size();
+    if (tiles.empty()) tiles.push_back(DevTile{0, 0, 0, 1, 0, 0});
+    c->mixed.tiles = c->upload(tiles.data(), tiles.size());
+    c->mixed.ntiles = c->ntiles;
+    c->mixed.col16 = col16;
+    c->mixed.fa = c->dalloc<float>(N);
+    c->mixed.fb = c->dalloc<float>(N);
+  }
+
   // ---- contact structures
   if (c->multi) {
     // uniform grid: cell >= every search radius the controller can pick
@@ -399,11 +497,27 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     }
     s.cell = cell;
     s.ncells = ncells;
+    // cell capacity from the densest cell of the reference configuration, with
+    // room for compaction
+    {
+      std::vector<int> cnt(ncells, 0);
+      for (int64_t i = 0; i < N; i++) {
+        int64_t cc = 0, mul = 1;
+        for (int k = 0; k < dim; k++) {
+          double q = std::floor((d->x_ref[3 * i + k] + d->u[3 * i + k] * 0 - s.glo[k]) / cell);
+          int64_t qi = q < 0 ? 0 : (q > s.gn[k] - 1 ? s.gn[k] - 1 : (int64_t)q);
+          cc += qi * mul;
+          mul *= s.gn[k];
+        }
+        cnt[cc]++;
+      }
+      int mx = 0;
+      for (int v : cnt) mx = std::max(mx, v);
+      int capc = std::max(64, ((2 * mx + 32) + 31) / 32 * 32);
+      s.cell_cap = capc;
+    }
     s.cell_count = c->dalloc<int>(ncells + 1);
-    s.cell_start = c->dalloc<int>(ncells + 1);
-    s.cell_nodes = c->dalloc<int>(N);
-    s.node_cell = c->dalloc<int>(N);
-    s.node_slot = c->dalloc<int>(N);
+    s.cell_nodes = c->dalloc<int>((size_t)ncells * s.cell_cap, false);
     // candidate capacity: nodes of a ball of the largest search radius at the
     // finest mesh spacing
     double hmin = 1e300;
@@ -415,6 +529,7 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     c->cap = cap;
     s.cap = (int)cap;
     s.ccount = c->dalloc<int>(N);
+    s.active = c->dalloc<int>(N);
     s.cand = c->dalloc<int>((size_t)cap * N, false);
     // damping bookkeeping: wall node compact ids, per-particle wall bitmaps
     std::vector<int> widx(N, -1), wgid;
@@ -437,12 +552,35 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     s.wlo = c->upload(lo_init.data(), lo_init.size());
     s.whi = c->upload(hi_init.data(), hi_init.size());
     s.fdamp = c->dalloc<double>((size_t)c->np * 3);
+    {
+      const int nn = std::max(1, s.n_nonwall);
+      std::vector<double> rad(nn, 0.), rho(nn, 0.);
+      std::vector<int> zn(nn, 0);
+      for (int q = 0; q < s.n_nonwall; q++) {
+        rad[q] = c->hp[nonwall[q]].bounding_radius;
+        rho[q] = c->hp[nonwall[q]].density;
+        zn[q] = c->hp[nonwall[q]].zone;
+      }
+      s.prad = c->upload(rad.data(), nn);
+      s.prho = c->upload(rho.data(), nn);
+      s.pzone = c->upload(zn.data(), nn);
+      s.pcx = c->dalloc<double>(3 * (size_t)nn);
+      s.pcv = c->dalloc<double>(3 * (size_t)nn);
+    }
+  }
+  {
+    std::vector<int> meta(c->np);
+    for (int p = 0; p < c->np; p++) meta[p] = (c->hp[p].zone & 0xffff) | ((c->hp[p].type_index == 1) << 16);
+    s.pmeta = c->upload(meta.data(), meta.size());
+    s.nblockmax = (int)blocks(N);
+    s.blockmax = c->dalloc<unsigned long long>(s.nblockmax);
   }
   // weighted volume for every node (computeStateMx), then the init-time
   // contact search (DEMModel::init calls updateContactNeighborlist once)
   dispatch(
       c, [&] { hipLaunchKernelGGL(k_mx<2>, dim3(blocks(N)), dim3(TPB), 0, c->stream, s); },
       [&] { hipLaunchKernelGGL(k_mx<3>, dim3(blocks(N)), dim3(TPB), 0, c->stream, s); });
+  // nodes that never search keep an empty candidate list
   check_launch(c);
   if (c->multi) {
     Ctrl h2 = h;
@@ -600,7 +738,7 @@ int pdem_compute_forces(pdem_ctx* c) {
 
 int pdem_compute_peridynamic_forces(pdem_ctx* c) {
   return guarded(c, [&] {
-    dispatch(c, [&] { Launch<2>::pd_forces(c); }, [&] { Launch<3>::pd_forces(c); });
+    dispatch(c, [&] { Launch<2>::pd_forces(c, 0); }, [&] { Launch<3>::pd_forces(c, 0); });
     check_launch(c);
   });
 }

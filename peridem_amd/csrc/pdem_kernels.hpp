@@ -17,6 +17,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <climits>
 
 namespace pdem {
 
@@ -53,7 +54,7 @@ struct Ctrl {
   int32_t rebuild;
   int32_t overflow;
   int32_t vdirty;
-  int32_t pad;
+  uint32_t nactive;  // nodes with a non-empty candidate list (compute-force particles)
   unsigned long long vmax_acc;  // bit pattern of max |v| (non-negative doubles order as integers)
   double vmax_last;
   // scratch results
@@ -88,16 +89,15 @@ struct DS {
   int cap;
   int* ccount;
   int* cand;
+  int* active;  // [N] node ids with candidates, appended at each rebuild
   // cell grid
   double glo[3];
   double cell;
   int gn[3];
   int64_t ncells;
-  int* cell_count;
-  int* cell_start;
-  int* cell_nodes;
-  int* node_cell;
-  int* node_slot;
+  int cell_cap;     // fixed capacity of a cell
+  int* cell_count;  // [ncells]
+  int* cell_nodes;  // [ncells][cell_cap], unordered inside a cell
   // damping
   int n_nonwall;
   const int* nonwall_pid;
@@ -108,6 +108,17 @@ struct DS {
   int* wlo;
   int* whi;
   double* fdamp;  // [np][3]
+  // per non-wall particle, packed each step for the damping pass
+  double* pcx;      // [3][n_nonwall] centre position
+  double* pcv;      // [3][n_nonwall] centre velocity
+  const double* prad;  // [n_nonwall] bounding radius
+  const double* prho;  // [n_nonwall] density
+  const int* pzone;    // [n_nonwall] zone
+  // per particle: zone | wall << 16, for the contact loops
+  const int* pmeta;
+  // node-update block maxima of |v|
+  unsigned long long* blockmax;
+  int nblockmax;
   double gravity[3];
   int has_gravity;
   Ctrl* ctrl;
@@ -137,6 +148,9 @@ __device__ __forceinline__ double dot3(const double* a, const double* b) {
 }
 
 __device__ __forceinline__ int64_t gid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+
+template <int DIM>
+__device__ void bin_node(const DS& s, int64_t i);
 
 // ------------------------------------------------- PD neighbor list build
 // j in the same particle, j != i, 0 + dx^2 + dy^2 + dz^2 < horizon^2
@@ -187,6 +201,34 @@ __global__ void k_pd_fill(DS s) {
   }
 }
 
+// per node: smallest and largest neighbor id (tile windows of the mixed kernels)
+__global__ void k_nbr_range(DS s, int* lo, int* hi) {
+  int64_t i = gid();
+  if (i >= s.N) return;
+  const int64_t sl = i / WAVE, lane = i % WAVE;
+  const int* c = s.col + s.sptr[sl] + lane;
+  int a = (int)i, b = (int)i;
+  for (int k = 0; k < s.deg[i]; k++) {
+    const int j = c[(int64_t)k * WAVE];
+    a = j < a ? j : a;
+    b = j > b ? j : b;
+  }
+  lo[i] = a;
+  hi[i] = b;
+}
+
+// neighbor ids relative to the particle's first node, 16 bit
+__global__ void k_col16(DS s, uint16_t* col16) {
+  int64_t i = gid();
+  if (i >= s.N) return;
+  const int64_t sl = i / WAVE, lane = i % WAVE;
+  const int64_t gs = s.P[s.ptid[i]].glob_start;
+  for (int k = 0; k < s.deg[i]; k++) {
+    const int64_t o = s.sptr[sl] + (int64_t)k * WAVE + lane;
+    col16[o] = (uint16_t)(s.col[o] - gs);
+  }
+}
+
 // m_x = sum_j r^2 J(r/delta) V_j^c  (materialUtil.cpp:21-73)
 template <int DIM>
 __global__ void k_mx(DS s) {
@@ -214,19 +256,118 @@ __global__ void k_mx(DS s) {
   s.mx[i] = m;
 }
 
+// ---------------------------------------------- contact search controller
+// DEMModel::updateContactNeighborSearchParameters (demModel.cpp:1808-1955):
+// decides whether this step rebuilds the candidate lists.  Run by one thread.
+__device__ void ctrl_decide(Ctrl* cg) {
+  Ctrl h = *cg;
+  Ctrl* c = &h;
+  if (c->vdirty) {
+    c->vmax_last = __longlong_as_double((long long)c->vmax_acc);
+    c->vdirty = 0;
+  }
+  bool upd;
+  const int64_t deckI = c->deck_interval;
+  if (c->interval == 0 && isLess(c->radius, 1.e-16)) {
+    c->interval = deckI;
+    c->counter = c->n % c->interval;
+    c->radius = c->max_contact_r * c->s_factor;
+  }
+  if (c->n == 0 && (c->init_calls + 1) <= 2) {
+    c->init_calls++;
+    if (c->init_calls == 1) {
+      upd = true;
+    } else {
+      c->counter++;
+      upd = (c->counter - 1) % c->interval == 0;
+    }
+  } else {
+    if (c->n == 0) c->init_calls++;
+    if (c->interval == 1) {
+      c->radius = c->max_contact_r;
+      c->counter++;
+      upd = (c->counter - 1) % c->interval == 0;
+    } else {
+      int64_t upi = c->interval > 5 ? (int64_t)(0.2 * (double)c->interval) : 1;
+      if (c->counter > 0 && c->counter % upi != 0) {
+        c->counter++;
+        upd = (c->counter - 1) % c->interval == 0;
+      } else {
+        c->max_velocity = c->vmax_last;
+        double safety = c->s_factor > 5 ? c->s_factor : 10;
+        double r_contact = c->s_factor * c->max_contact_r;
+        double r_vel = c->max_velocity * c->current_dt * (double)deckI * safety;
+        if (isGreater(r_vel, r_contact)) {
+          c->interval = (int64_t)(c->max_contact_r / (c->max_velocity * c->current_dt));
+          c->radius = r_contact;
+          c->counter = 0;
+          if (c->interval < 1) {
+            c->interval = 1;
+            c->radius = c->max_contact_r;
+          }
+        } else {
+          c->radius = c->interval < 2 ? c->max_contact_r : r_contact;
+        }
+        c->counter++;
+        upd = (c->counter - 1) % c->interval == 0;
+      }
+    }
+  }
+  c->rebuild = upd ? 1 : 0;
+  if (upd) {
+    c->n_rebuilds++;
+    c->nactive = 0u;
+  }
+  *cg = h;
+}
+
+// One workgroup: reduce the node-update block maxima of |v|, run the
+// controller, pack the damping particles' centre data, and clear the cell
+// table when this step rebuilds the candidate lists.
+__global__ void __launch_bounds__(1024) k_ctrl(DS s) {
+  __shared__ unsigned long long red[1024 / WAVE];
+  __shared__ int reb;
+  Ctrl* c = s.ctrl;
+  const int t = threadIdx.x;
+  if (c->vdirty) {
+    unsigned long long m = 0;
+    for (int k = t; k < s.nblockmax; k += blockDim.x) m = s.blockmax[k] > m ? s.blockmax[k] : m;
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+      unsigned long long o = __shfl_xor(m, off);
+      m = o > m ? o : m;
+    }
+    if ((t & (WAVE - 1)) == 0) red[t / WAVE] = m;
+    __syncthreads();
+    if (t == 0) {
+      unsigned long long mm = 0;
+      for (int k = 0; k < (int)(blockDim.x / WAVE); k++) mm = red[k] > mm ? red[k] : mm;
+      c->vmax_acc = mm;
+    }
+  }
+  if (t == 0) {
+    ctrl_decide(c);
+    reb = c->rebuild;
+  }
+  const int nn = s.n_nonwall;
+  for (int q = t; q < nn; q += blockDim.x) {
+    const int64_t cn = s.P[s.nonwall_pid[q]].center;
+    for (int d = 0; d < 3; d++) {
+      s.pcx[d * nn + q] = s.x[d] ? s.x[d][cn] : 0.;
+      s.pcv[d * nn + q] = s.v[d] ? s.v[d][cn] : 0.;
+    }
+  }
+  __syncthreads();
+  if (reb)
+    for (int64_t k = t; k < s.ncells; k += blockDim.x) s.cell_count[k] = 0;
+}
+
 // ------------------------------------------------------------ node update
 // integrateCD (demModel.cpp:349-390): semi-implicit Euler on free dofs.
 template <int DIM>
-__global__ void k_integrate(DS s) {
+__global__ void __launch_bounds__(256) k_integrate(DS s, int multi) {
   int64_t i = gid();
   Ctrl* ct = s.ctrl;
   const double dt = ct->dt;
-  if (i == 0) {
-    ct->current_dt = dt;
-    ct->n += 1;
-    ct->time += dt;
-    ct->vdirty = 1;
-  }
   double vm = 0.;
   if (i < s.N) {
     const DevParticle& p = s.P[s.ptid[i]];
@@ -248,14 +389,36 @@ __global__ void k_integrate(DS s) {
       s.vmag[i] = vm;
     }
   }
-  // global max |v| for the search cadence: wave max, one atomic per wave;
-  // every lane takes part in the shuffles
+  if (!multi) {
+    if (i == 0) {
+      ct->current_dt = dt;
+      ct->n += 1;
+      ct->time += dt;
+    }
+    return;
+  }
+  // block maximum of |v| (non-negative doubles order like their bit patterns),
+  // reduced over blocks by the controller
+  __shared__ unsigned long long wmax[256 / WAVE];
   unsigned long long b = (unsigned long long)__double_as_longlong(vm);
   for (int off = WAVE / 2; off > 0; off >>= 1) {
     unsigned long long o = __shfl_xor(b, off);
     b = o > b ? o : b;
   }
-  if ((threadIdx.x & (WAVE - 1)) == 0 && b > 0) atomicMax(&ct->vmax_acc, b);
+  const int w = threadIdx.x / WAVE;
+  if ((threadIdx.x & (WAVE - 1)) == 0) wmax[w] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = 0;
+    for (int k = 0; k < (int)(blockDim.x / WAVE); k++) m = wmax[k] > m ? wmax[k] : m;
+    s.blockmax[blockIdx.x] = m;
+    if (blockIdx.x == 0) {
+      ct->current_dt = dt;
+      ct->n += 1;
+      ct->time += dt;
+      ct->vdirty = 1;
+    }
+  }
 }
 
 // ---------------------------------------------------------- boundary data
@@ -346,13 +509,23 @@ __global__ void k_fbc(DS s, DevBc bc) {
 }
 
 // -------------------------------------------------------- PD bond forces
+template <int INF>
+__device__ __forceinline__ double infJ(const DS& s, double r_over_delta, double rji, double horizon) {
+  if (INF == 0) return s.inf_p0;
+  if (INF == 1) return s.inf_p0 + s.inf_p1 * (rji / horizon);
+  const double r = rji / horizon;
+  return s.inf_p0 * exp(-r * r / s.inf_p1);
+}
+
 // Bond-based RNP: computePeridynamicForces force pass (demModel.cpp:682-788)
 // with RnpMaterial::getBondEF (material.h:342-368).  One lane per node; the
 // force sum runs over the neighbor list in order, as the reference's does.
-template <int DIM>
-__global__ void __launch_bounds__(256) k_pd_rnp(DS s) {
+// The next neighbor's data is loaded while the current bond is evaluated.
+template <int DIM, int INF>
+__global__ void __launch_bounds__(256) k_pd_rnp(DS s, int bin) {
   int64_t i = gid();
   if (i >= s.N) return;
+  if (bin && s.ctrl->rebuild) bin_node<DIM>(s, i);
   const DevParticle& p = s.P[s.ptid[i]];
   if (!p.compute_force) {
     for (int d = 0; d < DIM; d++) s.f[d][i] = 0.;
@@ -365,7 +538,7 @@ __global__ void __launch_bounds__(256) k_pd_rnp(DS s) {
   const bool irrev = p.irreversible != 0;
   const double Kn = p.Kn, Rc = p.Rc;
   const int64_t sl = i / WAVE, lane = i % WAVE;
-  const int* c = s.col + s.sptr[sl] + lane;
+  const int* __restrict__ c = s.col + s.sptr[sl] + lane;
   unsigned* bw = s.bits + s.bptr[sl] + lane;
   const int n = s.deg[i];
   double xi[3], ui[3];
@@ -376,41 +549,61 @@ __global__ void __launch_bounds__(256) k_pd_rnp(DS s) {
   double force[3] = {0., 0., 0.};
   float Zi = 0.f;
   unsigned word = 0, wnew = 0;
+  double xn[3], un[3], vn = 0.;
+  if (n > 0) {
+    const int j0 = c[0];
+    for (int d = 0; d < DIM; d++) {
+      xn[d] = s.xr[d][j0];
+      un[d] = s.u[d][j0];
+    }
+    vn = s.vol[j0];
+  }
   for (int k = 0; k < n; k++) {
     if ((k & 31) == 0) {
       if (k > 0 && wnew != word) bw[(int64_t)((k >> 5) - 1) * WAVE] = wnew;
       word = bw[(int64_t)(k >> 5) * WAVE];
       wnew = word;
     }
-    const int j = c[(int64_t)k * WAVE];
+    double xj[3], uj[3];
+    for (int d = 0; d < DIM; d++) {
+      xj[d] = xn[d];
+      uj[d] = un[d];
+    }
+    double volj = vn;
+    if (k + 1 < n) {
+      const int jn = c[(int64_t)(k + 1) * WAVE];
+      for (int d = 0; d < DIM; d++) {
+        xn[d] = s.xr[d][jn];
+        un[d] = s.u[d][jn];
+      }
+      vn = s.vol[jn];
+    }
     bool fs = (word >> (k & 31)) & 1u;
     double dx[3], du[3];
     for (int d = 0; d < DIM; d++) {
-      dx[d] = s.xr[d][j] - xi[d];
-      du[d] = s.u[d][j] - ui[d];
+      dx[d] = xj[d] - xi[d];
+      du[d] = uj[d] - ui[d];
     }
-    double volj = s.vol[j];
     const double rji = len3<DIM>(dx);
     const double Sji = dot3<DIM>(dx, du) / dot3<DIM>(dx, dx);
+    const double Sc = rbar / sqrt(rji);
     if (!fs) {
       if (isGreater(rji, check_low)) volj *= (check_up - rji) / h;
-      const double J = infFn(s, rji / horizon);
-      if (irrev && isGreater(fabs(Sji), fsc * (rbar / sqrt(rji)))) fs = true;
+      const double J = infJ<INF>(s, 0., rji, horizon);
+      if (irrev && isGreater(fabs(Sji), fsc * Sc)) fs = true;
       double ef = 0.;
       if (!fs) ef = J * 4. * Sji * Cc * beta * exp(-beta * rji * Sji * Sji) / invF;
       if (fs) wnew |= 1u << (k & 31);
       const double scalar_f = ef * volj;
-      const double l = len3<DIM>(dx);
-      for (int d = 0; d < DIM; d++) force[d] += scalar_f * (dx[d] / l);
+      for (int d = 0; d < DIM; d++) force[d] += scalar_f * (dx[d] / rji);
     } else {
       double y[3];
-      for (int d = 0; d < DIM; d++) y[d] = (s.xr[d][j] + s.u[d][j]) - (xi[d] + ui[d]);
+      for (int d = 0; d < DIM; d++) y[d] = (xj[d] + uj[d]) - (xi[d] + ui[d]);
       const double R = len3<DIM>(y);
       double scalar_f = Kn * volj * (R - Rc) / R;
       if (scalar_f > 0.) scalar_f = 0.;
       for (int d = 0; d < DIM; d++) force[d] += scalar_f * y[d];
     }
-    const double Sc = rbar / sqrt(rji);
     if (isGreater(fabs(Sji / Sc), (double)Zi)) Zi = (float)fabs(Sji / Sc);
   }
   if (n > 0 && wnew != word) bw[(int64_t)((n - 1) >> 5) * WAVE] = wnew;
@@ -420,10 +613,11 @@ __global__ void __launch_bounds__(256) k_pd_rnp(DS s) {
 
 // State-based dilation pass (demModel.cpp:610-679): bond breaking on the
 // actual stretch, theta_i = 3 sum r (|y|-r) J V^c / m_i.
-template <int DIM>
-__global__ void __launch_bounds__(256) k_pd_dilation(DS s) {
+template <int DIM, int INF>
+__global__ void __launch_bounds__(256) k_pd_dilation(DS s, int bin) {
   int64_t i = gid();
   if (i >= s.N) return;
+  if (bin && s.ctrl->rebuild) bin_node<DIM>(s, i);
   const DevParticle& p = s.P[s.ptid[i]];
   if (!p.compute_force || p.material_type != 1) return;
   const double horizon = p.horizon, h = p.h;
@@ -464,7 +658,7 @@ __global__ void __launch_bounds__(256) k_pd_dilation(DS s) {
     if (!fs) {
       double volj = s.vol[j];
       if (isGreater(rji, check_low)) volj *= (check_up - rji) / h;
-      theta += rji * change_length * infFn(s, rji / horizon) * volj;
+      theta += rji * change_length * infJ<INF>(s, 0., rji, horizon) * volj;
     }
   }
   if (n > 0 && wnew != word) bw[(int64_t)((n - 1) >> 5) * WAVE] = wnew;
@@ -472,10 +666,11 @@ __global__ void __launch_bounds__(256) k_pd_dilation(DS s) {
 }
 
 // State-based force pass with PdState::getBondEF for both ends (material.h:1415-1429).
-template <int DIM>
-__global__ void __launch_bounds__(256) k_pd_state(DS s) {
+template <int DIM, int INF>
+__global__ void __launch_bounds__(256) k_pd_state(DS s, int bin) {
   int64_t i = gid();
   if (i >= s.N) return;
+  if (bin && s.ctrl->rebuild) bin_node<DIM>(s, i);
   const DevParticle& p = s.P[s.ptid[i]];
   if (!p.compute_force) {
     for (int d = 0; d < DIM; d++) s.f[d][i] = 0.;
@@ -517,7 +712,7 @@ __global__ void __launch_bounds__(256) k_pd_state(DS s) {
     const double Sji = (ly - len3<DIM>(dx)) / len3<DIM>(dx);
     if (!fs) {
       if (isGreater(rji, check_low)) volj *= (check_up - rji) / h;
-      const double J = infFn(s, rji / horizon);
+      const double J = infJ<INF>(s, 0., rji, horizon);
       const double mj = s.mx[j], thetaj = s.theta[j];
       const double cl = Sji * rji;
       const double ef_i = J * (rji * thetai * factor_i + cl * alpha_i);
@@ -540,153 +735,39 @@ __global__ void __launch_bounds__(256) k_pd_state(DS s) {
   s.Z[i] = Zi;
 }
 
-// ---------------------------------------------- contact search controller
-// DEMModel::updateContactNeighborSearchParameters (demModel.cpp:1808-1955),
-// one thread; decides whether this step rebuilds the candidate lists.
-__global__ void k_ctrl(DS s) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  Ctrl* c = s.ctrl;
-  if (c->vdirty) {
-    c->vmax_last = __longlong_as_double((long long)c->vmax_acc);
-    c->vmax_acc = 0ull;
-    c->vdirty = 0;
-  }
-  bool upd;
-  const int64_t deckI = c->deck_interval;
-  if (c->interval == 0 && isLess(c->radius, 1.e-16)) {
-    c->interval = deckI;
-    c->counter = c->n % c->interval;
-    c->radius = c->max_contact_r * c->s_factor;
-  }
-  if (c->n == 0 && (c->init_calls + 1) <= 2) {
-    c->init_calls++;
-    if (c->init_calls == 1) {
-      upd = true;
-    } else {
-      c->counter++;
-      upd = (c->counter - 1) % c->interval == 0;
-    }
-  } else {
-    if (c->n == 0) c->init_calls++;
-    if (c->interval == 1) {
-      c->radius = c->max_contact_r;
-      c->counter++;
-      upd = (c->counter - 1) % c->interval == 0;
-    } else {
-      int64_t upi = c->interval > 5 ? (int64_t)(0.2 * (double)c->interval) : 1;
-      if (c->counter > 0 && c->counter % upi != 0) {
-        c->counter++;
-        upd = (c->counter - 1) % c->interval == 0;
-      } else {
-        c->max_velocity = c->vmax_last;
-        double safety = c->s_factor > 5 ? c->s_factor : 10;
-        double r_contact = c->s_factor * c->max_contact_r;
-        double r_vel = c->max_velocity * c->current_dt * (double)deckI * safety;
-        if (isGreater(r_vel, r_contact)) {
-          c->interval = (int64_t)(c->max_contact_r / (c->max_velocity * c->current_dt));
-          c->radius = r_contact;
-          c->counter = 0;
-          if (c->interval < 1) {
-            c->interval = 1;
-            c->radius = c->max_contact_r;
-          }
-        } else {
-          c->radius = c->interval < 2 ? c->max_contact_r : r_contact;
-        }
-        c->counter++;
-        upd = (c->counter - 1) % c->interval == 0;
-      }
-    }
-  }
-  c->rebuild = upd ? 1 : 0;
-  if (upd) c->n_rebuilds++;
-}
-
 // ---------------------------------------------------------- cell list
+// Uniform grid with cell >= the largest search radius; each cell holds up to
+// cell_cap node ids in arbitrary order (candidate sets do not depend on it).
 __device__ __forceinline__ int cellc(double a, double lo, double cell, int n) {
   double k = floor((a - lo) / cell);
   int ki = k < 0. ? 0 : (k > (double)(n - 1) ? n - 1 : (int)k);
   return ki;
 }
 
-__global__ void k_cell_clear(DS s) {
-  if (!s.ctrl->rebuild) return;
-  for (int64_t c = gid(); c <= s.ncells; c += (int64_t)gridDim.x * blockDim.x) s.cell_count[c] = 0;
-}
-
 template <int DIM>
-__global__ void k_cell_count(DS s) {
-  if (!s.ctrl->rebuild) return;
-  int64_t i = gid();
-  if (i >= s.N) return;
+__device__ void bin_node(const DS& s, int64_t i) {
   int cx = cellc(s.x[0][i], s.glo[0], s.cell, s.gn[0]);
   int cy = cellc(s.x[1][i], s.glo[1], s.cell, s.gn[1]);
   int cz = DIM == 3 ? cellc(s.x[2][i], s.glo[2], s.cell, s.gn[2]) : 0;
   int c = (cz * s.gn[1] + cy) * s.gn[0] + cx;
-  s.node_cell[i] = c;
-  s.node_slot[i] = atomicAdd(&s.cell_count[c], 1);
+  int slot = atomicAdd(&s.cell_count[c], 1);
+  if (slot < s.cell_cap)
+    s.cell_nodes[(int64_t)c * s.cell_cap + slot] = (int)i;
+  else
+    s.ctrl->overflow = 2;
 }
 
-// exclusive scan of ncells counts, one workgroup of 1024 threads
-__global__ void __launch_bounds__(1024) k_cell_scan(DS s) {
-  if (!s.ctrl->rebuild) return;
-  __shared__ int part[1024];
-  const int64_t n = s.ncells;
-  const int t = threadIdx.x;
-  const int64_t chunk = (n + 1023) / 1024;
-  const int64_t b = t * chunk, e = b + chunk < n ? b + chunk : n;
-  int sum = 0;
-  for (int64_t c = b; c < e; c++) sum += s.cell_count[c];
-  part[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = t >= off ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int run = t > 0 ? part[t - 1] : 0;
-  for (int64_t c = b; c < e; c++) {
-    int v = s.cell_count[c];
-    s.cell_start[c] = run;
-    run += v;
-  }
-  if (t == 1023) s.cell_start[n] = part[1023];
-}
-
-__global__ void k_cell_scatter(DS s) {
+template <int DIM>
+__global__ void k_bin(DS s) {
   if (!s.ctrl->rebuild) return;
   int64_t i = gid();
-  if (i >= s.N) return;
-  s.cell_nodes[s.cell_start[s.node_cell[i]] + s.node_slot[i]] = (int)i;
+  if (i < s.N) bin_node<DIM>(s, i);
 }
 
-// ascending node order inside each cell (deterministic candidate order)
-__global__ void k_cell_sort(DS s) {
-  if (!s.ctrl->rebuild) return;
-  int64_t c = gid();
-  if (c >= s.ncells) return;
-  int b = s.cell_start[c], e = s.cell_start[c + 1];
-  for (int a = b + 1; a < e; a++) {
-    int key = s.cell_nodes[a];
-    int q = a - 1;
-    while (q >= b && s.cell_nodes[q] > key) {
-      s.cell_nodes[q + 1] = s.cell_nodes[q];
-      q--;
-    }
-    s.cell_nodes[q + 1] = key;
-  }
-}
-
-// radiusSearchExcludeTag for searching nodes (demModel.cpp:1717-1756):
+// radiusSearchExcludeTag for one searching node (demModel.cpp:1717-1756):
 // other particles, j != i, 0 + dx^2 + dy^2 + dz^2 < r^2.
 template <int DIM>
-__global__ void __launch_bounds__(256) k_contact_query(DS s) {
-  if (!s.ctrl->rebuild) return;
-  int64_t i = gid();
-  if (i >= s.N) return;
-  const int pi = s.ptid[i];
-  if (!s.P[pi].searches) return;
+__device__ void query_node(const DS& s, int64_t i, int pi) {
   const double r = s.ctrl->radius, r2 = r * r;
   double xi[3] = {0., 0., 0.};
   for (int d = 0; d < DIM; d++) xi[d] = s.x[d][i];
@@ -701,9 +782,11 @@ __global__ void __launch_bounds__(256) k_contact_query(DS s) {
     for (int y = (cy > 0 ? cy - 1 : 0); y <= (cy < s.gn[1] - 1 ? cy + 1 : cy); y++)
       for (int x = (cx > 0 ? cx - 1 : 0); x <= (cx < s.gn[0] - 1 ? cx + 1 : cx); x++) {
         const int cc = (z * s.gn[1] + y) * s.gn[0] + x;
-        const int b = s.cell_start[cc], e = s.cell_start[cc + 1];
-        for (int q = b; q < e; q++) {
-          const int j = s.cell_nodes[q];
+        int m = s.cell_count[cc];
+        m = m < s.cell_cap ? m : s.cell_cap;
+        const int* cn = s.cell_nodes + (int64_t)cc * s.cell_cap;
+        for (int q = 0; q < m; q++) {
+          const int j = cn[q];
           if (j == i || s.ptid[j] == pi) continue;
           double d2 = 0.;
           for (int d = 0; d < DIM; d++) {
@@ -723,72 +806,154 @@ __global__ void __launch_bounds__(256) k_contact_query(DS s) {
   s.ccount[i] = cnt;
 }
 
-// clear candidate counts of nodes that do not search (only at a rebuild)
-__global__ void k_contact_clear(DS s) {
-  if (!s.ctrl->rebuild) return;
-  int64_t i = gid();
-  if (i >= s.N) return;
-  if (!s.P[s.ptid[i]].searches) s.ccount[i] = 0;
+// one active contact pair's two force terms on node i (demModel.cpp:868-904):
+// the normal term sf en and the friction term (mu sf) et; the reference adds
+// them to the node's force one after the other
+template <int DIM>
+__device__ __forceinline__ void contact_pair(const DS& s, const double* vi, int j,
+                                             const DevContact& ct, double Rji, const double* yji,
+                                             double* fn, double* ft) {
+  const double volj = s.vol[j];
+  double vji[3], en[3], et[3];
+  for (int d = 0; d < DIM; d++) {
+    vji[d] = s.v[d][j] - vi[d];
+    en[d] = yji[d] / Rji;
+  }
+  const double vn_mag = dot3<DIM>(vji, en);
+  for (int d = 0; d < DIM; d++) et[d] = vji[d] - vn_mag * en[d];
+  const double etl = len3<DIM>(et);
+  if (isGreater(etl, 0.)) {
+    for (int d = 0; d < DIM; d++) et[d] = et[d] / etl;
+  } else {
+    for (int d = 0; d < DIM; d++) et[d] = 0.;
+  }
+  double scalar_f = ct.Kn * (Rji - ct.contact_r) * volj;
+  if (scalar_f > 0.) scalar_f = 0.;
+  const double mf = ct.mu * scalar_f;
+  for (int d = 0; d < DIM; d++) {
+    fn[d] = scalar_f * en[d];
+    ft[d] = mf * et[d];
+  }
 }
 
 // ------------------------------------------------------- contact forces
-// computeContactForces node loop (demModel.cpp:832-934).  Also records the
-// wall nodes within contact distance of each particle for the damping pass
-// (the condensed wall list, demModel.cpp:998-1022).
+// Candidate query on a rebuild: one lane per searching node; nodes that end
+// with candidates (and compute forces) are appended to the active list.
 template <int DIM>
-__global__ void __launch_bounds__(256) k_contact_force(DS s) {
+__global__ void __launch_bounds__(256) k_query(DS s) {
+  if (!s.ctrl->rebuild) return;
   int64_t i = gid();
   if (i >= s.N) return;
-  const int n = s.ccount[i];
-  if (n == 0) return;
   const int pid = s.ptid[i];
   const DevParticle& pi = s.P[pid];
-  if (!pi.compute_force) return;
-  double yi[3], vi[3];
-  for (int d = 0; d < DIM; d++) {
-    yi[d] = s.x[d][i];
-    vi[d] = s.v[d][i];
-  }
-  double force[3] = {0., 0., 0.};
-  const int nwi = pi.nonwall_index;
-  for (int k = 0; k < n; k++) {
-    const int j = s.cand[(int64_t)k * s.N + i];
-    const int pjid = s.ptid[j];
-    const DevParticle& pj = s.P[pjid];
-    double yji[3];
-    for (int d = 0; d < DIM; d++) yji[d] = s.x[d][j] - yi[d];
-    const double Rji = len3<DIM>(yji);
-    const bool both_walls = pi.type_index == 1 && pj.type_index == 1;
-    if (j == i || pjid == pid || both_walls) continue;
-    const DevContact ct = s.CT[pi.zone * s.nz + pj.zone];
-    if (!isLess(Rji, ct.contact_r)) continue;
-    if (pj.type_index == 1 && nwi >= 0) {
-      const int w = s.widx[j];
-      atomicOr(&s.wbits[(int64_t)nwi * s.nwords + (w >> 5)], 1u << (w & 31));
-      atomicMin(&s.wlo[nwi], w >> 5);
-      atomicMax(&s.whi[nwi], w >> 5);
-    }
-    const double volj = s.vol[j];
-    double vji[3], en[3], et[3];
+  if (!pi.searches) return;
+  query_node<DIM>(s, i, pid);
+  if (s.ccount[i] > 0 && pi.compute_force) s.active[atomicAdd(&s.ctrl->nactive, 1u)] = (int)i;
+}
+
+// computeContactForces node loop (demModel.cpp:832-934), one wavefront per
+// node of the active list: lanes test candidates in parallel, the active
+// pairs' terms go to LDS, and one lane adds them in ascending node order (the
+// order of the oracle's sorted candidate lists).  Wall nodes within contact
+// distance are recorded for the damping pass (demModel.cpp:998-1022).
+constexpr int CWAVES = 4;      // wavefronts per workgroup
+constexpr int MAXACT = 64;     // active pairs per node held in LDS
+
+template <int DIM>
+__global__ void __launch_bounds__(CWAVES * WAVE) k_contact_force(DS s) {
+  __shared__ int sj[CWAVES][MAXACT];
+  __shared__ double st[CWAVES][MAXACT][6];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = threadIdx.x / WAVE;
+  const int nwaves = gridDim.x * CWAVES;
+  const int nact_nodes = (int)s.ctrl->nactive;
+  for (int a = blockIdx.x * CWAVES + wv; a < nact_nodes; a += nwaves) {
+    const int i = s.active[a];
+    const int n = s.ccount[i];
+    const int pid = s.ptid[i];
+    const int mi = s.pmeta[pid];
+    const int zi = mi & 0xffff;
+    const bool wi = (mi >> 16) != 0;
+    const int nwi = s.P[pid].nonwall_index;
+    double yi[3], vi[3];
     for (int d = 0; d < DIM; d++) {
-      vji[d] = s.v[d][j] - vi[d];
-      en[d] = yji[d] / Rji;
+      yi[d] = s.x[d][i];
+      vi[d] = s.v[d][i];
     }
-    const double vn_mag = dot3<DIM>(vji, en);
-    for (int d = 0; d < DIM; d++) et[d] = vji[d] - vn_mag * en[d];
-    const double etl = len3<DIM>(et);
-    if (isGreater(etl, 0.)) {
-      for (int d = 0; d < DIM; d++) et[d] = et[d] / etl;
-    } else {
-      for (int d = 0; d < DIM; d++) et[d] = 0.;
+    int cnt = 0;
+    for (int base = 0; base < n; base += WAVE) {
+      const int k = base + lane;
+      bool on = false;
+      int j = 0;
+      double term[6] = {0., 0., 0., 0., 0., 0.};
+      if (k < n) {
+        j = s.cand[(int64_t)k * s.N + i];
+        const int pjid = s.ptid[j];
+        const int mj = s.pmeta[pjid];
+        const bool wj = (mj >> 16) != 0;
+        if (!(j == i || pjid == pid || (wi && wj))) {
+          double yji[3];
+          for (int d = 0; d < DIM; d++) yji[d] = s.x[d][j] - yi[d];
+          const DevContact ct = s.CT[zi * s.nz + (mj & 0xffff)];
+          // d^2 >= rc^2 implies R >= rc (1 - 2^-53): isLess(R, rc) is false
+          const double d2 = dot3<DIM>(yji, yji);
+          if (d2 < ct.contact_r * ct.contact_r) {
+            const double Rji = len3<DIM>(yji);
+            if (isLess(Rji, ct.contact_r)) {
+              on = true;
+              if (wj && nwi >= 0) {
+                const int w = s.widx[j];
+                atomicOr(&s.wbits[(int64_t)nwi * s.nwords + (w >> 5)], 1u << (w & 31));
+                atomicMin(&s.wlo[nwi], w >> 5);
+                atomicMax(&s.whi[nwi], w >> 5);
+              }
+              contact_pair<DIM>(s, vi, j, ct, Rji, yji, term, term + 3);
+            }
+          }
+        }
+      }
+      const unsigned long long m = __ballot(on);
+      if (on) {
+        const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
+        if (pos < MAXACT) {
+          sj[wv][pos] = j;
+          for (int d = 0; d < 6; d++) st[wv][pos][d] = term[d];
+        }
+      }
+      cnt += __popcll(m);
     }
-    double scalar_f = ct.Kn * (Rji - ct.contact_r) * volj;
-    if (scalar_f > 0.) scalar_f = 0.;
-    for (int d = 0; d < DIM; d++) force[d] += scalar_f * en[d];
-    const double mf = ct.mu * scalar_f;
-    for (int d = 0; d < DIM; d++) force[d] += mf * et[d];
+    // LDS is private to this wavefront's slot and LDS instructions of one
+    // wavefront complete in order: no workgroup barrier is needed (the waves
+    // of a workgroup run different trip counts of this loop)
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0 && cnt > 0) {
+      if (cnt > MAXACT) {
+        s.ctrl->overflow = 3;
+        cnt = MAXACT;
+      }
+      // ascending node order: insertion sort of the (few) active pairs
+      for (int q = 1; q < cnt; q++) {
+        const int kj = sj[wv][q];
+        double kt[6];
+        for (int d = 0; d < 6; d++) kt[d] = st[wv][q][d];
+        int r = q - 1;
+        while (r >= 0 && sj[wv][r] > kj) {
+          sj[wv][r + 1] = sj[wv][r];
+          for (int d = 0; d < 6; d++) st[wv][r + 1][d] = st[wv][r][d];
+          r--;
+        }
+        sj[wv][r + 1] = kj;
+        for (int d = 0; d < 6; d++) st[wv][r + 1][d] = kt[d];
+      }
+      double force[3] = {0., 0., 0.};
+      for (int q = 0; q < cnt; q++) {
+        for (int d = 0; d < DIM; d++) force[d] += st[wv][q][d];
+        for (int d = 0; d < DIM; d++) force[d] += st[wv][q][3 + d];
+      }
+      for (int d = 0; d < DIM; d++) s.f[d][i] += force[d];
+    }
+    __builtin_amdgcn_wave_barrier();
   }
-  for (int d = 0; d < DIM; d++) s.f[d][i] += force[d];
 }
 
 // Damping (demModel.cpp:937-1052): one wavefront per non-wall particle.
@@ -797,41 +962,41 @@ __global__ void __launch_bounds__(256) k_contact_force(DS s) {
 template <int DIM>
 __global__ void __launch_bounds__(64) k_damping(DS s) {
   const int q = blockIdx.x;
-  if (q >= s.n_nonwall) return;
+  const int nn = s.n_nonwall;
+  if (q >= nn) return;
   const int lane = threadIdx.x;
   const int pid = s.nonwall_pid[q];
-  const DevParticle& pi = s.P[pid];
-  const double Ri = pi.Rb;
+  const double Ri = s.prad[q];
   const double vol_pi = M_PI * Ri * Ri;
-  const double rhoi = pi.rho;
+  const double rhoi = s.prho[q];
+  const int zi = s.pzone[q];
   double xc[3], vc[3];
   for (int d = 0; d < DIM; d++) {
-    xc[d] = s.x[d][pi.center];
-    vc[d] = s.v[d][pi.center];
+    xc[d] = s.pcx[d * nn + q];
+    vc[d] = s.pcv[d * nn + q];
   }
   double force[3] = {0., 0., 0.};
-  for (int base = 0; base < s.n_nonwall; base += WAVE) {
+  for (int base = 0; base < nn; base += WAVE) {
     const int qq = base + lane;
     double term[3] = {0., 0., 0.};
     bool on = false;
-    if (qq < s.n_nonwall && qq != q) {
-      const DevParticle& pj = s.P[s.nonwall_pid[qq]];
-      const double Rj = pj.Rb;
+    if (qq < nn && qq != q) {
+      const double Rj = s.prad[qq];
       double xcj[3];
-      for (int d = 0; d < DIM; d++) xcj[d] = s.x[d][pj.center] - xc[d];
+      for (int d = 0; d < DIM; d++) xcj[d] = s.pcx[d * nn + qq] - xc[d];
       const double dist = len3<DIM>(xcj);
-      const DevContact ct = s.CT[pi.zone * s.nz + pj.zone];
+      const DevContact ct = s.CT[zi * s.nz + s.pzone[qq]];
       if (isLess(dist, Rj + Ri + 1.01 * ct.contact_r)) {
         on = true;
         const double vol_pj = M_PI * Rj * Rj;
-        const double m1 = rhoi * vol_pi, m2 = pj.rho * vol_pj;
+        const double m1 = rhoi * vol_pi, m2 = s.prho[qq] * vol_pj;
         const double meq = 2. * m1 * m2 / (m1 + m2);
         const double beta_n = ct.betan * sqrt(ct.kappa * ct.contact_r * meq);
         double hat[3] = {0., 0., 0.};
         if (isGreater(dist, 0.))
           for (int d = 0; d < DIM; d++) hat[d] = xcj[d] / dist;
         double vcj[3];
-        for (int d = 0; d < DIM; d++) vcj[d] = s.v[d][pj.center] - vc[d];
+        for (int d = 0; d < DIM; d++) vcj[d] = s.pcv[d * nn + qq] - vc[d];
         double vc_mag = dot3<DIM>(vcj, hat);
         if (vc_mag > 0.) vc_mag = 0.;
         for (int d = 0; d < DIM; d++) term[d] = ((beta_n * vc_mag) * hat[d]) / vol_pi;
@@ -855,9 +1020,8 @@ __global__ void __launch_bounds__(64) k_damping(DS s) {
         const int b = __ffs(word) - 1;
         word &= word - 1;
         const int j = s.wall_gid[w * 32 + b];
-        const DevParticle& pj = s.P[s.ptid[j]];
         const double meq = rhoi * vol_pi;
-        const DevContact ct = s.CT[pi.zone * s.nz + pj.zone];
+        const DevContact ct = s.CT[zi * s.nz + (s.pmeta[s.ptid[j]] & 0xffff)];
         const double beta_n = ct.betan * sqrt(ct.kappa * ct.contact_r * meq);
         double xcj[3], hat[3] = {0., 0., 0.}, vcj[3];
         for (int d = 0; d < DIM; d++) xcj[d] = s.x[d][j] - xc[d];
