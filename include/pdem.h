@@ -228,6 +228,9 @@ int pdem_broken_bonds(pdem_ctx *ctx, int64_t *broken, int64_t *total);
 int pdem_profile_enable(pdem_ctx *ctx, int32_t on);
 int pdem_profile_get(pdem_ctx *ctx, double *pd_kernel_ms_avg, int64_t *pd_kernel_launches);
 
+/* number of HIP devices visible to the engine (0 when none); no device work */
+int pdem_device_count(void);
+
 /* sizeof of the ABI structures, for binding checks:
  * 0 pdem_desc, 1 pdem_particle, 2 pdem_contact_pair, 3 pdem_bc_set, 4 pdem_contact_state */
 int64_t pdem_abi_sizeof(int32_t which);

@@ -324,12 +324,19 @@ def engine_library():
             fn = getattr(lib, "pdem_" + nm)
             fn.restype = C.c_int
             fn.argtypes = [C.c_void_p]
+        lib.pdem_device_count.restype = C.c_int
+        lib.pdem_device_count.argtypes = []
         lib.pdem_profile_enable.restype = C.c_int
         lib.pdem_profile_enable.argtypes = [C.c_void_p, C.c_int32]
         lib.pdem_profile_get.restype = C.c_int
         lib.pdem_profile_get.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         _engine_lib = lib
     return _engine_lib
+
+
+def device_count() -> int:
+    """HIP devices visible to the engine library (0 without a GPU)."""
+    return int(engine_library().pdem_device_count())
 
 
 class GpuEngine(Engine):

@@ -59,7 +59,7 @@ struct pdem_ctx {
   bool state_any = false, rnp_any = false;
   int64_t cap = 0;
   MixedArgs mixed{};
-  int ntiles = 0;
+  int ntiles = 0, ntiles_global = 0;
   bool graph_ready = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -112,8 +112,13 @@ struct Launch {
     if (c->precision == PDEM_PREC_MIXED && c->ntiles > 0) {
       hipLaunchKernelGGL(k_bin_zero<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
       if (c->rnp_any) hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(nb), dim3(TPB), 0, st, c->ds, 0);
-      hipLaunchKernelGGL((k_mixed_dilation<DIM, INF>), dim3(c->ntiles), dim3(MIXED_TPB), 0, st, c->ds,
-                         c->mixed);
+      const MixedArgs& ma = c->mixed;
+      if (ma.ntiles_lds)
+        hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, true>), dim3(ma.ntiles_lds), dim3(MIXED_TPB), 0,
+                           st, c->ds, ma);
+      if (ma.ntiles_hbm)
+        hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, false>), dim3(ma.ntiles_hbm), dim3(MIXED_TPB), 0,
+                           st, c->ds, ma);
       std::pair<hipEvent_t, hipEvent_t> ev{};
       if (c->profiling) {
         if (c->ev_free.empty()) {
@@ -126,8 +131,12 @@ struct Launch {
         c->ev_free.pop_back();
         HIPCHK(hipEventRecord(ev.first, st));
       }
-      hipLaunchKernelGGL((k_mixed_force<DIM, INF>), dim3(c->ntiles), dim3(MIXED_TPB), 0, st, c->ds,
-                         c->mixed);
+      if (ma.ntiles_lds)
+        hipLaunchKernelGGL((k_mixed_force<DIM, INF, true>), dim3(ma.ntiles_lds), dim3(MIXED_TPB), 0, st,
+                           c->ds, ma);
+      if (ma.ntiles_hbm)
+        hipLaunchKernelGGL((k_mixed_force<DIM, INF, false>), dim3(ma.ntiles_hbm), dim3(MIXED_TPB), 0, st,
+                           c->ds, ma);
       if (c->profiling) {
         HIPCHK(hipEventRecord(ev.second, st));
         c->ev_used.push_back(ev);
@@ -177,6 +186,7 @@ struct Launch {
     hipLaunchKernelGGL(k_query<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
   }
   static void contact_force_kernel(pdem_ctx* c) {
+    hipLaunchKernelGGL(k_contact_light<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds);
     const int64_t waves = std::min<int64_t>(c->N, 8192);
     hipLaunchKernelGGL(k_contact_force<DIM>, dim3((unsigned)((waves + CWAVES - 1) / CWAVES)),
                        dim3(CWAVES * WAVE), 0, c->stream, c->ds);
@@ -430,8 +440,15 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     for (auto& q : c->hp)
       if (q.glob_end - q.glob_start > 65536)
         throw HipError{"mixed precision needs particles of at most 65536 nodes"};
-    uint16_t* col16 = c->dalloc<uint16_t>(c->nnz_padded);
-    hipLaunchKernelGGL(k_col16, dim3(blocks(N)), dim3(TPB), 0, c->stream, s, col16);
+    std::vector<int64_t> sp16(c->nslices + 1, 0);
+    for (int64_t sl = 0; sl < c->nslices; sl++) {
+      const int64_t w = (c->h_sptr[sl + 1] - c->h_sptr[sl]) / WAVE;
+      sp16[sl + 1] = sp16[sl] + ((w + KB - 1) / KB) * KB * WAVE;
+    }
+    int64_t* d_sp16 = c->upload(sp16.data(), sp16.size());
+    uint16_t* col16 = c->dalloc<uint16_t>(sp16[c->nslices] + 8);
+    hipLaunchKernelGGL(k_col16, dim3(blocks(N)), dim3(TPB), 0, c->stream, s, col16, d_sp16);
+    c->mixed.sptr16 = d_sp16;
     int* dlo = c->dalloc<int>(N);
     int* dhi = c->dalloc<int>(N);
     hipLaunchKernelGGL(k_nbr_range, dim3(blocks(N)), dim3(TPB), 0, c->stream, s, dlo, dhi);
@@ -444,20 +461,41 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     for (int p = 0; p < c->np; p++) {
       const auto& q = c->hp[p];
       if (!q.compute_force || q.material_type != PDEM_MAT_PDSTATE) continue;
-      for (int64_t ts = q.glob_start; ts < q.glob_end; ts += TILE_NODES) {
-        const int64_t te = std::min<int64_t>(q.glob_end, ts + TILE_NODES);
-        int ws = (int)ts, we = (int)te;
+      // largest tile (power of two, 64..TILE_NODES) whose windows all fit in LDS
+      auto window = [&](int64_t ts, int64_t te, int& ws, int& we) {
+        ws = (int)ts;
+        we = (int)te;
         for (int64_t i = ts; i < te; i++) {
           ws = std::min(ws, lo[i]);
           we = std::max(we, hi[i] + 1);
         }
+      };
+      int64_t tile = TILE_NODES;
+      for (; tile > 64; tile /= 2) {
+        bool fits = true;
+        for (int64_t ts = q.glob_start; ts < q.glob_end && fits; ts += tile) {
+          int ws, we;
+          window(ts, std::min<int64_t>(q.glob_end, ts + tile), ws, we);
+          fits = (we - ws) <= LDS_NODES;
+        }
+        if (fits) break;
+      }
+      for (int64_t ts = q.glob_start; ts < q.glob_end; ts += tile) {
+        const int64_t te = std::min<int64_t>(q.glob_end, ts + tile);
+        int ws, we;
+        window(ts, te, ws, we);
         tiles.push_back(DevTile{(int)ts, (int)te, ws, we, p, (we - ws) <= LDS_NODES ? 1 : 0});
       }
     }
+    // LDS tiles first, then the tiles that gather from HBM
+    std::stable_sort(tiles.begin(), tiles.end(), [](const DevTile& a, const DevTile& b) { return a.lds > b.lds; });
     c->ntiles = (int)tiles.size();
+    c->ntiles_global = 0;
+    for (auto& t : tiles) c->ntiles_global += t.lds ? 0 : 1;
+    c->mixed.ntiles_lds = c->ntiles - c->ntiles_global;
+    c->mixed.ntiles_hbm = c->ntiles_global;
     if (tiles.empty()) tiles.push_back(DevTile{0, 0, 0, 1, 0, 0});
     c->mixed.tiles = c->upload(tiles.data(), tiles.size());
-    c->mixed.ntiles = c->ntiles;
     c->mixed.col16 = col16;
     c->mixed.fa = c->dalloc<float>(N);
     c->mixed.fb = c->dalloc<float>(N);
@@ -465,8 +503,10 @@ void build(pdem_ctx* c, const pdem_desc* d) {
 
   // ---- contact structures
   if (c->multi) {
-    // uniform grid: cell >= every search radius the controller can pick
-    double cell = std::max(1.0, d->search_factor) * maxR;
+    // uniform grid: csub cells span the largest search radius the controller
+    // can pick; the query scans +-csub cells around a node
+    s.csub = dim == 3 ? 2 : 1;
+    double cell = std::max(1.0, d->search_factor) * maxR / s.csub;
     if (!(cell > 0.)) cell = 1.;
     double lo[3], hi[3];
     for (int k = 0; k < 3; k++) {
@@ -517,6 +557,7 @@ void build(pdem_ctx* c, const pdem_desc* d) {
       s.cell_cap = capc;
     }
     s.cell_count = c->dalloc<int>(ncells + 1);
+    s.cell_owner = c->dalloc<int>(ncells + 1);
     s.cell_nodes = c->dalloc<int>((size_t)ncells * s.cell_cap, false);
     // candidate capacity: nodes of a ball of the largest search radius at the
     // finest mesh spacing
@@ -530,6 +571,7 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     s.cap = (int)cap;
     s.ccount = c->dalloc<int>(N);
     s.active = c->dalloc<int>(N);
+    s.light = c->dalloc<int>(N);
     s.cand = c->dalloc<int>((size_t)cap * N, false);
     // damping bookkeeping: wall node compact ids, per-particle wall bitmaps
     std::vector<int> widx(N, -1), wgid;
@@ -572,6 +614,12 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     std::vector<int> meta(c->np);
     for (int p = 0; p < c->np; p++) meta[p] = (c->hp[p].zone & 0xffff) | ((c->hp[p].type_index == 1) << 16);
     s.pmeta = c->upload(meta.data(), meta.size());
+    double rc = 0.;
+    for (int k = 0; k < c->nz * c->nz && c->multi; k++) rc = std::max(rc, d->contact[k].contact_r);
+    s.rc2_max = rc * rc * (1.0 + 1e-12);
+    // 2D lists are short and few: a wavefront per node wins; 3D has many
+    // nodes with a handful of candidates: a lane per node wins
+    s.light_max = dim == 3 ? LIGHT_MAX_C : 0;
     s.nblockmax = (int)blocks(N);
     s.blockmax = c->dalloc<unsigned long long>(s.nblockmax);
   }
@@ -636,6 +684,12 @@ void ensure_graph(pdem_ctx* c) {
 
 // ======================================================================= ABI
 extern "C" {
+
+int pdem_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
 
 int64_t pdem_abi_sizeof(int32_t which) {
   switch (which) {

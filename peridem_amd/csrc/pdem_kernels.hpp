@@ -22,6 +22,7 @@
 namespace pdem {
 
 constexpr int WAVE = 64;
+constexpr int LIGHT_MAX_C = 16;  // candidate count up to which one lane takes a node
 constexpr double COMPARE_EPS = 1e-5;
 
 struct DevParticle {
@@ -54,7 +55,8 @@ struct Ctrl {
   int32_t rebuild;
   int32_t overflow;
   int32_t vdirty;
-  uint32_t nactive;  // nodes with a non-empty candidate list (compute-force particles)
+  uint32_t nactive;  // nodes with more than LIGHT_MAX candidates (wavefront per node)
+  uint32_t nlight;   // nodes with 1..LIGHT_MAX candidates (lane per node)
   unsigned long long vmax_acc;  // bit pattern of max |v| (non-negative doubles order as integers)
   double vmax_last;
   // scratch results
@@ -89,15 +91,18 @@ struct DS {
   int cap;
   int* ccount;
   int* cand;
-  int* active;  // [N] node ids with candidates, appended at each rebuild
+  int* active;  // [N] node ids with many candidates, appended at each rebuild
+  int* light;   // [N] node ids with few candidates
   // cell grid
   double glo[3];
   double cell;
   int gn[3];
+  int csub;  // cells per largest search radius: the query scans +-csub cells
   int64_t ncells;
   int cell_cap;     // fixed capacity of a cell
   int* cell_count;  // [ncells]
   int* cell_nodes;  // [ncells][cell_cap], unordered inside a cell
+  int* cell_owner;  // [ncells] particle id when one particle fills the cell, -1 mixed, -2 empty
   // damping
   int n_nonwall;
   const int* nonwall_pid;
@@ -116,6 +121,8 @@ struct DS {
   const int* pzone;    // [n_nonwall] zone
   // per particle: zone | wall << 16, for the contact loops
   const int* pmeta;
+  double rc2_max;  // largest contact radius squared: distance prefilter
+  int light_max;   // nodes with at most this many candidates take one lane
   // node-update block maxima of |v|
   unsigned long long* blockmax;
   int nblockmax;
@@ -217,15 +224,16 @@ __global__ void k_nbr_range(DS s, int* lo, int* hi) {
   hi[i] = b;
 }
 
-// neighbor ids relative to the particle's first node, 16 bit
-__global__ void k_col16(DS s, uint16_t* col16) {
+// neighbor ids relative to the particle's first node, 16 bit, in blocks of
+// 8 per lane (layout of the mixed-precision kernels)
+__global__ void k_col16(DS s, uint16_t* col16, const int64_t* sptr16) {
   int64_t i = gid();
   if (i >= s.N) return;
   const int64_t sl = i / WAVE, lane = i % WAVE;
   const int64_t gs = s.P[s.ptid[i]].glob_start;
   for (int k = 0; k < s.deg[i]; k++) {
     const int64_t o = s.sptr[sl] + (int64_t)k * WAVE + lane;
-    col16[o] = (uint16_t)(s.col[o] - gs);
+    col16[sptr16[sl] + (int64_t)(k / 8) * WAVE * 8 + lane * 8 + (k % 8)] = (uint16_t)(s.col[o] - gs);
   }
 }
 
@@ -317,6 +325,7 @@ __device__ void ctrl_decide(Ctrl* cg) {
   if (upd) {
     c->n_rebuilds++;
     c->nactive = 0u;
+    c->nlight = 0u;
   }
   *cg = h;
 }
@@ -358,7 +367,10 @@ __global__ void __launch_bounds__(1024) k_ctrl(DS s) {
   }
   __syncthreads();
   if (reb)
-    for (int64_t k = t; k < s.ncells; k += blockDim.x) s.cell_count[k] = 0;
+    for (int64_t k = t; k < s.ncells; k += blockDim.x) {
+      s.cell_count[k] = 0;
+      s.cell_owner[k] = -2;
+    }
 }
 
 // ------------------------------------------------------------ node update
@@ -750,6 +762,9 @@ __device__ void bin_node(const DS& s, int64_t i) {
   int cy = cellc(s.x[1][i], s.glo[1], s.cell, s.gn[1]);
   int cz = DIM == 3 ? cellc(s.x[2][i], s.glo[2], s.cell, s.gn[2]) : 0;
   int c = (cz * s.gn[1] + cy) * s.gn[0] + cx;
+  const int pid = s.ptid[i];
+  const int old = atomicCAS(&s.cell_owner[c], -2, pid);
+  if (old != -2 && old != pid) s.cell_owner[c] = -1;
   int slot = atomicAdd(&s.cell_count[c], 1);
   if (slot < s.cell_cap)
     s.cell_nodes[(int64_t)c * s.cell_cap + slot] = (int)i;
@@ -771,17 +786,31 @@ __device__ void query_node(const DS& s, int64_t i, int pi) {
   const double r = s.ctrl->radius, r2 = r * r;
   double xi[3] = {0., 0., 0.};
   for (int d = 0; d < DIM; d++) xi[d] = s.x[d][i];
-  const int cx = cellc(xi[0], s.glo[0], s.cell, s.gn[0]);
-  const int cy = cellc(xi[1], s.glo[1], s.cell, s.gn[1]);
-  const int cz = DIM == 3 ? cellc(xi[2], s.glo[2], s.cell, s.gn[2]) : 0;
+  int cc0[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int d = 0; d < DIM; d++) {
+    cc0[d] = cellc(xi[d], s.glo[d], s.cell, s.gn[d]);
+    lo[d] = cc0[d] - s.csub > 0 ? cc0[d] - s.csub : 0;
+    hi[d] = cc0[d] + s.csub < s.gn[d] - 1 ? cc0[d] + s.csub : s.gn[d] - 1;
+  }
   int cnt = 0;
   const int cap = s.cap;
-  const int z0 = DIM == 3 ? (cz > 0 ? cz - 1 : 0) : 0;
-  const int z1 = DIM == 3 ? (cz < s.gn[2] - 1 ? cz + 1 : cz) : 0;
-  for (int z = z0; z <= z1; z++)
-    for (int y = (cy > 0 ? cy - 1 : 0); y <= (cy < s.gn[1] - 1 ? cy + 1 : cy); y++)
-      for (int x = (cx > 0 ? cx - 1 : 0); x <= (cx < s.gn[0] - 1 ? cx + 1 : cx); x++) {
+  for (int z = lo[2]; z <= hi[2]; z++)
+    for (int y = lo[1]; y <= hi[1]; y++)
+      for (int x = lo[0]; x <= hi[0]; x++) {
+        // skip cells whose box lies beyond the search radius (clamped edge
+        // cells hold nodes outside the grid: never skipped)
+        const int c3[3] = {x, y, z};
+        double dmin2 = 0.;
+        bool edge = false;
+        for (int d = 0; d < DIM; d++) {
+          if (c3[d] == 0 || c3[d] == s.gn[d] - 1) edge = true;
+          const double b0 = s.glo[d] + c3[d] * s.cell, b1 = b0 + s.cell;
+          const double t = xi[d] < b0 ? b0 - xi[d] : (xi[d] > b1 ? xi[d] - b1 : 0.);
+          dmin2 += t * t;
+        }
+        if (!edge && dmin2 > r2 * 1.0000001) continue;
         const int cc = (z * s.gn[1] + y) * s.gn[0] + x;
+        if (s.cell_owner[cc] == pi) continue;  // only the node's own particle
         int m = s.cell_count[cc];
         m = m < s.cell_cap ? m : s.cell_cap;
         const int* cn = s.cell_nodes + (int64_t)cc * s.cell_cap;
@@ -848,7 +877,90 @@ __global__ void __launch_bounds__(256) k_query(DS s) {
   const DevParticle& pi = s.P[pid];
   if (!pi.searches) return;
   query_node<DIM>(s, i, pid);
-  if (s.ccount[i] > 0 && pi.compute_force) s.active[atomicAdd(&s.ctrl->nactive, 1u)] = (int)i;
+  const int cnt = s.ccount[i];
+  if (cnt > 0 && pi.compute_force) {
+    if (cnt <= s.light_max)
+      s.light[atomicAdd(&s.ctrl->nlight, 1u)] = (int)i;
+    else
+      s.active[atomicAdd(&s.ctrl->nactive, 1u)] = (int)i;
+  }
+}
+
+// computeContactForces node loop for nodes with few candidates: one lane per
+// node.  The active pairs' ids (all of them, kept sorted in registers) fix the
+// summation order: ascending node id, the order of the oracle's lists.
+constexpr int NACT = LIGHT_MAX_C;  // every candidate of a light node fits
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_contact_light(DS s) {
+  const int nl = (int)s.ctrl->nlight;
+  for (int a = (int)gid(); a < nl; a += gridDim.x * blockDim.x) {
+    const int i = s.light[a];
+    const int n = s.ccount[i];
+    const int pid = s.ptid[i];
+    const int mi = s.pmeta[pid];
+    const int zi = mi & 0xffff;
+    const bool wi = (mi >> 16) != 0;
+    const int nwi = s.P[pid].nonwall_index;
+    double yi[3], vi[3];
+    for (int d = 0; d < DIM; d++) {
+      yi[d] = s.x[d][i];
+      vi[d] = s.v[d][i];
+    }
+    int act[NACT];
+#pragma unroll
+    for (int q = 0; q < NACT; q++) act[q] = INT_MAX;
+    int nact = 0;
+    int jn = s.cand[i];
+    for (int k = 0; k < n; k++) {
+      const int j = jn;
+      if (k + 1 < n) jn = s.cand[(int64_t)(k + 1) * s.N + i];
+      double yji[3];
+      for (int d = 0; d < DIM; d++) yji[d] = s.x[d][j] - yi[d];
+      // distance prefilter before any particle data is loaded
+      const double d2 = dot3<DIM>(yji, yji);
+      if (d2 >= s.rc2_max) continue;
+      const int pjid = s.ptid[j];
+      const int mj = s.pmeta[pjid];
+      const bool wj = (mj >> 16) != 0;
+      if (j == i || pjid == pid || (wi && wj)) continue;
+      const double rc = s.CT[zi * s.nz + (mj & 0xffff)].contact_r;
+      // d^2 >= rc^2 implies R >= rc (1 - 2^-53): isLess(R, rc) is false
+      if (d2 >= rc * rc) continue;
+      if (!isLess(len3<DIM>(yji), rc)) continue;
+      if (wj && nwi >= 0) {
+        const int w = s.widx[j];
+        atomicOr(&s.wbits[(int64_t)nwi * s.nwords + (w >> 5)], 1u << (w & 31));
+        atomicMin(&s.wlo[nwi], w >> 5);
+        atomicMax(&s.whi[nwi], w >> 5);
+      }
+      nact++;
+      int v = j;
+#pragma unroll
+      for (int q = 0; q < NACT; q++) {
+        const int lo = v < act[q] ? v : act[q];
+        const int hi = v < act[q] ? act[q] : v;
+        act[q] = lo;
+        v = hi;
+      }
+    }
+    if (nact == 0) continue;
+    double force[3] = {0., 0., 0.};
+    for (int r = 0; r < nact; r++) {
+      int j = act[0];
+#pragma unroll
+      for (int q = 0; q < NACT; q++)
+        if (q == r) j = act[q];
+      const int mj = s.pmeta[s.ptid[j]];
+      const DevContact ct = s.CT[zi * s.nz + (mj & 0xffff)];
+      double yji[3], fn[3], ft[3];
+      for (int d = 0; d < DIM; d++) yji[d] = s.x[d][j] - yi[d];
+      contact_pair<DIM>(s, vi, j, ct, len3<DIM>(yji), yji, fn, ft);
+      for (int d = 0; d < DIM; d++) force[d] += fn[d];
+      for (int d = 0; d < DIM; d++) force[d] += ft[d];
+    }
+    for (int d = 0; d < DIM; d++) s.f[d][i] += force[d];
+  }
 }
 
 // computeContactForces node loop (demModel.cpp:832-934), one wavefront per
