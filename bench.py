@@ -4,7 +4,10 @@ A step is one central-difference time step (node update, displacement BC,
 peridynamic forces, contact search at its natural cadence, contact and
 damping forces, external forces) over the whole synthetic model, with all
 state resident in HBM.  Default workload: the 2D compressive test with 500
-particles (~400 nodes each, ~200k nodes), FP64.
+particles (~400 nodes each, ~200k nodes), FP64.  Other workloads: the 3D
+packing of 2000 spheres (mixed precision) and a batched ensemble of 512
+independent 2D simulations in one model.  Before the warm-up the launch
+shape of the bond-force pass is autotuned (outside the timed regions).
 
 Prints one JSON line.  value = (nodes integrated per step) x steps / seconds
 of the timed region (HIP-graph replay of the step).  The roofline entry is
@@ -36,6 +39,8 @@ def workload(name: str):
         return configs.packing_3d(n_spheres=2000, nodes_per_sphere=1000, seed=0)
     if name == "two_particle":
         return configs.two_particle()
+    if name == "ensemble_2d":
+        return configs.ensemble_2d(n_instances=512, seed=0)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -118,6 +123,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--prof-steps", type=int, default=300)
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU (particle-sharded) "
@@ -134,6 +140,11 @@ def main():
     print(f"# workload {spec.name}: {spec.n_nodes} nodes, {spec.n_particles} particles, "
           f"setup {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
     eng.integrate_begin()
+    tuned = None
+    if not args.no_autotune:
+        tms = eng.autotune(reps=5)
+        tuned = dict(zip(("pd_tpb", "mixed_tpb", "tile_nodes"), eng.launch_config()), force_pass_ms=round(tms, 5))
+        print(f"# autotune: {tuned}", file=sys.stderr, flush=True)
     eng.integrate_cd_graph(args.warmup)
     eng.synchronize()
     t0 = time.perf_counter()
@@ -192,7 +203,7 @@ def main():
                    "precision": args.precision, "step_mode": "hip graph replay",
                    "sim_time_end": t, "contact_rebuilds": st["n_rebuilds"],
                    "contact_interval": st["interval"], "contact_candidates": n_cand,
-                   "broken_bond_fraction": broken / max(1, total)},
+                   "broken_bond_fraction": broken / max(1, total), "launch": tuned},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

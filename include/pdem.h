@@ -228,6 +228,15 @@ int pdem_broken_bonds(pdem_ctx *ctx, int64_t *broken, int64_t *total);
 int pdem_profile_enable(pdem_ctx *ctx, int32_t on);
 int pdem_profile_get(pdem_ctx *ctx, double *pd_kernel_ms_avg, int64_t *pd_kernel_launches);
 
+/* Launch-shape autotuner for the peridynamic force pass. No counterpart in
+ * the reference (its OpenMP loops have no launch shape). Times `reps` force
+ * passes on the current state per candidate (FP64 kernels: workgroup size
+ * 64/128/256; mixed tile kernels: workgroup 256/1024 x tile 256/512/1024
+ * nodes), keeps the fastest and re-captures the step graph. Results do not
+ * depend on the choice: forces are written per node, bond bits per lane. */
+int pdem_autotune(pdem_ctx *ctx, int32_t reps, double *best_ms);
+int pdem_get_launch_config(pdem_ctx *ctx, int32_t *pd_tpb, int32_t *mixed_tpb, int32_t *tile_nodes);
+
 /* number of HIP devices visible to the engine (0 when none); no device work */
 int pdem_device_count(void);
 

@@ -330,6 +330,10 @@ def engine_library():
         lib.pdem_profile_enable.argtypes = [C.c_void_p, C.c_int32]
         lib.pdem_profile_get.restype = C.c_int
         lib.pdem_profile_get.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        lib.pdem_autotune.restype = C.c_int
+        lib.pdem_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
+        lib.pdem_get_launch_config.restype = C.c_int
+        lib.pdem_get_launch_config.argtypes = [C.c_void_p] + [C.POINTER(C.c_int32)] * 3
         _engine_lib = lib
     return _engine_lib
 
@@ -359,6 +363,18 @@ class GpuEngine(Engine):
         n = C.c_int64()
         self._check(self.lib.pdem_profile_get(self.ctx, C.byref(ms), C.byref(n)), "profile_get")
         return ms.value, n.value
+
+    def autotune(self, reps=5):
+        """Pick the fastest launch shape of the force pass; returns its ms."""
+        ms = C.c_double()
+        self._check(self.lib.pdem_autotune(self.ctx, reps, C.byref(ms)), "autotune")
+        return ms.value
+
+    def launch_config(self):
+        """(pd_tpb, mixed_tpb, tile_nodes) currently in use."""
+        v = [C.c_int32() for _ in range(3)]
+        self._check(self.lib.pdem_get_launch_config(self.ctx, *[C.byref(x) for x in v]), "launch_config")
+        return tuple(x.value for x in v)
 
     def update_contact_neighborlist(self):
         self._check(self.lib.pdem_update_contact_neighborlist(self.ctx), "update_contact_neighborlist")

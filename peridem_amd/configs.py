@@ -245,3 +245,116 @@ def single_particle_3d(R=1.0e-3, nodes=1000, state=True, stretch=0.0):
     b = ModelBuilder(3, 1e-8, zones, {}, multi_particle=False)
     b.add_particle(0, ball, bvol, cl, R)
     return b.build(name="single_3d")
+
+
+def ensemble_2d(n_instances=512, seed=0, dt=5.0e-8, n_steps=5000, R_lo=0.8e-3, R_hi=1.2e-3,
+                n_min=2, n_max=64, ratios=(6, 8, 10), speed_lo=0.05, speed_hi=1.0,
+                angle_max_deg=60.0, kn_factor=1.0e-2, pitch_factor=1.06):
+    """Config 3: a batch of independent 2D simulations in one model.
+
+    Instance i (seed + i): particle count ~ U[n_min, n_max], mesh ratio
+    h = R / r with r drawn from `ratios`, circles and hexagons alternating,
+    R ~ U[R_lo, R_hi], seeded jittered grid, every particle moving toward the
+    instance centre at speed ~ U[speed_lo, speed_hi] along a direction turned
+    by an angle ~ U[0, angle_max] (impact angle).  State-based material (the
+    reference's PDState deck values), mixed precision as config 2.
+
+    Instances occupy disjoint regions of one model, separated by far more
+    than the contact search radius plus the distance travelled, so no
+    contact, damping or candidate pair ever joins two instances: the batch is
+    the union of the independent simulations, advanced in one launch per
+    kernel.  Zones: one per (shape, mesh ratio); a zone pair's contact radius
+    is 0.95 times the finest mesh size of that ratio class, the value the
+    reference would compute for the instance alone."""
+    import numpy as _np
+
+    mat = pdstate_material()
+    zones = []
+    meshes = []
+    for r in ratios:
+        h_ref = 1.0e-3 / r
+        for shape in ("circle", "hexagon"):
+            zones.append(ZoneDeck(mat))
+            x, v = disk_mesh(1.0e-3, h_ref) if shape == "circle" else hexagon_mesh(1.0e-3, h_ref)
+            meshes.append((x, v, center_node(x, (0, 0, 0), 1.0e-3)))
+    nz = len(zones)
+    contact = {}
+    for a in range(nz):
+        for b in range(a, nz):
+            ra, rb = ratios[a // 2], ratios[b // 2]
+            h = 1.0e-3 / max(ra, rb) * R_lo / 1.0e-3
+            delta = 3.0 * h
+            contact[(a, b)] = ContactPairDeck(contact_radius_factor=None, contact_radius=0.95 * h,
+                                              Kn=_kn(mat, delta), Kn_factor=kn_factor, epsilon=0.7,
+                                              beta_n_factor=1.0, friction_coeff=0.5)
+    b = ModelBuilder(2, dt, zones, contact, search_factor=5.0, neigh_update_interval=40)
+    pitch = 2.0 * R_hi * pitch_factor
+    box = int(math.ceil(math.sqrt(n_max))) * pitch
+    spacing = box + 10.0e-3
+    ncol_inst = int(math.ceil(math.sqrt(n_instances)))
+    instances = []
+    if n_instances < 1:
+        raise ValueError("an ensemble needs at least one instance")
+    for k in range(n_instances):
+        rng = _np.random.default_rng(seed + k)
+        n_p = int(rng.integers(n_min, n_max + 1))
+        ri = int(rng.integers(0, len(ratios)))
+        ox = (k % ncol_inst) * spacing
+        oy = (k // ncol_inst) * spacing
+        ncol = int(math.ceil(math.sqrt(n_p)))
+        cxy = _np.array([ox + 0.5 * ncol * pitch, oy + 0.5 * int(math.ceil(n_p / ncol)) * pitch])
+        first = len(b.parts)
+        for q in range(n_p):
+            rad = rng.uniform(R_lo, R_hi)
+            jit = (pitch / 2 - rad) * 0.9
+            cx = ox + (q % ncol + 0.5) * pitch + rng.uniform(-jit, jit)
+            cy = oy + (q // ncol + 0.5) * pitch + rng.uniform(-jit, jit)
+            z = 2 * ri + (q % 2)
+            x, vol, cl = meshes[z]
+            s = rad / 1.0e-3
+            pid = b.add_particle(z, transform(x, (cx, cy, 0.0), rng.uniform(0, 2 * math.pi), s),
+                                 vol * s * s, cl, rad)
+            d = cxy - _np.array([cx, cy])
+            nd = _np.linalg.norm(d)
+            d = d / nd if nd > 0 else _np.array([1.0, 0.0])
+            ang = math.radians(rng.uniform(0.0, angle_max_deg)) * (1 if rng.random() < 0.5 else -1)
+            c, sn = math.cos(ang), math.sin(ang)
+            sp = rng.uniform(speed_lo, speed_hi)
+            vx, vy = sp * (c * d[0] - sn * d[1]), sp * (sn * d[0] + c * d[1])
+            b.ic.append(([pid], (vx, vy, 0.0)))
+        instances.append((first, len(b.parts), ratios[ri]))
+    spec = b.build(name=f"ensemble_2d_x{n_instances}", n_steps=n_steps)
+    spec.info.update(instances=instances, seed=seed)
+    return spec
+
+
+def ensemble_instance(spec, k):
+    """The k-th instance of an ensemble as a model of its own (same nodes,
+    constants and contact table), for running it separately."""
+    import copy
+
+    import numpy as _np
+
+    first, last, _ = spec.info["instances"][k]
+    p0, p1 = spec.particles[first], spec.particles[last - 1]
+    n0, n1 = p0["glob_start"], p1["glob_end"]
+    s = copy.copy(spec)
+    s.x_ref = spec.x_ref[n0:n1].copy()
+    s.u = spec.u[n0:n1].copy()
+    s.v = spec.v[n0:n1].copy()
+    s.vol = spec.vol[n0:n1].copy()
+    s.fix = spec.fix[n0:n1].copy()
+    s.pt_id = spec.pt_id[n0:n1] - first
+    s.n_nodes = int(n1 - n0)
+    parts = []
+    for p in spec.particles[first:last]:
+        q = dict(p)
+        q["glob_start"] -= n0
+        q["glob_end"] -= n0
+        q["center_node"] -= n0
+        parts.append(q)
+    s.particles = parts
+    s.ubc, s.fbc = [], []
+    s.info = dict(spec.info, instance=k)
+    s.name = f"{spec.name}_i{k}"
+    return s

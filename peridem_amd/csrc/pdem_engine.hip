@@ -60,6 +60,12 @@ struct pdem_ctx {
   int64_t cap = 0;
   MixedArgs mixed{};
   int ntiles = 0, ntiles_global = 0;
+  DevTile* d_tiles = nullptr;
+  size_t d_tiles_cap = 0;
+  int pd_tpb = 256;        // FP64 bond kernels' workgroup size (autotuned)
+  int mixed_tpb = 1024;    // mixed tile kernels' workgroup size (autotuned)
+  int tile_nodes = TILE_NODES;
+  std::vector<int> nbr_lo, nbr_hi;  // per-node neighbor id range (tile windows)
   bool graph_ready = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -105,69 +111,77 @@ namespace {
 
 template <int DIM>
 struct Launch {
+  // HIP events around the bond-force kernel when profiling
+  static std::pair<hipEvent_t, hipEvent_t> ev_begin(pdem_ctx* c) {
+    std::pair<hipEvent_t, hipEvent_t> ev{};
+    if (!c->profiling) return ev;
+    if (c->ev_free.empty()) {
+      hipEvent_t a, b;
+      HIPCHK(hipEventCreate(&a));
+      HIPCHK(hipEventCreate(&b));
+      c->ev_free.push_back({a, b});
+    }
+    ev = c->ev_free.back();
+    c->ev_free.pop_back();
+    HIPCHK(hipEventRecord(ev.first, c->stream));
+    return ev;
+  }
+  static void ev_end(pdem_ctx* c, std::pair<hipEvent_t, hipEvent_t> ev) {
+    if (!c->profiling) return;
+    HIPCHK(hipEventRecord(ev.second, c->stream));
+    c->ev_used.push_back(ev);
+  }
+  template <int INF, int T>
+  static void mixed_dilation(pdem_ctx* c) {
+    const MixedArgs& ma = c->mixed;
+    if (ma.ntiles_lds)
+      hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, true, T>), dim3(ma.ntiles_lds), dim3(T), 0, c->stream,
+                         c->ds, ma);
+    if (ma.ntiles_hbm)
+      hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, false, T>), dim3(ma.ntiles_hbm), dim3(T), 0,
+                         c->stream, c->ds, ma);
+  }
+  template <int INF, int T>
+  static void mixed_force(pdem_ctx* c) {
+    const MixedArgs& ma = c->mixed;
+    if (ma.ntiles_lds)
+      hipLaunchKernelGGL((k_mixed_force<DIM, INF, true, T>), dim3(ma.ntiles_lds), dim3(T), 0, c->stream,
+                         c->ds, ma);
+    if (ma.ntiles_hbm)
+      hipLaunchKernelGGL((k_mixed_force<DIM, INF, false, T>), dim3(ma.ntiles_hbm), dim3(T), 0, c->stream,
+                         c->ds, ma);
+  }
   template <int INF>
   static void pd_inf(pdem_ctx* c, int bin) {
     hipStream_t st = c->stream;
     const unsigned nb = blocks(c->N);
     if (c->precision == PDEM_PREC_MIXED && c->ntiles > 0) {
-      hipLaunchKernelGGL(k_bin_zero<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+      hipLaunchKernelGGL(k_bin_zero<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds, bin);
       if (c->rnp_any) hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(nb), dim3(TPB), 0, st, c->ds, 0);
-      const MixedArgs& ma = c->mixed;
-      if (ma.ntiles_lds)
-        hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, true>), dim3(ma.ntiles_lds), dim3(MIXED_TPB), 0,
-                           st, c->ds, ma);
-      if (ma.ntiles_hbm)
-        hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, false>), dim3(ma.ntiles_hbm), dim3(MIXED_TPB), 0,
-                           st, c->ds, ma);
-      std::pair<hipEvent_t, hipEvent_t> ev{};
-      if (c->profiling) {
-        if (c->ev_free.empty()) {
-          hipEvent_t a, b;
-          HIPCHK(hipEventCreate(&a));
-          HIPCHK(hipEventCreate(&b));
-          c->ev_free.push_back({a, b});
-        }
-        ev = c->ev_free.back();
-        c->ev_free.pop_back();
-        HIPCHK(hipEventRecord(ev.first, st));
-      }
-      if (ma.ntiles_lds)
-        hipLaunchKernelGGL((k_mixed_force<DIM, INF, true>), dim3(ma.ntiles_lds), dim3(MIXED_TPB), 0, st,
-                           c->ds, ma);
-      if (ma.ntiles_hbm)
-        hipLaunchKernelGGL((k_mixed_force<DIM, INF, false>), dim3(ma.ntiles_hbm), dim3(MIXED_TPB), 0, st,
-                           c->ds, ma);
-      if (c->profiling) {
-        HIPCHK(hipEventRecord(ev.second, st));
-        c->ev_used.push_back(ev);
-      }
+      if (c->mixed_tpb == 256)
+        mixed_dilation<INF, 256>(c);
+      else
+        mixed_dilation<INF, 1024>(c);
+      auto ev = ev_begin(c);
+      if (c->mixed_tpb == 256)
+        mixed_force<INF, 256>(c);
+      else
+        mixed_force<INF, 1024>(c);
+      ev_end(c, ev);
       return;
     }
-    std::pair<hipEvent_t, hipEvent_t> ev{};
-    if (c->profiling) {
-      if (c->ev_free.empty()) {
-        hipEvent_t a, b;
-        HIPCHK(hipEventCreate(&a));
-        HIPCHK(hipEventCreate(&b));
-        c->ev_free.push_back({a, b});
-      }
-      ev = c->ev_free.back();
-      c->ev_free.pop_back();
-    }
+    const unsigned pb = (unsigned)c->pd_tpb, pnb = blocks(c->N, c->pd_tpb);
     if (c->ds.state_global) {
-      hipLaunchKernelGGL((k_pd_dilation<DIM, INF>), dim3(nb), dim3(TPB), 0, st, c->ds, bin);
+      hipLaunchKernelGGL((k_pd_dilation<DIM, INF>), dim3(pnb), dim3(pb), 0, st, c->ds, bin);
       bin = 0;
     }
-    if (c->profiling) HIPCHK(hipEventRecord(ev.first, st));
+    auto ev = ev_begin(c);
     if (c->rnp_any || !c->state_any) {
-      hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(nb), dim3(TPB), 0, st, c->ds, bin);
+      hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(pnb), dim3(pb), 0, st, c->ds, bin);
       bin = 0;
     }
-    if (c->state_any) hipLaunchKernelGGL((k_pd_state<DIM, INF>), dim3(nb), dim3(TPB), 0, st, c->ds, bin);
-    if (c->profiling) {
-      HIPCHK(hipEventRecord(ev.second, st));
-      c->ev_used.push_back(ev);
-    }
+    if (c->state_any) hipLaunchKernelGGL((k_pd_state<DIM, INF>), dim3(pnb), dim3(pb), 0, st, c->ds, bin);
+    ev_end(c, ev);
   }
   static void pd_forces(pdem_ctx* c, int bin) {
     if (c->ds.inf_type == 0)
@@ -184,6 +198,7 @@ struct Launch {
     hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, st, c->ds);
     hipLaunchKernelGGL(k_bin<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
     hipLaunchKernelGGL(k_query<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+    damp_pairs(c);
   }
   static void contact_force_kernel(pdem_ctx* c) {
     hipLaunchKernelGGL(k_contact_light<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds);
@@ -194,6 +209,10 @@ struct Launch {
   static void damping(pdem_ctx* c) {
     if (c->ds.n_nonwall > 0)
       hipLaunchKernelGGL(k_damping<DIM>, dim3(c->ds.n_nonwall), dim3(64), 0, c->stream, c->ds);
+  }
+  static void damp_pairs(pdem_ctx* c) {
+    if (c->ds.n_nonwall > 0)
+      hipLaunchKernelGGL(k_damp_pairs<DIM>, dim3(c->ds.n_nonwall), dim3(64), 0, c->stream, c->ds);
   }
   static void contact_forces(pdem_ctx* c, bool apply_damping) {
     hipStream_t st = c->stream;
@@ -226,6 +245,7 @@ struct Launch {
     pd_forces(c, c->multi ? 1 : 0);
     if (c->multi) {
       hipLaunchKernelGGL(k_query<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds);
+      damp_pairs(c);
       contact_force_kernel(c);
       damping(c);
     }
@@ -271,11 +291,61 @@ void check_overflow(pdem_ctx* c) {
   if (h.overflow == 1)
     throw HipError{"contact candidate capacity exceeded (" + std::to_string(c->cap) +
                    " per node); create the model with a larger contact_capacity"};
+  if (h.overflow == 4)
+    throw HipError{"more than 256 damping partners within reach of one particle"};
   if (h.overflow == 3)
     throw HipError{"more than 64 active contact pairs at one node"};
   if (h.overflow == 2)
     throw HipError{"cell capacity exceeded (" + std::to_string(c->ds.cell_cap) +
                    " nodes per search cell): nodes crowded far beyond their initial packing"};
+}
+
+// Tiles of the mixed kernels: for every state-based particle, the largest
+// tile size (power of two, 64..tile_max) whose neighbor windows all fit in
+// LDS; LDS tiles first, then the tiles that gather from HBM.
+void build_tiles(pdem_ctx* c, int64_t tile_max) {
+  const auto& lo = c->nbr_lo;
+  const auto& hi = c->nbr_hi;
+  std::vector<DevTile> tiles;
+  for (int p = 0; p < c->np; p++) {
+    const auto& q = c->hp[p];
+    if (!q.compute_force || q.material_type != PDEM_MAT_PDSTATE) continue;
+    auto window = [&](int64_t ts, int64_t te, int& ws, int& we) {
+      ws = (int)ts;
+      we = (int)te;
+      for (int64_t i = ts; i < te; i++) {
+        ws = std::min(ws, lo[i]);
+        we = std::max(we, hi[i] + 1);
+      }
+    };
+    int64_t tile = tile_max;
+    for (; tile > 64; tile /= 2) {
+      bool fits = true;
+      for (int64_t ts = q.glob_start; ts < q.glob_end && fits; ts += tile) {
+        int ws, we;
+        window(ts, std::min<int64_t>(q.glob_end, ts + tile), ws, we);
+        fits = (we - ws) <= LDS_NODES;
+      }
+      if (fits) break;
+    }
+    for (int64_t ts = q.glob_start; ts < q.glob_end; ts += tile) {
+      const int64_t te = std::min<int64_t>(q.glob_end, ts + tile);
+      int ws, we;
+      window(ts, te, ws, we);
+      tiles.push_back(DevTile{(int)ts, (int)te, ws, we, p, (we - ws) <= LDS_NODES ? 1 : 0});
+    }
+  }
+  std::stable_sort(tiles.begin(), tiles.end(), [](const DevTile& a, const DevTile& b) { return a.lds > b.lds; });
+  if (tiles.size() > c->d_tiles_cap) throw HipError{"tile buffer too small"};
+  c->ntiles = (int)tiles.size();
+  c->ntiles_global = 0;
+  for (auto& t : tiles) c->ntiles_global += t.lds ? 0 : 1;
+  c->mixed.ntiles_lds = c->ntiles - c->ntiles_global;
+  c->mixed.ntiles_hbm = c->ntiles_global;
+  c->tile_nodes = (int)tile_max;
+  if (!tiles.empty())
+    HIPCHK(hipMemcpy(c->d_tiles, tiles.data(), tiles.size() * sizeof(DevTile), hipMemcpyHostToDevice));
+  c->mixed.tiles = c->d_tiles;
 }
 
 void build(pdem_ctx* c, const pdem_desc* d) {
@@ -453,49 +523,18 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     int* dhi = c->dalloc<int>(N);
     hipLaunchKernelGGL(k_nbr_range, dim3(blocks(N)), dim3(TPB), 0, c->stream, s, dlo, dhi);
     check_launch(c);
-    std::vector<int> lo(N), hi(N);
-    HIPCHK(hipMemcpyAsync(lo.data(), dlo, N * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(hi.data(), dhi, N * 4, hipMemcpyDeviceToHost, c->stream));
+    c->nbr_lo.resize(N);
+    c->nbr_hi.resize(N);
+    HIPCHK(hipMemcpyAsync(c->nbr_lo.data(), dlo, N * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->nbr_hi.data(), dhi, N * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    std::vector<DevTile> tiles;
-    for (int p = 0; p < c->np; p++) {
-      const auto& q = c->hp[p];
-      if (!q.compute_force || q.material_type != PDEM_MAT_PDSTATE) continue;
-      // largest tile (power of two, 64..TILE_NODES) whose windows all fit in LDS
-      auto window = [&](int64_t ts, int64_t te, int& ws, int& we) {
-        ws = (int)ts;
-        we = (int)te;
-        for (int64_t i = ts; i < te; i++) {
-          ws = std::min(ws, lo[i]);
-          we = std::max(we, hi[i] + 1);
-        }
-      };
-      int64_t tile = TILE_NODES;
-      for (; tile > 64; tile /= 2) {
-        bool fits = true;
-        for (int64_t ts = q.glob_start; ts < q.glob_end && fits; ts += tile) {
-          int ws, we;
-          window(ts, std::min<int64_t>(q.glob_end, ts + tile), ws, we);
-          fits = (we - ws) <= LDS_NODES;
-        }
-        if (fits) break;
-      }
-      for (int64_t ts = q.glob_start; ts < q.glob_end; ts += tile) {
-        const int64_t te = std::min<int64_t>(q.glob_end, ts + tile);
-        int ws, we;
-        window(ts, te, ws, we);
-        tiles.push_back(DevTile{(int)ts, (int)te, ws, we, p, (we - ws) <= LDS_NODES ? 1 : 0});
-      }
-    }
-    // LDS tiles first, then the tiles that gather from HBM
-    std::stable_sort(tiles.begin(), tiles.end(), [](const DevTile& a, const DevTile& b) { return a.lds > b.lds; });
-    c->ntiles = (int)tiles.size();
-    c->ntiles_global = 0;
-    for (auto& t : tiles) c->ntiles_global += t.lds ? 0 : 1;
-    c->mixed.ntiles_lds = c->ntiles - c->ntiles_global;
-    c->mixed.ntiles_hbm = c->ntiles_global;
-    if (tiles.empty()) tiles.push_back(DevTile{0, 0, 0, 1, 0, 0});
-    c->mixed.tiles = c->upload(tiles.data(), tiles.size());
+    // tile buffer sized for the smallest tile size the autotuner tries
+    int64_t maxt = 1;
+    for (auto& q : c->hp)
+      if (q.compute_force && q.material_type == PDEM_MAT_PDSTATE) maxt += (q.glob_end - q.glob_start + 63) / 64;
+    c->d_tiles = c->dalloc<DevTile>(maxt);
+    c->d_tiles_cap = maxt;
+    build_tiles(c, TILE_NODES);
     c->mixed.col16 = col16;
     c->mixed.fa = c->dalloc<float>(N);
     c->mixed.fb = c->dalloc<float>(N);
@@ -608,6 +647,9 @@ void build(pdem_ctx* c, const pdem_desc* d) {
       s.pzone = c->upload(zn.data(), nn);
       s.pcx = c->dalloc<double>(3 * (size_t)nn);
       s.pcv = c->dalloc<double>(3 * (size_t)nn);
+      s.dcap = std::min(nn, 256);
+      s.dcount = c->dalloc<int>(nn);
+      s.dlist = c->dalloc<int>((size_t)nn * s.dcap);
     }
   }
   {
@@ -1042,6 +1084,79 @@ int pdem_profile_get(pdem_ctx* c, double* ms, int64_t* n) {
     c->ev_used.clear();
     *ms = c->prof_n ? c->prof_ms / (double)c->prof_n : 0.;
     *n = c->prof_n;
+  });
+}
+
+int pdem_autotune(pdem_ctx* c, int32_t reps, double* best_ms) {
+  return guarded(c, [&] {
+    if (reps < 1) throw HipError{"reps must be positive"};
+    // Time one peridynamic force pass (idempotent: same positions, same
+    // bond bits) per candidate launch shape and keep the fastest.
+    auto time_pass = [&]() {
+      hipEvent_t a, b;
+      HIPCHK(hipEventCreate(&a));
+      HIPCHK(hipEventCreate(&b));
+      auto run = [&] { dispatch(c, [&] { Launch<2>::pd_forces(c, 0); }, [&] { Launch<3>::pd_forces(c, 0); }); };
+      run();  // warm-up
+      HIPCHK(hipEventRecord(a, c->stream));
+      for (int r = 0; r < reps; r++) run();
+      HIPCHK(hipEventRecord(b, c->stream));
+      HIPCHK(hipEventSynchronize(b));
+      check_launch(c);
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, a, b));
+      HIPCHK(hipEventDestroy(a));
+      HIPCHK(hipEventDestroy(b));
+      return (double)ms / reps;
+    };
+    const bool prof = c->profiling;
+    c->profiling = false;
+    double best = 1e300;
+    if (c->precision == PDEM_PREC_MIXED && c->ntiles > 0) {
+      int bt = c->mixed_tpb, bn = c->tile_nodes;
+      for (int tpb : {256, 1024})
+        for (int tn : {256, 512, 1024}) {
+          c->mixed_tpb = tpb;
+          build_tiles(c, tn);
+          const double ms = time_pass();
+          if (ms < best) {
+            best = ms;
+            bt = tpb;
+            bn = tn;
+          }
+        }
+      c->mixed_tpb = bt;
+      build_tiles(c, bn);
+    } else {
+      int bp = c->pd_tpb;
+      for (int tpb : {64, 128, 256}) {
+        c->pd_tpb = tpb;
+        const double ms = time_pass();
+        if (ms < best) {
+          best = ms;
+          bp = tpb;
+        }
+      }
+      c->pd_tpb = bp;
+    }
+    c->profiling = prof;
+    // launch shapes are baked into a captured step: capture again
+    if (c->graph_ready) {
+      HIPCHK(hipGraphExecDestroy(c->gexec));
+      HIPCHK(hipGraphDestroy(c->graph));
+      c->gexec = nullptr;
+      c->graph = nullptr;
+      c->graph_ready = false;
+    }
+    if (best_ms) *best_ms = best;
+  });
+}
+
+int pdem_get_launch_config(pdem_ctx* c, int32_t* pd_tpb, int32_t* mixed_tpb, int32_t* tile_nodes) {
+  return guarded(c, [&] {
+    *pd_tpb = c->pd_tpb;
+    *mixed_tpb = c->mixed_tpb;
+    *tile_nodes = c->tile_nodes;
   });
 }
 

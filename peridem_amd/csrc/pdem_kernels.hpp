@@ -119,6 +119,12 @@ struct DS {
   const double* prad;  // [n_nonwall] bounding radius
   const double* prho;  // [n_nonwall] density
   const int* pzone;    // [n_nonwall] zone
+  // damping partners per non-wall particle, rebuilt with the candidate lists:
+  // particles whose centres were within damping reach plus twice the search
+  // radius, ascending
+  int dcap;
+  int* dcount;      // [n_nonwall]
+  int* dlist;       // [n_nonwall][dcap] non-wall indices
   // per particle: zone | wall << 16, for the contact loops
   const int* pmeta;
   double rc2_max;  // largest contact radius squared: distance prefilter
@@ -1068,6 +1074,49 @@ __global__ void __launch_bounds__(CWAVES * WAVE) k_contact_force(DS s) {
   }
 }
 
+// Damping partners on a rebuild: one wavefront per non-wall particle, lanes
+// over the other particles, ballot-ordered append (ascending index).
+template <int DIM>
+__global__ void __launch_bounds__(64) k_damp_pairs(DS s) {
+  if (!s.ctrl->rebuild) return;
+  const int q = blockIdx.x;
+  const int nn = s.n_nonwall;
+  if (q >= nn) return;
+  const int lane = threadIdx.x;
+  const double Ri = s.prad[q];
+  const double margin = 2.0 * s.ctrl->radius;
+  const int zi = s.pzone[q];
+  double xc[3];
+  for (int d = 0; d < DIM; d++) xc[d] = s.pcx[d * nn + q];
+  int cnt = 0;
+  for (int base = 0; base < nn; base += WAVE) {
+    const int qq = base + lane;
+    bool on = false;
+    if (qq < nn && qq != q) {
+      double d2 = 0.;
+      for (int d = 0; d < DIM; d++) {
+        const double t = s.pcx[d * nn + qq] - xc[d];
+        d2 += t * t;
+      }
+      const double reach = Ri + s.prad[qq] + 1.01 * s.CT[zi * s.nz + s.pzone[qq]].contact_r + margin;
+      on = d2 < reach * reach;
+    }
+    const unsigned long long m = __ballot(on);
+    if (on) {
+      const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
+      if (pos < s.dcap) s.dlist[(int64_t)q * s.dcap + pos] = qq;
+    }
+    cnt += __popcll(m);
+  }
+  if (lane == 0) {
+    if (cnt > s.dcap) {
+      s.ctrl->overflow = 4;
+      cnt = s.dcap;
+    }
+    s.dcount[q] = cnt;
+  }
+}
+
 // Damping (demModel.cpp:937-1052): one wavefront per non-wall particle.
 // Particle-particle terms are evaluated by all lanes and accumulated in
 // ascending particle order; condensed wall nodes in ascending node order.
@@ -1088,11 +1137,13 @@ __global__ void __launch_bounds__(64) k_damping(DS s) {
     vc[d] = s.pcv[d * nn + q];
   }
   double force[3] = {0., 0., 0.};
-  for (int base = 0; base < nn; base += WAVE) {
-    const int qq = base + lane;
+  const int np_ = s.dcount[q];
+  const int* dl = s.dlist + (int64_t)q * s.dcap;
+  for (int base = 0; base < np_; base += WAVE) {
+    const int qq = base + lane < np_ ? dl[base + lane] : -1;
     double term[3] = {0., 0., 0.};
     bool on = false;
-    if (qq < nn && qq != q) {
+    if (qq >= 0) {
       const double Rj = s.prad[qq];
       double xcj[3];
       for (int d = 0; d < DIM; d++) xcj[d] = s.pcx[d * nn + qq] - xc[d];

@@ -8,7 +8,8 @@
 // Horizons never cross particles, so a tile's neighbors lie in a window of
 // node ids [ws, we) inside its particle.  LDS tiles (the window fits) stage
 // the window once per workgroup (structure of arrays, 48 B per node: FP32
-// position offset from the window's first reference node, FP64 displacement,
+// position offset from the particle's first reference node (independent of
+// the tile shape), FP64 displacement,
 // FP32 volume, FP32 dilation terms) and every bond reads LDS; the remaining
 // tiles gather from HBM through the caches (a separate instantiation).
 //
@@ -29,7 +30,7 @@ namespace pdem {
 
 constexpr int TILE_NODES = 1024;
 constexpr int LDS_NODES = 1280;   // 1280 x 48 B = 60 KB of LDS per workgroup
-constexpr int MIXED_TPB = 1024;
+constexpr int MIXED_TPB = 1024;   // default workgroup size (the autotuner may pick 256)
 constexpr int KB = 8;             // neighbor ids per 16-byte block
 
 struct DevTile {
@@ -111,8 +112,8 @@ __device__ __forceinline__ void stage_window(const DS& s, const MixedArgs& m, co
 }
 
 // Dilation pass (demModel.cpp:610-679) in mixed precision.
-template <int DIM, int INF, bool LDS>
-__global__ void __launch_bounds__(MIXED_TPB) k_mixed_dilation(DS s, MixedArgs m) {
+template <int DIM, int INF, bool LDS, int TPB>
+__global__ void __launch_bounds__(TPB) k_mixed_dilation(DS s, MixedArgs m) {
 #pragma clang fp contract(fast)
   __shared__ float lx[LDS ? 3 * LDS_NODES : 1];
   __shared__ double lu[LDS ? 3 * LDS_NODES : 1];
@@ -120,7 +121,7 @@ __global__ void __launch_bounds__(MIXED_TPB) k_mixed_dilation(DS s, MixedArgs m)
   const DevTile t = m.tiles[LDS ? blockIdx.x : m.ntiles_lds + blockIdx.x];
   const DevParticle& p = s.P[t.pid];
   Win<DIM, LDS> w{lx, lu, lv, nullptr, nullptr, &s, &m, t.ws, {0., 0., 0.}};
-  for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][t.ws];
+  for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][p.glob_start];
   if (LDS) stage_window<DIM, false>(s, m, t, lx, lu, lv, nullptr, nullptr, w.org);
   const float horizon = (float)p.horizon, h = (float)p.h;
   const float check_up = horizon + 0.5f * h, check_low = horizon - 0.5f * h;
@@ -197,8 +198,8 @@ __global__ void __launch_bounds__(MIXED_TPB) k_mixed_dilation(DS s, MixedArgs m)
 
 // Force pass (demModel.cpp:682-788, PdState::getBondEF material.h:1415-1429)
 // in mixed precision: ef_i + ef_j = J (r (b_i + b_j) + cl (a_i + a_j)).
-template <int DIM, int INF, bool LDS>
-__global__ void __launch_bounds__(MIXED_TPB) k_mixed_force(DS s, MixedArgs m) {
+template <int DIM, int INF, bool LDS, int TPB>
+__global__ void __launch_bounds__(TPB) k_mixed_force(DS s, MixedArgs m) {
 #pragma clang fp contract(fast)
   __shared__ float lx[LDS ? 3 * LDS_NODES : 1];
   __shared__ double lu[LDS ? 3 * LDS_NODES : 1];
@@ -208,7 +209,7 @@ __global__ void __launch_bounds__(MIXED_TPB) k_mixed_force(DS s, MixedArgs m) {
   const DevTile t = m.tiles[LDS ? blockIdx.x : m.ntiles_lds + blockIdx.x];
   const DevParticle& p = s.P[t.pid];
   Win<DIM, LDS> w{lx, lu, lv, la, lb, &s, &m, t.ws, {0., 0., 0.}};
-  for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][t.ws];
+  for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][p.glob_start];
   if (LDS) stage_window<DIM, true>(s, m, t, lx, lu, lv, la, lb, w.org);
   const float horizon = (float)p.horizon, h = (float)p.h;
   const float check_up = horizon + 0.5f * h, check_low = horizon - 0.5f * h;
@@ -284,10 +285,10 @@ __global__ void __launch_bounds__(MIXED_TPB) k_mixed_force(DS s, MixedArgs m) {
 
 // binning on a rebuild and the force reset of nodes that take no PD force
 template <int DIM>
-__global__ void k_bin_zero(DS s) {
+__global__ void k_bin_zero(DS s, int bin) {
   int64_t i = gid();
   if (i >= s.N) return;
-  if (s.ctrl->rebuild) bin_node<DIM>(s, i);
+  if (bin && s.ctrl->rebuild) bin_node<DIM>(s, i);
   if (!s.P[s.ptid[i]].compute_force)
     for (int d = 0; d < DIM; d++) s.f[d][i] = 0.;
 }

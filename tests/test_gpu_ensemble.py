@@ -1,0 +1,98 @@
+"""Batched ensemble: many independent 2D simulations advanced as one model.
+
+Instances sit far apart, so no contact, damping or candidate pair joins two
+of them and the batch must reproduce each instance run on its own: in FP64
+parity mode bit for bit (the contact sums run in ascending node id and the
+damping sums in ascending particle id in both runs; instances keep their
+relative order).  The oracle shows the same identity on the CPU
+(test_oracle_invariants.test_ensemble_instances_independent).  The launch
+shape autotuner must not change results beyond FP32 rounding of the mixed
+kernels' block sums (1e-6 of the largest force; in practice identical)."""
+import numpy as np
+import pytest
+
+from peridem_amd import capi, configs
+import oracle
+
+from parity_util import compare_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_ensemble(n=4, seed=5):
+    return configs.ensemble_2d(n_instances=n, n_min=2, n_max=6, ratios=(6,), speed_lo=2.0,
+                               speed_hi=4.0, pitch_factor=1.01, seed=seed)
+
+
+def _slice(spec, k):
+    first, last, _ = spec.info["instances"][k]
+    return spec.particles[first]["glob_start"], spec.particles[last - 1]["glob_end"]
+
+
+def test_ensemble_batched_equals_separate():
+    spec = _small_ensemble()
+    K = 300
+    print(f"[ensemble] nodes={spec.n_nodes} particles={len(spec.particles)} steps={K}", flush=True)
+    g = capi.GpuEngine(spec, capi.PREC_FP64)
+    g.integrate_begin()
+    g.integrate_cd_graph(K)
+    r = oracle.OracleEngine(spec, 8)
+    r.integrate_begin()
+    r.integrate_cd(K)
+    compare_state(g, r, tag="ensemble batched")
+    F, U, V = g.field(capi.FIELD_F), g.field(capi.FIELD_U), g.field(capi.FIELD_V)
+    assert np.abs(F).max() > 0.0
+    touched = 0
+    for k in range(len(spec.info["instances"])):
+        s = configs.ensemble_instance(spec, k)
+        e = capi.GpuEngine(s, capi.PREC_FP64)
+        e.integrate_begin()
+        e.integrate_cd_graph(K)
+        n0, n1 = _slice(spec, k)
+        np.testing.assert_array_equal(e.field(capi.FIELD_F), F[n0:n1])
+        np.testing.assert_array_equal(e.field(capi.FIELD_U), U[n0:n1])
+        np.testing.assert_array_equal(e.field(capi.FIELD_V), V[n0:n1])
+        touched += int(np.abs(F[n0:n1]).max() > 1e3)
+        print(f"[ensemble] instance {k} equal", flush=True)
+    assert touched >= 2, "too few instances reached contact in the test window"
+
+
+def test_ensemble_mixed_against_oracle():
+    spec = _small_ensemble(n=6, seed=11)
+    K = 200
+    print(f"[ensemble-mixed] nodes={spec.n_nodes} steps={K}", flush=True)
+    g = capi.GpuEngine(spec, capi.PREC_MIXED)
+    r = oracle.OracleEngine(spec, 8)
+    g.integrate_begin()
+    r.integrate_begin()
+    g.integrate_cd_graph(K)
+    r.integrate_cd(K)
+    fg, fr = g.field(capi.FIELD_F), r.field(capi.FIELD_F)
+    den = np.abs(fr).max()
+    emax = float(np.abs(fg - fr).max() / den)
+    erms = float(np.sqrt(((fg - fr) ** 2).sum(1).mean()) / den)
+    print(f"[ensemble-mixed] force max err {emax:.3e} rms {erms:.3e}", flush=True)
+    assert emax < 1e-4 and erms < 1e-5
+
+
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_MIXED])
+def test_autotune_keeps_results(precision):
+    spec = _small_ensemble(n=4, seed=7)
+    a = capi.GpuEngine(spec, precision)
+    b = capi.GpuEngine(spec, precision)
+    a.integrate_begin()
+    b.integrate_begin()
+    b.integrate_cd_graph(5)
+    a.integrate_cd_graph(5)
+    ms = b.autotune(reps=3)
+    cfg = b.launch_config()
+    print(f"[autotune] precision={precision} best {ms:.4f} ms config {cfg}", flush=True)
+    assert ms > 0.0
+    a.integrate_cd_graph(60)
+    b.integrate_cd_graph(60)
+    fa, fb = a.field(capi.FIELD_F), b.field(capi.FIELD_F)
+    if precision == capi.PREC_FP64:
+        np.testing.assert_array_equal(fa, fb)
+    else:
+        assert np.abs(fa - fb).max() <= 1e-6 * np.abs(fa).max()
+    np.testing.assert_array_equal(a.field(capi.FIELD_BONDS), b.field(capi.FIELD_BONDS))

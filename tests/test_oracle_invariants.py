@@ -1,7 +1,8 @@
 """Closed-form and invariant checks that pin the oracle where the reference's
 code cannot be run: homogeneous stretch, rigid motion, momentum balance,
 free fall, prescribed displacement, the compressive-test reaction at rest,
-bond breaking under uniform strain and the contact-search cadence."""
+bond breaking under uniform strain, the contact-search cadence and the
+independence of the instances of a batched ensemble."""
 import math
 
 import numpy as np
@@ -257,3 +258,26 @@ def test_contact_search_cadence_shrinks_with_speed():
     # interval = floor(maxR / (vmax dt)) (demModel.cpp:1898-1925)
     assert vmax > 0
     assert st["interval"] == max(1, int(maxR / (vmax * spec.dt)))
+
+
+def test_ensemble_instances_independent():
+    """A batch of far-apart instances evolves as the instances do alone
+    (bitwise: sums run in ascending node/particle id in both)."""
+    spec = configs.ensemble_2d(n_instances=3, n_min=2, n_max=5, ratios=(6,), speed_lo=2.0,
+                               speed_hi=4.0, pitch_factor=1.01, seed=5)
+    K = 250
+    o = oracle.OracleEngine(spec, 4)
+    o.integrate_begin()
+    o.integrate_cd(K)
+    F = o.field(capi.FIELD_F)
+    touched = 0
+    for k in range(3):
+        s = configs.ensemble_instance(spec, k)
+        e = oracle.OracleEngine(s, 4)
+        e.integrate_begin()
+        e.integrate_cd(K)
+        first, last, _ = spec.info["instances"][k]
+        n0, n1 = spec.particles[first]["glob_start"], spec.particles[last - 1]["glob_end"]
+        np.testing.assert_array_equal(e.field(capi.FIELD_F), F[n0:n1])
+        touched += int(np.abs(F[n0:n1]).max() > 1e3)
+    assert touched >= 1
