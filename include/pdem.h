@@ -237,6 +237,40 @@ int pdem_profile_get(pdem_ctx *ctx, double *pd_kernel_ms_avg, int64_t *pd_kernel
 int pdem_autotune(pdem_ctx *ctx, int32_t reps, double *best_ms);
 int pdem_get_launch_config(pdem_ctx *ctx, int32_t *pd_tpb, int32_t *mixed_tpb, int32_t *tile_nodes);
 
+/* ---------------------------------------------------------------------
+ * Learned pair-contact closure network.  No counterpart in the reference:
+ * a 16 -> 128 -> 128 -> 128 -> 4 perceptron (ReLU) whose products run on
+ * BF16 MFMA with FP32 accumulation; FP32 master weights and Adam state.
+ * Parameters: row-major W1[16][128], b1[128], W2[128][128], b2, W3, b3,
+ * W4[128][4], b4[4] (pdem_mlp_param_count() floats).  Inputs are normalised
+ * as (x - in_mean) * in_scale, targets as t * out_scale; the loss is
+ * mean over rows of sum_o out_weight[o] (y_o - t_o out_scale_o)^2.
+ * All buffers are host memory; n rows of 16 (inputs) or 4 (outputs) floats.
+ * ------------------------------------------------------------------- */
+#define PDEM_MLP_IN 16
+#define PDEM_MLP_HIDDEN 128
+#define PDEM_MLP_OUT 4
+
+typedef struct pdem_mlp pdem_mlp;
+
+int pdem_mlp_create(uint64_t seed, pdem_mlp **out);
+int pdem_mlp_destroy(pdem_mlp *m);
+const char *pdem_mlp_last_error(const pdem_mlp *m);
+int64_t pdem_mlp_param_count(void);
+int pdem_mlp_get_params(pdem_mlp *m, float *params);
+int pdem_mlp_set_params(pdem_mlp *m, const float *params);   /* resets Adam state */
+int pdem_mlp_set_normalization(pdem_mlp *m, const float *in_mean, const float *in_scale,
+                               const float *out_scale, const float *out_weight);
+int pdem_mlp_get_normalization(pdem_mlp *m, float *norm40); /* mean|scale|out_scale|out_weight */
+/* y = network(x), de-normalised (y_n / out_scale) */
+int pdem_mlp_forward(pdem_mlp *m, const float *x, int64_t n, float *y);
+/* gradient of the loss over exactly these n rows (no update) */
+int pdem_mlp_gradient(pdem_mlp *m, const float *x, const float *t, int64_t n, float *grad, float *loss);
+/* iters Adam steps on minibatches drawn with replacement (seeded), cosine
+ * learning-rate decay to 2 %; loss_curve (may be NULL) gets iters losses */
+int pdem_mlp_train(pdem_mlp *m, const float *x, const float *t, int64_t n, int32_t iters, int32_t batch,
+                   float lr, uint64_t seed, float *loss_curve);
+
 /* number of HIP devices visible to the engine (0 when none); no device work */
 int pdem_device_count(void);
 

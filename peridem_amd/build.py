@@ -22,7 +22,8 @@ def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     out = os.path.join(LIBDIR, "libpdem_hip.so")
     tmp = out + ".tmp"
-    cmd = [HIPCC] + FLAGS + [os.path.join(CSRC, "pdem_engine.hip"), "-o", tmp]
+    srcs = [os.path.join(CSRC, f) for f in ("pdem_engine.hip", "pdem_mlp.hip")]
+    cmd = [HIPCC] + FLAGS + srcs + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

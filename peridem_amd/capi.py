@@ -388,3 +388,102 @@ class GpuEngine(Engine):
     def compute_external_displacement_bc(self):
         self._check(self.lib.pdem_compute_external_displacement_bc(self.ctx),
                     "compute_external_displacement_bc")
+
+
+class Mlp:
+    """The closure network (pdem_mlp_*): BF16-MFMA forward/backward, Adam on the GPU."""
+
+    NIN, NH, NOUT = 16, 128, 4
+
+    def __init__(self, seed=0):
+        lib = engine_library()
+        self.lib = lib
+        P = C.POINTER
+        f32p = P(C.c_float)
+        sig = {
+            "create": ([C.c_uint64, P(C.c_void_p)], C.c_int),
+            "destroy": ([C.c_void_p], C.c_int),
+            "last_error": ([C.c_void_p], C.c_char_p),
+            "param_count": ([], C.c_int64),
+            "get_params": ([C.c_void_p, f32p], C.c_int),
+            "set_params": ([C.c_void_p, f32p], C.c_int),
+            "set_normalization": ([C.c_void_p, f32p, f32p, f32p, f32p], C.c_int),
+            "get_normalization": ([C.c_void_p, f32p], C.c_int),
+            "forward": ([C.c_void_p, f32p, C.c_int64, f32p], C.c_int),
+            "gradient": ([C.c_void_p, f32p, f32p, C.c_int64, f32p, f32p], C.c_int),
+            "train": ([C.c_void_p, f32p, f32p, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
+                       f32p], C.c_int),
+        }
+        for nm, (args, res) in sig.items():
+            fn = getattr(lib, "pdem_mlp_" + nm)
+            fn.argtypes = args
+            fn.restype = res
+        self.h = C.c_void_p()
+        if lib.pdem_mlp_create(int(seed), C.byref(self.h)) != 0:
+            raise RuntimeError("pdem_mlp_create: " + lib.pdem_mlp_last_error(None).decode())
+        self.nparam = int(lib.pdem_mlp_param_count())
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"pdem_mlp_{what}: " + self.lib.pdem_mlp_last_error(self.h).decode())
+
+    @staticmethod
+    def _f32(a, shape=None):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if shape is not None:
+            a = a.reshape(shape)
+        return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+    def close(self):
+        if self.h:
+            self.lib.pdem_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.empty(self.nparam, np.float32)
+        self._check(self.lib.pdem_mlp_get_params(self.h, self._f32(out)[1]), "get_params")
+        return out
+
+    def set_params(self, p):
+        a, ptr = self._f32(p)
+        assert a.size == self.nparam
+        self._check(self.lib.pdem_mlp_set_params(self.h, ptr), "set_params")
+
+    def set_normalization(self, in_mean, in_scale, out_scale, out_weight=(1.0, 1.0, 1.0, 1.0)):
+        bufs = [self._f32(v) for v in (in_mean, in_scale, out_scale, out_weight)]
+        self._check(self.lib.pdem_mlp_set_normalization(self.h, *[b[1] for b in bufs]), "set_normalization")
+
+    def normalization(self):
+        out = np.empty(40, np.float32)
+        self._check(self.lib.pdem_mlp_get_normalization(self.h, self._f32(out)[1]), "get_normalization")
+        return out[:16], out[16:32], out[32:36], out[36:40]
+
+    def forward(self, x):
+        x, px = self._f32(x)
+        n = x.shape[0]
+        y = np.empty((n, self.NOUT), np.float32)
+        self._check(self.lib.pdem_mlp_forward(self.h, px, n, self._f32(y)[1]), "forward")
+        return y
+
+    def gradient(self, x, t):
+        x, px = self._f32(x)
+        t, pt = self._f32(t)
+        g = np.empty(self.nparam, np.float32)
+        loss = np.empty(1, np.float32)
+        self._check(self.lib.pdem_mlp_gradient(self.h, px, pt, x.shape[0], self._f32(g)[1],
+                                               self._f32(loss)[1]), "gradient")
+        return float(loss[0]), g
+
+    def train(self, x, t, iters, batch=1024, lr=1e-3, seed=0):
+        x, px = self._f32(x)
+        t, pt = self._f32(t)
+        curve = np.empty(iters, np.float32)
+        self._check(self.lib.pdem_mlp_train(self.h, px, pt, x.shape[0], int(iters), int(batch), float(lr),
+                                            int(seed), self._f32(curve)[1]), "train")
+        return curve

@@ -259,6 +259,9 @@ def ensemble_2d(n_instances=512, seed=0, dt=5.0e-8, n_steps=5000, R_lo=0.8e-3, R
     by an angle ~ U[0, angle_max] (impact angle).  State-based material (the
     reference's PDState deck values), mixed precision as config 2.
 
+    Particles sit on a jittered grid whose gaps are at least 1.2 contact
+    radii (no contact force at t = 0); pitch_factor widens the grid.
+
     Instances occupy disjoint regions of one model, separated by far more
     than the contact search radius plus the distance travelled, so no
     contact, damping or candidate pair ever joins two instances: the batch is
@@ -288,7 +291,9 @@ def ensemble_2d(n_instances=512, seed=0, dt=5.0e-8, n_steps=5000, R_lo=0.8e-3, R
                                               Kn=_kn(mat, delta), Kn_factor=kn_factor, epsilon=0.7,
                                               beta_n_factor=1.0, friction_coeff=0.5)
     b = ModelBuilder(2, dt, zones, contact, search_factor=5.0, neigh_update_interval=40)
-    pitch = 2.0 * R_hi * pitch_factor
+    # start every pair out of contact: gaps of at least 1.2 contact radii
+    rc_max = max(c.contact_radius for c in contact.values())
+    pitch = 2.0 * R_hi * pitch_factor + 1.2 * rc_max
     box = int(math.ceil(math.sqrt(n_max))) * pitch
     spacing = box + 10.0e-3
     ncol_inst = int(math.ceil(math.sqrt(n_instances)))
@@ -306,7 +311,7 @@ def ensemble_2d(n_instances=512, seed=0, dt=5.0e-8, n_steps=5000, R_lo=0.8e-3, R
         first = len(b.parts)
         for q in range(n_p):
             rad = rng.uniform(R_lo, R_hi)
-            jit = (pitch / 2 - rad) * 0.9
+            jit = (pitch / 2 - rad - 0.6 * rc_max) * 0.9
             cx = ox + (q % ncol + 0.5) * pitch + rng.uniform(-jit, jit)
             cy = oy + (q // ncol + 0.5) * pitch + rng.uniform(-jit, jit)
             z = 2 * ri + (q % 2)

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "pdem.h"
+#include "pdem_host.hpp"
 #include "pdem_kernels.hpp"
 #include "pdem_mixed.hpp"
 
@@ -24,22 +25,6 @@ using namespace pdem;
 
 namespace {
 thread_local std::string g_create_err;
-
-struct HipError {
-  std::string msg;
-};
-
-#define HIPCHK(call)                                                                  \
-  do {                                                                                \
-    hipError_t e_ = (call);                                                           \
-    if (e_ != hipSuccess)                                                             \
-      throw HipError{std::string(#call) + ": " + hipGetErrorString(e_)};              \
-  } while (0)
-
-constexpr int TPB = 256;
-
-inline unsigned blocks(int64_t n, int tpb = TPB) { return (unsigned)((n + tpb - 1) / tpb); }
-
 }  // namespace
 
 struct pdem_ctx {
@@ -1109,6 +1094,17 @@ int pdem_autotune(pdem_ctx* c, int32_t reps, double* best_ms) {
       HIPCHK(hipEventDestroy(b));
       return (double)ms / reps;
     };
+    // The trial passes overwrite the force density (bond part only), the
+    // dilation and the damage: keep the step's values and put them back.
+    const size_t nN = (size_t)c->N;
+    double* keep = nullptr;
+    HIPCHK(hipMalloc(&keep, (4 * nN + 1) * sizeof(double)));
+    for (int d = 0; d < c->dim; d++)
+      HIPCHK(hipMemcpyAsync(keep + d * nN, c->ds.f[d], nN * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(keep + 3 * nN, c->ds.theta, nN * 8, hipMemcpyDeviceToDevice, c->stream));
+    float* keepZ = nullptr;
+    HIPCHK(hipMalloc(&keepZ, (nN + 1) * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(keepZ, c->ds.Z, nN * 4, hipMemcpyDeviceToDevice, c->stream));
     const bool prof = c->profiling;
     c->profiling = false;
     double best = 1e300;
@@ -1140,6 +1136,13 @@ int pdem_autotune(pdem_ctx* c, int32_t reps, double* best_ms) {
       c->pd_tpb = bp;
     }
     c->profiling = prof;
+    for (int d = 0; d < c->dim; d++)
+      HIPCHK(hipMemcpyAsync(c->ds.f[d], keep + d * nN, nN * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->ds.theta, keep + 3 * nN, nN * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->ds.Z, keepZ, nN * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(keep));
+    HIPCHK(hipFree(keepZ));
     // launch shapes are baked into a captured step: capture again
     if (c->graph_ready) {
       HIPCHK(hipGraphExecDestroy(c->gexec));

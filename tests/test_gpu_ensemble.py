@@ -20,8 +20,10 @@ pytestmark = pytest.mark.gpu
 
 
 def _small_ensemble(n=4, seed=5):
+    # equal radii on the tightest grid: gaps of 1.2 contact radii, closed
+    # within a few hundred steps at 2-4 m/s
     return configs.ensemble_2d(n_instances=n, n_min=2, n_max=6, ratios=(6,), speed_lo=2.0,
-                               speed_hi=4.0, pitch_factor=1.01, seed=seed)
+                               speed_hi=4.0, pitch_factor=1.0, R_lo=1.0e-3, R_hi=1.0e-3, seed=seed)
 
 
 def _slice(spec, k):
@@ -31,7 +33,7 @@ def _slice(spec, k):
 
 def test_ensemble_batched_equals_separate():
     spec = _small_ensemble()
-    K = 300
+    K = 400
     print(f"[ensemble] nodes={spec.n_nodes} particles={len(spec.particles)} steps={K}", flush=True)
     g = capi.GpuEngine(spec, capi.PREC_FP64)
     g.integrate_begin()
@@ -41,7 +43,7 @@ def test_ensemble_batched_equals_separate():
     r.integrate_cd(K)
     compare_state(g, r, tag="ensemble batched")
     F, U, V = g.field(capi.FIELD_F), g.field(capi.FIELD_U), g.field(capi.FIELD_V)
-    assert np.abs(F).max() > 0.0
+    Z = g.field(capi.FIELD_Z)
     touched = 0
     for k in range(len(spec.info["instances"])):
         s = configs.ensemble_instance(spec, k)
@@ -52,14 +54,14 @@ def test_ensemble_batched_equals_separate():
         np.testing.assert_array_equal(e.field(capi.FIELD_F), F[n0:n1])
         np.testing.assert_array_equal(e.field(capi.FIELD_U), U[n0:n1])
         np.testing.assert_array_equal(e.field(capi.FIELD_V), V[n0:n1])
-        touched += int(np.abs(F[n0:n1]).max() > 1e3)
+        touched += int(Z[n0:n1].max() > 0.0)
         print(f"[ensemble] instance {k} equal", flush=True)
-    assert touched >= 2, "too few instances reached contact in the test window"
+    assert touched >= 2, "too few instances deformed by contact in the test window"
 
 
 def test_ensemble_mixed_against_oracle():
     spec = _small_ensemble(n=6, seed=11)
-    K = 200
+    K = 400
     print(f"[ensemble-mixed] nodes={spec.n_nodes} steps={K}", flush=True)
     g = capi.GpuEngine(spec, capi.PREC_MIXED)
     r = oracle.OracleEngine(spec, 8)
@@ -71,7 +73,8 @@ def test_ensemble_mixed_against_oracle():
     den = np.abs(fr).max()
     emax = float(np.abs(fg - fr).max() / den)
     erms = float(np.sqrt(((fg - fr) ** 2).sum(1).mean()) / den)
-    print(f"[ensemble-mixed] force max err {emax:.3e} rms {erms:.3e}", flush=True)
+    print(f"[ensemble-mixed] max |f| {den:.3e} force max err {emax:.3e} rms {erms:.3e}", flush=True)
+    assert den > 1e8  # contact forces, particles deform
     assert emax < 1e-4 and erms < 1e-5
 
 
