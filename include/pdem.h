@@ -271,6 +271,37 @@ int pdem_mlp_gradient(pdem_mlp *m, const float *x, const float *t, int64_t n, fl
 int pdem_mlp_train(pdem_mlp *m, const float *x, const float *t, int64_t n, int32_t iters, int32_t batch,
                    float lr, uint64_t seed, float *loss_curve);
 
+/* ---------------------------------------------------------------------
+ * Learned pair-contact closure (2D multi-particle models).  No counterpart in
+ * the reference: for particle pairs in light, undamaged contact the node-level
+ * contact terms of computeContactForces (demModel.cpp:832-934) are replaced
+ * by the network's pair force and moment, spread over the contact patch.
+ *  setup:   particle_code[n_particles] is feature 11/12 of the pairs (shape
+ *           or material class); may be NULL (all 0).
+ *  enable:  pairs with penetration Rc - gap <= pen_max_h * min(h_a, h_b), no
+ *           broken bonds (max Z < 1) and predicted local damage <= dz_max use
+ *           the network; all others keep the plain kernels.  net may be NULL
+ *           (or pen_max_h <= 0): then no pair uses the network and the step
+ *           equals the plain step.  Every check_every steps (0: never) the
+ *           plain pair force of each closure pair is also evaluated and the
+ *           difference accumulated.  The network must outlive the context's
+ *           use of it and stay on the context's device.
+ *  stats:   pairs per mode since enable (0 no contact possible, 1 closure,
+ *           2 plain), checked pairs, mean and max |F_closure - F_plain| and
+ *           mean |F_plain|, in units of Kn Rc Vbar_a Vbar_b.
+ *  log:     training rows of the current state: 16 features and 4 targets
+ *           (F.n, F.t, M_a / h_a in those units, local max Z) for every pair
+ *           with gap < Rc + band_h * min(h_a, h_b); *n gets the number of
+ *           rows found (rows beyond cap are dropped).
+ * ------------------------------------------------------------------- */
+int pdem_closure_setup(pdem_ctx *ctx, const float *particle_code);
+int pdem_closure_enable(pdem_ctx *ctx, const pdem_mlp *net, double pen_max_h, double dz_max,
+                        int32_t check_every);
+int pdem_closure_disable(pdem_ctx *ctx);
+int pdem_closure_stats(pdem_ctx *ctx, int64_t *pairs_by_mode3, int64_t *checked, double *err_mean,
+                       double *err_max, double *plain_mean);
+int pdem_pair_log(pdem_ctx *ctx, double band_h, int64_t cap, float *features, float *targets, int64_t *n);
+
 /* number of HIP devices visible to the engine (0 when none); no device work */
 int pdem_device_count(void);
 

@@ -330,6 +330,19 @@ def engine_library():
         lib.pdem_profile_enable.argtypes = [C.c_void_p, C.c_int32]
         lib.pdem_profile_get.restype = C.c_int
         lib.pdem_profile_get.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        lib.pdem_closure_setup.restype = C.c_int
+        lib.pdem_closure_setup.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+        lib.pdem_closure_enable.restype = C.c_int
+        lib.pdem_closure_enable.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int32]
+        lib.pdem_closure_disable.restype = C.c_int
+        lib.pdem_closure_disable.argtypes = [C.c_void_p]
+        lib.pdem_closure_stats.restype = C.c_int
+        lib.pdem_closure_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double)]
+        lib.pdem_pair_log.restype = C.c_int
+        lib.pdem_pair_log.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.POINTER(C.c_float),
+                                      C.POINTER(C.c_float), C.POINTER(C.c_int64)]
         lib.pdem_autotune.restype = C.c_int
         lib.pdem_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
         lib.pdem_get_launch_config.restype = C.c_int
@@ -375,6 +388,44 @@ class GpuEngine(Engine):
         v = [C.c_int32() for _ in range(3)]
         self._check(self.lib.pdem_get_launch_config(self.ctx, *[C.byref(x) for x in v]), "launch_config")
         return tuple(x.value for x in v)
+
+    def closure_setup(self, codes=None):
+        """Closure data; codes: one float per particle (shape/material class)."""
+        if codes is None:
+            ptr = None
+        else:
+            self._codes = np.ascontiguousarray(codes, dtype=np.float32)
+            ptr = self._codes.ctypes.data_as(C.POINTER(C.c_float))
+        self._check(self.lib.pdem_closure_setup(self.ctx, ptr), "closure_setup")
+
+    def closure_enable(self, net=None, pen_max_h=0.05, dz_max=0.5, check_every=100):
+        self._closure_net = net  # keep the network alive while the context uses it
+        h = net.h if net is not None else None
+        self._check(self.lib.pdem_closure_enable(self.ctx, h, float(pen_max_h), float(dz_max), int(check_every)),
+                    "closure_enable")
+
+    def closure_disable(self):
+        self._check(self.lib.pdem_closure_disable(self.ctx), "closure_disable")
+        self._closure_net = None
+
+    def closure_stats(self):
+        modes = (C.c_int64 * 3)()
+        chk = C.c_int64()
+        em, ex, pm = C.c_double(), C.c_double(), C.c_double()
+        self._check(self.lib.pdem_closure_stats(self.ctx, modes, C.byref(chk), C.byref(em), C.byref(ex),
+                                                C.byref(pm)), "closure_stats")
+        return {"no_contact": modes[0], "closure": modes[1], "plain": modes[2], "checked": chk.value,
+                "check_err_mean": em.value, "check_err_max": ex.value, "check_plain_mean": pm.value}
+
+    def pair_log(self, band_h=0.5, cap=1 << 20):
+        """Training rows of the current state: (features [n][16], targets [n][4])."""
+        f = np.empty((cap, 16), np.float32)
+        t = np.empty((cap, 4), np.float32)
+        n = C.c_int64()
+        self._check(self.lib.pdem_pair_log(self.ctx, float(band_h), int(cap), f.ctypes.data_as(C.POINTER(C.c_float)),
+                                           t.ctypes.data_as(C.POINTER(C.c_float)), C.byref(n)), "pair_log")
+        m = min(n.value, cap)
+        return f[:m].copy(), t[:m].copy()
 
     def update_contact_neighborlist(self):
         self._check(self.lib.pdem_update_contact_neighborlist(self.ctx), "update_contact_neighborlist")

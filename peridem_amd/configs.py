@@ -133,7 +133,7 @@ def compressive_2d(n_particles=500, R_lo=0.8e-3, R_hi=1.2e-3, nodes_per_particle
     b.ubc.append(BcDeck(particles=[top_id], directions=[2], time_fn=capi.UFN_LINEAR,
                         time_params=[-wall_speed]))
     spec = b.build(name=f"compressive_2d_n{n_particles}", n_steps=n_steps, wall_id=top_id, f_dir=1)
-    spec.info.update(box=(W, H), seed=seed)
+    spec.info.update(box=(W, H), seed=seed, shape_of_zone=[0.0, 1.0, 0.0, 0.0])
     return spec
 
 
@@ -329,7 +329,7 @@ def ensemble_2d(n_instances=512, seed=0, dt=5.0e-8, n_steps=5000, R_lo=0.8e-3, R
             b.ic.append(([pid], (vx, vy, 0.0)))
         instances.append((first, len(b.parts), ratios[ri]))
     spec = b.build(name=f"ensemble_2d_x{n_instances}", n_steps=n_steps)
-    spec.info.update(instances=instances, seed=seed)
+    spec.info.update(instances=instances, seed=seed, shape_of_zone=[float(z % 2) for z in range(nz)])
     return spec
 
 

@@ -1,0 +1,400 @@
+// pdem_closure.hpp -- learned pair-contact closure for 2D particle pairs.
+//
+// For every pair of non-wall particles listed at the last contact rebuild
+// (a < b, centres within the damping reach plus twice the search radius) the
+// closure kernel works in the pair frame n = (c_b - c_a) / |c_b - c_a|:
+//
+//   extents   ext_a = max (x_i - c_a).n over a's surface nodes, ext_b likewise
+//             along -n; gap = |c_b - c_a| - ext_a - ext_b.  Every node pair
+//             (i in a, j in b) is at least `gap` apart, so gap >= Rc means no
+//             contact force is possible (mode 0, exact).
+//   gating    penetration Rc - gap <= pen_max_h * min(h_a, h_b), no damage in
+//             either particle (max Z < 1) and a predicted local damage below
+//             dz_max: closure (mode 1); otherwise the plain node-level contact
+//             kernels handle the pair (mode 2).
+//   network   16 features -> (F.n, F.t, M_a, local damage) in units of
+//             Kn Rc Vbar_a Vbar_b (and h_a for M), BF16 MFMA (pdem_mlp.hpp).
+//   transfer  the pair force F on a (-F on b) and the moments M_a, M_b are
+//             spread over the patch of surface nodes within Rc of the extent:
+//             a uniform force density F / sum V plus a torque term
+//             k perp(x - xbar) with zero net force, so the patch receives
+//             exactly F and M.
+//
+// A sampled check every `check_every` steps evaluates the plain node-level
+// pair force of every closure pair from the candidate lists and accumulates
+// the normalised difference.  k_pair_log writes the same features and the
+// plain targets for training (pairs within a band around contact).
+//
+// Lanes: a wavefront takes 16 pairs; the 4 lanes l, l+16, l+32, l+48 of a
+// pair split its node loops and meet through shuffles, and hold the 4
+// feature quarters the MFMA input fragment needs.
+#pragma once
+#include "pdem_kernels.hpp"
+#include "pdem_mlp.hpp"
+
+namespace pdem {
+
+struct ClosureArgs {
+  mlp::Net net;
+  int use_net;                // 0: no network (closure band empty: gating only)
+  double pen_max_h;           // band: Rc - gap <= pen_max_h * min(h_a, h_b)
+  float dz_max;               // predicted local damage above this: plain
+  int check_every;            // sampled plain check cadence in steps (0: never)
+  const int* ref_node;        // [nn] orientation reference node
+  const double* vbar;         // [nn] mean nodal volume
+  const float* code;          // [nn] particle code (feature)
+  unsigned long long* stats;  // [3] pairs per mode (cumulative), [3] checked pairs, [4] max error bits
+  double* err_sum;            // [0] sum |F_cl - F_plain| / Fs, [1] sum |F_plain| / Fs
+};
+
+constexpr int CL_NFEAT = 16;
+constexpr double CL_V0 = 1.0;  // velocity scale of the features, m/s
+
+__device__ __forceinline__ double sub_max(double v) {
+  v = fmax(v, __shfl_xor(v, 16));
+  return fmax(v, __shfl_xor(v, 32));
+}
+__device__ __forceinline__ double sub_sum(double v) {
+  v += __shfl_xor(v, 16);
+  return v + __shfl_xor(v, 32);
+}
+
+struct PairGeo {
+  int a, b, pa, pb, sa;
+  double ca[2], cb[2], n[2], d;
+  double ext_a, ext_b, gap, rc, ha, hb;
+  DevContact ct;
+};
+
+// largest (x_i - c).dir over the surface nodes of non-wall particle q
+__device__ __forceinline__ double extent(const DS& s, int q, const double* c, const double* dir, int sub,
+                                         bool on) {
+  double m = -1e300;
+  if (on)
+    for (int k = s.surf_ptr[q] + sub; k < s.surf_ptr[q + 1]; k += 4) {
+      const int i = s.surf_idx[k];
+      const double e = (s.x[0][i] - c[0]) * dir[0] + (s.x[1][i] - c[1]) * dir[1];
+      m = fmax(m, e);
+    }
+  return sub_max(m);
+}
+
+__device__ __forceinline__ void pair_geometry(const DS& s, int k, bool valid, int sub, PairGeo& g) {
+  const int nn = s.n_nonwall;
+  g.a = valid ? s.cpairs[2 * (int64_t)k] : 0;
+  g.sa = valid ? s.cpairs[2 * (int64_t)k + 1] : 0;
+  g.b = valid ? s.dlist[(int64_t)g.a * s.dcap + g.sa] : 0;
+  g.pa = s.nonwall_pid[g.a];
+  g.pb = s.nonwall_pid[g.b];
+  for (int d = 0; d < 2; d++) {
+    g.ca[d] = s.pcx[d * nn + g.a];
+    g.cb[d] = s.pcx[d * nn + g.b];
+  }
+  const double dx = g.cb[0] - g.ca[0], dy = g.cb[1] - g.ca[1];
+  g.d = sqrt(dx * dx + dy * dy);
+  const bool ok = valid && g.d > 0.;
+  g.n[0] = ok ? dx / g.d : 1.;
+  g.n[1] = ok ? dy / g.d : 0.;
+  const double nb[2] = {-g.n[0], -g.n[1]};
+  g.ct = s.CT[s.pzone[g.a] * s.nz + s.pzone[g.b]];
+  g.rc = g.ct.contact_r;
+  g.ha = s.P[g.pa].h;
+  g.hb = s.P[g.pb].h;
+  // nodes lie within the bounding radius of the geometric centre, which is
+  // within h of the centre node; 1.5 h per particle leaves room for strain
+  const double reach = s.prad[g.a] + s.prad[g.b] + 1.5 * (g.ha + g.hb);
+  const bool far = ok && g.d - reach >= g.rc;
+  g.ext_a = extent(s, g.a, g.ca, g.n, sub, ok && !far);
+  g.ext_b = extent(s, g.b, g.cb, nb, sub, ok && !far);
+  if (far) {
+    g.ext_a = s.prad[g.a] + 1.5 * g.ha;
+    g.ext_b = s.prad[g.b] + 1.5 * g.hb;
+  }
+  g.gap = ok ? g.d - g.ext_a - g.ext_b : -1e300;  // coincident centres: plain
+}
+
+__device__ __forceinline__ double particle_zmax(const DS& s, int pid, int sub, bool on) {
+  double z = 0.;
+  if (on) {
+    const DevParticle& P = s.P[pid];
+    for (int64_t i = P.glob_start + sub; i < P.glob_end; i += 4) z = fmax(z, (double)s.Z[i]);
+  }
+  return sub_max(z);
+}
+
+__device__ __forceinline__ double omega(const DS& s, const ClosureArgs& ca, int q, const double* c) {
+  const int nn = s.n_nonwall;
+  const int r = ca.ref_node[q];
+  const double rx = s.x[0][r] - c[0], ry = s.x[1][r] - c[1];
+  const double vx = s.v[0][r] - s.pcv[q], vy = s.v[1][r] - s.pcv[nn + q];
+  const double r2 = rx * rx + ry * ry;
+  return r2 > 0. ? (rx * vy - ry * vx) / r2 : 0.;
+}
+
+__device__ __forceinline__ void pair_features(const DS& s, const ClosureArgs& ca, const PairGeo& g, double za,
+                                              double zb, float* x16) {
+  const int nn = s.n_nonwall;
+  const double t[2] = {-g.n[1], g.n[0]};
+  const double dvx = s.pcv[g.b] - s.pcv[g.a], dvy = s.pcv[nn + g.b] - s.pcv[nn + g.a];
+  const double Ra = s.prad[g.a], Rb = s.prad[g.b];
+  x16[0] = (float)(g.gap / g.rc);
+  x16[1] = (float)(g.ha / g.rc);
+  x16[2] = (float)(g.hb / g.rc);
+  x16[3] = (float)(Ra / g.ha);
+  x16[4] = (float)(Rb / g.hb);
+  x16[5] = (float)(g.ext_a / Ra);
+  x16[6] = (float)(g.ext_b / Rb);
+  x16[7] = (float)((dvx * g.n[0] + dvy * g.n[1]) / CL_V0);
+  x16[8] = (float)((dvx * t[0] + dvy * t[1]) / CL_V0);
+  x16[9] = (float)(omega(s, ca, g.a, g.ca) * Ra / CL_V0);
+  x16[10] = (float)(omega(s, ca, g.b, g.cb) * Rb / CL_V0);
+  x16[11] = ca.code[g.a];
+  x16[12] = ca.code[g.b];
+  x16[13] = (float)g.ct.mu;
+  x16[14] = (float)za;
+  x16[15] = (float)zb;
+}
+
+// force unit of a pair: Kn Rc Vbar_a Vbar_b
+__device__ __forceinline__ double pair_unit(const ClosureArgs& ca, const PairGeo& g) {
+  return g.ct.Kn * g.rc * ca.vbar[g.a] * ca.vbar[g.b];
+}
+
+// Plain net contact force and moment (about c_a) on particle a from particle
+// b: the node-level terms of computeContactForces (demModel.cpp:868-904)
+// summed with the nodal volumes, from the candidate lists; zloc = largest Z
+// of a node of a with an active contact with b.
+__device__ __forceinline__ void plain_pair(const DS& s, const PairGeo& g, int sub, bool on, double* F, double& M,
+                                           double& zloc) {
+  double f0 = 0., f1 = 0., m = 0., z = 0.;
+  if (on) {
+    const DevParticle& P = s.P[g.pa];
+    const DevContact ct = g.ct;
+    for (int64_t i = P.glob_start + sub; i < P.glob_end; i += 4) {
+      const int nc = s.ccount[i];
+      if (nc == 0) continue;
+      double yi[2] = {s.x[0][i], s.x[1][i]}, vi[2] = {s.v[0][i], s.v[1][i]};
+      double fi[2] = {0., 0.};
+      bool any = false;
+      for (int k = 0; k < nc; k++) {
+        const int j = s.cand[(int64_t)k * s.N + i];
+        if (s.ptid[j] != g.pb) continue;
+        double yji[2] = {s.x[0][j] - yi[0], s.x[1][j] - yi[1]};
+        const double R = len3<2>(yji);
+        if (!isLess(R, ct.contact_r)) continue;
+        double fn[2], ft[2];
+        contact_pair<2>(s, vi, j, ct, R, yji, fn, ft);
+        fi[0] += fn[0];
+        fi[1] += fn[1];
+        fi[0] += ft[0];
+        fi[1] += ft[1];
+        any = true;
+      }
+      if (any) {
+        const double V = s.vol[i];
+        f0 += V * fi[0];
+        f1 += V * fi[1];
+        m += V * ((yi[0] - g.ca[0]) * fi[1] - (yi[1] - g.ca[1]) * fi[0]);
+        z = fmax(z, (double)s.Z[i]);
+      }
+    }
+  }
+  F[0] = sub_sum(f0);
+  F[1] = sub_sum(f1);
+  M = sub_sum(m);
+  zloc = sub_max(z);
+}
+
+// patch of q's surface nodes with (x - c).dir >= thr: sum V, sum V r, sum V |r|^2 (r = x - c)
+__device__ __forceinline__ void patch_sums(const DS& s, int q, const double* c, const double* dir, double thr,
+                                           int sub, bool on, double& sv, double* sx, double& sxx) {
+  double a = 0., b0 = 0., b1 = 0., e2 = 0.;
+  if (on)
+    for (int k = s.surf_ptr[q] + sub; k < s.surf_ptr[q + 1]; k += 4) {
+      const int i = s.surf_idx[k];
+      const double r0 = s.x[0][i] - c[0], r1 = s.x[1][i] - c[1];
+      if (r0 * dir[0] + r1 * dir[1] < thr) continue;
+      const double V = s.vol[i];
+      a += V;
+      b0 += V * r0;
+      b1 += V * r1;
+      e2 += V * (r0 * r0 + r1 * r1);
+    }
+  sv = sub_sum(a);
+  sx[0] = sub_sum(b0);
+  sx[1] = sub_sum(b1);
+  sxx = sub_sum(e2);
+}
+
+// record of one side of a closure pair: dir, thr, uniform force density,
+// torque coefficient, patch centroid (relative to the particle centre)
+__device__ __forceinline__ void closure_record(double* rec, const double* dir, double thr, const double* F,
+                                               double Mc, double sv, const double* sx, double sxx) {
+  double xb[2] = {0., 0.}, f0[2] = {0., 0.}, k = 0.;
+  if (sv > 0.) {
+    xb[0] = sx[0] / sv;
+    xb[1] = sx[1] / sv;
+    f0[0] = F[0] / sv;
+    f0[1] = F[1] / sv;
+    const double I = sxx - sv * (xb[0] * xb[0] + xb[1] * xb[1]);
+    const double dM = Mc - (xb[0] * F[1] - xb[1] * F[0]);
+    k = I > 0. ? dM / I : 0.;
+  }
+  rec[0] = dir[0];
+  rec[1] = dir[1];
+  rec[2] = thr;
+  rec[3] = f0[0];
+  rec[4] = f0[1];
+  rec[5] = k;
+  rec[6] = xb[0];
+  rec[7] = xb[1];
+}
+
+__device__ __forceinline__ int find_slot(const DS& s, int q, int target) {
+  const int* dl = s.dlist + (int64_t)q * s.dcap;
+  int lo = 0, hi = s.dcount[q] - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    if (dl[mid] == target) return mid;
+    if (dl[mid] < target)
+      lo = mid + 1;
+    else
+      hi = mid - 1;
+  }
+  return -1;
+}
+
+__global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, sub = lane >> 4, pl = lane & 15;
+  const int npairs = min((int)s.ctrl->npairs, s.cpair_cap);
+  const bool check = ca.check_every > 0 && (s.ctrl->n % ca.check_every) == 0;
+  for (int base = (blockIdx.x * 4 + wave) * 16; base < npairs; base += gridDim.x * 64) {
+    const int k = base + pl;
+    const bool valid = k < npairs;
+    PairGeo g;
+    pair_geometry(s, k, valid, sub, g);
+    const bool near = valid && g.gap < g.rc;
+    const double za = particle_zmax(s, g.pa, sub, near);
+    const double zb = particle_zmax(s, g.pb, sub, near);
+    int mode = 0;
+    if (near)
+      mode = (ca.use_net && g.rc - g.gap <= ca.pen_max_h * fmin(g.ha, g.hb) && za < 1. && zb < 1.) ? 1 : 2;
+    double Fa[2] = {0., 0.}, Ma = 0.;
+    const double unit = pair_unit(ca, g);
+    if (ca.use_net) {
+      float x16[CL_NFEAT];
+      pair_features(s, ca, g, za, zb, x16);
+      mlp::s4 h1[8], h2[8], h3[8];
+      const mlp::f4 y = mlp::forward(ca.net, mlp::input_frag(ca.net, x16, lane), h1, h2, h3, lane);
+      double ys[4];
+      for (int r = 0; r < 4; r++) ys[r] = (double)(__shfl(y[r], pl) / ca.net.out_scale[r]);
+      if (mode == 1 && ys[3] > (double)ca.dz_max) mode = 2;
+      const double Fn = ys[0] * unit, Ft = ys[1] * unit;
+      Fa[0] = Fn * g.n[0] - Ft * g.n[1];
+      Fa[1] = Fn * g.n[1] + Ft * g.n[0];
+      Ma = ys[2] * unit * g.ha;
+    }
+    const double nb[2] = {-g.n[0], -g.n[1]};
+    double sva, sxa[2], sxxa, svb, sxb[2], sxxb;
+    patch_sums(s, g.a, g.ca, g.n, g.ext_a - g.rc, sub, mode == 1, sva, sxa, sxxa);
+    patch_sums(s, g.b, g.cb, nb, g.ext_b - g.rc, sub, mode == 1, svb, sxb, sxxb);
+    const int sb = valid ? find_slot(s, g.b, g.a) : -1;
+    if (valid && sub == 0) {
+      const int64_t ia = (int64_t)g.a * s.dcap + g.sa;
+      s.cmode[ia] = (int8_t)mode;
+      if (sb >= 0) s.cmode[(int64_t)g.b * s.dcap + sb] = (int8_t)mode;
+      if (mode == 2) {
+        atomicOr(&s.plain_any[g.a], 1);
+        atomicOr(&s.plain_any[g.b], 1);
+      }
+      if (mode == 1 && sb >= 0) {
+        const double Fb[2] = {-Fa[0], -Fa[1]};
+        // torque balance: M_b = -M_a + (c_b - c_a) x F_a
+        const double Mb = -Ma + ((g.cb[0] - g.ca[0]) * Fa[1] - (g.cb[1] - g.ca[1]) * Fa[0]);
+        closure_record(s.cout + ia * 8, g.n, g.ext_a - g.rc, Fa, Ma, sva, sxa, sxxa);
+        closure_record(s.cout + ((int64_t)g.b * s.dcap + sb) * 8, nb, g.ext_b - g.rc, Fb, Mb, svb, sxb, sxxb);
+      }
+      if (sb < 0) s.ctrl->overflow = 6;
+      atomicAdd(&ca.stats[mode], 1ull);
+    }
+    if (check) {
+      double Fp[2], Mp, zl;
+      plain_pair(s, g, sub, valid && mode == 1, Fp, Mp, zl);
+      if (valid && sub == 0 && mode == 1) {
+        const double e = sqrt((Fa[0] - Fp[0]) * (Fa[0] - Fp[0]) + (Fa[1] - Fp[1]) * (Fa[1] - Fp[1])) / unit;
+        const double fp = sqrt(Fp[0] * Fp[0] + Fp[1] * Fp[1]) / unit;
+        atomicAdd(&ca.err_sum[0], e);
+        atomicAdd(&ca.err_sum[1], fp);
+        atomicAdd(&ca.stats[3], 1ull);
+        atomicMax(&ca.stats[4], (unsigned long long)__double_as_longlong(e));
+      }
+    }
+  }
+}
+
+// Training rows: every listed pair with gap < Rc + band_h * min(h_a, h_b):
+// features and the plain targets (F.n, F.t, M_a / h_a in force units, local Z).
+__global__ void __launch_bounds__(256) k_pair_log(DS s, ClosureArgs ca, double band_h, float* feat, float* targ,
+                                                  unsigned* count, int cap) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, sub = lane >> 4, pl = lane & 15;
+  const int npairs = min((int)s.ctrl->npairs, s.cpair_cap);
+  for (int base = (blockIdx.x * 4 + wave) * 16; base < npairs; base += gridDim.x * 64) {
+    const int k = base + pl;
+    const bool valid = k < npairs;
+    PairGeo g;
+    pair_geometry(s, k, valid, sub, g);
+    const bool take = valid && g.gap < g.rc + band_h * fmin(g.ha, g.hb);
+    const double za = particle_zmax(s, g.pa, sub, take);
+    const double zb = particle_zmax(s, g.pb, sub, take);
+    double Fp[2], Mp, zl;
+    plain_pair(s, g, sub, take, Fp, Mp, zl);
+    if (take && sub == 0) {
+      const unsigned row = atomicAdd(count, 1u);
+      if ((int)row < cap) {
+        float x16[CL_NFEAT];
+        pair_features(s, ca, g, za, zb, x16);
+        for (int f = 0; f < CL_NFEAT; f++) feat[(int64_t)row * CL_NFEAT + f] = x16[f];
+        const double unit = pair_unit(ca, g);
+        targ[(int64_t)row * 4 + 0] = (float)((Fp[0] * g.n[0] + Fp[1] * g.n[1]) / unit);
+        targ[(int64_t)row * 4 + 1] = (float)((-Fp[0] * g.n[1] + Fp[1] * g.n[0]) / unit);
+        targ[(int64_t)row * 4 + 2] = (float)(Mp / (unit * g.ha));
+        targ[(int64_t)row * 4 + 3] = (float)zl;
+      }
+    }
+  }
+}
+
+constexpr int CL_BLOCKS = 256;  // closure kernel grid (grid-stride over pair chunks)
+
+// Closure lists from the current candidate and partner lists (set-up and
+// logging; afterwards the rebuilds maintain them): wall_near per particle and
+// the pair list.
+__global__ void k_cl_wall_near(DS s) {
+  const int64_t i = gid();
+  if (i >= s.N) return;
+  const DevParticle& P = s.P[s.ptid[i]];
+  if (P.nonwall_index < 0) return;
+  const int cnt = s.ccount[i];
+  for (int k = 0; k < cnt; k++)
+    if (s.pmeta[s.ptid[s.cand[(int64_t)k * s.N + i]]] >> 16) {
+      atomicOr(&s.wall_near[P.nonwall_index], 1);
+      return;
+    }
+}
+__global__ void k_cl_pairs(DS s) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= s.n_nonwall) return;
+  const int cnt = s.dcount[q];
+  for (int k = 0; k < cnt; k++)
+    if (s.dlist[(int64_t)q * s.dcap + k] > q) {
+      const unsigned r = atomicAdd(&s.ctrl->npairs, 1u);
+      if ((int)r < s.cpair_cap) {
+        s.cpairs[2 * (int64_t)r] = q;
+        s.cpairs[2 * (int64_t)r + 1] = k;
+      } else {
+        s.ctrl->overflow = 5;
+      }
+    }
+}
+
+}  // namespace pdem

@@ -20,6 +20,7 @@
 #include "pdem_host.hpp"
 #include "pdem_kernels.hpp"
 #include "pdem_mixed.hpp"
+#include "pdem_closure.hpp"
 
 using namespace pdem;
 
@@ -51,6 +52,10 @@ struct pdem_ctx {
   int mixed_tpb = 1024;    // mixed tile kernels' workgroup size (autotuned)
   int tile_nodes = TILE_NODES;
   std::vector<int> nbr_lo, nbr_hi;  // per-node neighbor id range (tile windows)
+  // learned pair-contact closure
+  bool cl_ready = false;        // closure data allocated (pdem_closure_setup)
+  ClosureArgs ca{};
+  const pdem_mlp* cl_net = nullptr;
   bool graph_ready = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -68,6 +73,11 @@ struct pdem_ctx {
     allocs.push_back(p);
     if (zero) HIPCHK(hipMemset(p, 0, n * sizeof(T)));
     return (T*)p;
+  }
+  void dfree(void* p) {
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it != allocs.end()) allocs.erase(it);
+    HIPCHK(hipFree(p));
   }
   template <class T>
   T* upload(const T* h, size_t n) {
@@ -199,9 +209,15 @@ struct Launch {
     if (c->ds.n_nonwall > 0)
       hipLaunchKernelGGL(k_damp_pairs<DIM>, dim3(c->ds.n_nonwall), dim3(64), 0, c->stream, c->ds);
   }
+  // closure pairs: modes, network, patch records (2D, closure mode only)
+  static void closure(pdem_ctx* c) {
+    if (DIM == 2 && c->ds.cl_on)
+      hipLaunchKernelGGL(k_closure, dim3(CL_BLOCKS), dim3(256), 0, c->stream, c->ds, c->ca);
+  }
   static void contact_forces(pdem_ctx* c, bool apply_damping) {
     hipStream_t st = c->stream;
     const unsigned nb = blocks(c->N);
+    closure(c);
     contact_force_kernel(c);
     damping(c);
     if (apply_damping) hipLaunchKernelGGL(k_apply_damping<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
@@ -231,6 +247,7 @@ struct Launch {
     if (c->multi) {
       hipLaunchKernelGGL(k_query<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds);
       damp_pairs(c);
+      closure(c);
       contact_force_kernel(c);
       damping(c);
     }
@@ -283,6 +300,9 @@ void check_overflow(pdem_ctx* c) {
   if (h.overflow == 2)
     throw HipError{"cell capacity exceeded (" + std::to_string(c->ds.cell_cap) +
                    " nodes per search cell): nodes crowded far beyond their initial packing"};
+  if (h.overflow == 5)
+    throw HipError{"closure pair list capacity exceeded (" + std::to_string(c->ds.cpair_cap) + " pairs)"};
+  if (h.overflow == 6) throw HipError{"closure: damping partner lists are not symmetric"};
 }
 
 // Tiles of the mixed kernels: for every state-based particle, the largest
@@ -1072,6 +1092,190 @@ int pdem_profile_get(pdem_ctx* c, double* ms, int64_t* n) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+void drop_graph(pdem_ctx* c) {
+  if (!c->graph_ready) return;
+  HIPCHK(hipGraphExecDestroy(c->gexec));
+  HIPCHK(hipGraphDestroy(c->graph));
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->graph_ready = false;
+}
+
+// (re)build the closure pair list and wall flags from the current candidate
+// and partner lists
+void closure_lists(pdem_ctx* c) {
+  DS& s = c->ds;
+  const int nn = s.n_nonwall;
+  HIPCHK(hipMemsetAsync(s.wall_near, 0, (size_t)nn * sizeof(int), c->stream));
+  const uint32_t zero = 0;
+  HIPCHK(hipMemcpyAsync(&s.ctrl->npairs, &zero, 4, hipMemcpyHostToDevice, c->stream));
+  DS t = s;
+  t.cl_on = 1;
+  hipLaunchKernelGGL(k_cl_wall_near, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, t);
+  hipLaunchKernelGGL(k_cl_pairs, dim3(blocks(nn)), dim3(TPB), 0, c->stream, t);
+  check_launch(c);
+  check_overflow(c);
+}
+
+// surface nodes, orientation reference nodes, mean nodal volumes, codes and
+// the closure's device buffers
+void closure_setup(pdem_ctx* c, const float* particle_code) {
+  if (!c->multi) throw HipError{"the closure needs a multi-particle model"};
+  if (c->dim != 2) throw HipError{"the closure is implemented for 2D models"};
+  DS& s = c->ds;
+  const int64_t N = c->N;
+  std::vector<int> nonwall;
+  for (int p = 0; p < c->np; p++)
+    if (c->hp[p].type_index == 0) nonwall.push_back(p);
+  const int nn = (int)nonwall.size();
+  if (nn == 0) throw HipError{"the closure needs non-wall particles"};
+  std::vector<float> code(nn, 0.f);
+  for (int q = 0; q < nn; q++) code[q] = particle_code ? particle_code[nonwall[q]] : 0.f;
+  if (c->cl_ready) {
+    HIPCHK(hipMemcpy((void*)c->ca.code, code.data(), nn * sizeof(float), hipMemcpyHostToDevice));
+    return;
+  }
+  std::vector<double> xr0(N), xr1(N), vol(N);
+  HIPCHK(hipMemcpy(xr0.data(), s.xr[0], N * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(xr1.data(), s.xr[1], N * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(vol.data(), s.vol, N * 8, hipMemcpyDeviceToHost));
+  std::vector<uint8_t> surf(N, 0);
+  std::vector<int> sptr(nn + 1, 0), sidx, ref(nn);
+  std::vector<double> vbar(nn);
+  for (int q = 0; q < nn; q++) {
+    const auto& P = c->hp[nonwall[q]];
+    int dmax = 0;
+    for (int64_t i = P.glob_start; i < P.glob_end; i++) dmax = std::max(dmax, c->h_deg[i]);
+    // surface: nodes whose horizon is truncated by the boundary
+    double far = -1., vs = 0.;
+    int64_t fi = P.center_node;
+    for (int64_t i = P.glob_start; i < P.glob_end; i++) {
+      if (4 * c->h_deg[i] < 3 * dmax) {
+        surf[i] = 1;
+        sidx.push_back((int)i);
+      }
+      const double dx = xr0[i] - xr0[P.center_node], dy = xr1[i] - xr1[P.center_node];
+      if (dx * dx + dy * dy > far) {
+        far = dx * dx + dy * dy;
+        fi = i;
+      }
+      vs += vol[i];
+    }
+    sptr[q + 1] = (int)sidx.size();
+    ref[q] = (int)fi;
+    vbar[q] = vs / (double)(P.glob_end - P.glob_start);
+  }
+  if (sidx.empty()) sidx.push_back(0);
+  s.surf = c->upload(surf.data(), N);
+  s.surf_ptr = c->upload(sptr.data(), sptr.size());
+  s.surf_idx = c->upload(sidx.data(), sidx.size());
+  c->ca.ref_node = c->upload(ref.data(), nn);
+  c->ca.vbar = c->upload(vbar.data(), nn);
+  c->ca.code = c->upload(code.data(), nn);
+  s.cmode = c->dalloc<int8_t>((size_t)nn * s.dcap);
+  s.cout = c->dalloc<double>((size_t)nn * s.dcap * 8);
+  s.plain_any = c->dalloc<int>(nn);
+  s.wall_near = c->dalloc<int>(nn);
+  s.cpair_cap = (int)std::min<int64_t>((int64_t)nn * 16, (int64_t)nn * s.dcap);
+  s.cpair_cap = std::max(s.cpair_cap, 1);
+  s.cpairs = c->dalloc<int>(2 * (size_t)s.cpair_cap);
+  c->ca.stats = c->dalloc<unsigned long long>(8);
+  c->ca.err_sum = c->dalloc<double>(2);
+  c->cl_ready = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdem_closure_setup(pdem_ctx* c, const float* particle_code) {
+  return guarded(c, [&] { closure_setup(c, particle_code); });
+}
+
+int pdem_closure_enable(pdem_ctx* c, const pdem_mlp* net, double pen_max_h, double dz_max, int32_t check_every) {
+  return guarded(c, [&] {
+    if (!c->cl_ready) closure_setup(c, nullptr);
+    if (net) {
+      int dev = 0;
+      HIPCHK(hipGetDevice(&dev));
+      if (dev != net->device) throw HipError{"closure network lives on another device"};
+    }
+    if (check_every < 0) throw HipError{"check_every must be >= 0"};
+    c->ca.use_net = (net && pen_max_h > 0.) ? 1 : 0;
+    c->ca.net = net ? mlp::make_net(net->wb, net->wtb, net->p, net->norm) : mlp::Net{};
+    c->ca.pen_max_h = pen_max_h;
+    c->ca.dz_max = (float)dz_max;
+    c->ca.check_every = check_every;
+    c->cl_net = net;
+    if (net) HIPCHK(hipStreamSynchronize(net->stream));  // training finished before use
+    HIPCHK(hipMemsetAsync(c->ca.stats, 0, 8 * sizeof(unsigned long long), c->stream));
+    HIPCHK(hipMemsetAsync(c->ca.err_sum, 0, 2 * sizeof(double), c->stream));
+    closure_lists(c);
+    c->ds.cl_on = 1;
+    drop_graph(c);
+  });
+}
+
+int pdem_closure_disable(pdem_ctx* c) {
+  return guarded(c, [&] {
+    c->ds.cl_on = 0;
+    c->cl_net = nullptr;
+    drop_graph(c);
+  });
+}
+
+int pdem_closure_stats(pdem_ctx* c, int64_t* pairs_by_mode, int64_t* checked, double* err_mean,
+                       double* err_max, double* plain_mean) {
+  return guarded(c, [&] {
+    if (!c->cl_ready) throw HipError{"closure not set up"};
+    unsigned long long st[8];
+    double es[2];
+    HIPCHK(hipMemcpyAsync(st, c->ca.stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(es, c->ca.err_sum, sizeof(es), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 3; k++) pairs_by_mode[k] = (int64_t)st[k];
+    *checked = (int64_t)st[3];
+    *err_mean = st[3] ? es[0] / (double)st[3] : 0.;
+    *plain_mean = st[3] ? es[1] / (double)st[3] : 0.;
+    double mx;
+    std::memcpy(&mx, &st[4], 8);
+    *err_max = mx;
+  });
+}
+
+int pdem_pair_log(pdem_ctx* c, double band_h, int64_t cap, float* feat, float* targ, int64_t* n) {
+  return guarded(c, [&] {
+    if (!c->cl_ready) throw HipError{"call pdem_closure_setup before pdem_pair_log"};
+    if (cap < 0) throw HipError{"negative capacity"};
+    closure_lists(c);
+    float* df = c->dalloc<float>((size_t)std::max<int64_t>(cap, 1) * CL_NFEAT);
+    float* dt = c->dalloc<float>((size_t)std::max<int64_t>(cap, 1) * 4);
+    unsigned* dn = c->dalloc<unsigned>(1, false);
+    HIPCHK(hipMemsetAsync(dn, 0, sizeof(unsigned), c->stream));
+    DS t = c->ds;
+    t.cl_on = 1;
+    hipLaunchKernelGGL(k_pair_log, dim3(CL_BLOCKS), dim3(256), 0, c->stream, t, c->ca, band_h, df, dt, dn,
+                       (int)std::min<int64_t>(cap, INT_MAX));
+    check_launch(c);
+    unsigned cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, dn, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int64_t m = std::min<int64_t>(cnt, cap);
+    if (m > 0) {
+      HIPCHK(hipMemcpy(feat, df, m * CL_NFEAT * sizeof(float), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(targ, dt, m * 4 * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    *n = (int64_t)cnt;
+    c->dfree(df);
+    c->dfree(dt);
+    c->dfree(dn);
+  });
+}
+
 int pdem_autotune(pdem_ctx* c, int32_t reps, double* best_ms) {
   return guarded(c, [&] {
     if (reps < 1) throw HipError{"reps must be positive"};
@@ -1144,13 +1348,7 @@ int pdem_autotune(pdem_ctx* c, int32_t reps, double* best_ms) {
     HIPCHK(hipFree(keep));
     HIPCHK(hipFree(keepZ));
     // launch shapes are baked into a captured step: capture again
-    if (c->graph_ready) {
-      HIPCHK(hipGraphExecDestroy(c->gexec));
-      HIPCHK(hipGraphDestroy(c->graph));
-      c->gexec = nullptr;
-      c->graph = nullptr;
-      c->graph_ready = false;
-    }
+    drop_graph(c);
     if (best_ms) *best_ms = best;
   });
 }

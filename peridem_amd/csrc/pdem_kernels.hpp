@@ -57,6 +57,8 @@ struct Ctrl {
   int32_t vdirty;
   uint32_t nactive;  // nodes with more than LIGHT_MAX candidates (wavefront per node)
   uint32_t nlight;   // nodes with 1..LIGHT_MAX candidates (lane per node)
+  uint32_t npairs;   // closure: particle pairs listed at the last rebuild
+  uint32_t pad_;
   unsigned long long vmax_acc;  // bit pattern of max |v| (non-negative doubles order as integers)
   double vmax_last;
   // scratch results
@@ -129,6 +131,17 @@ struct DS {
   const int* pmeta;
   double rc2_max;  // largest contact radius squared: distance prefilter
   int light_max;   // nodes with at most this many candidates take one lane
+  // learned pair-contact closure (cl_on == 0: none of these is read)
+  int cl_on;
+  int* cpairs;          // [cpair_cap][2] (a, slot): pair a < dlist[a][slot], listed at rebuilds
+  int cpair_cap;
+  int8_t* cmode;        // [n_nonwall][dcap] 0: no contact possible, 1: closure, 2: plain
+  double* cout;         // [n_nonwall][dcap][8] closure force records
+  int* plain_any;       // [n_nonwall] some pair of the particle is plain this step
+  int* wall_near;       // [n_nonwall] a node has a wall candidate (set at rebuilds)
+  const uint8_t* surf;  // [N] surface node flag
+  const int* surf_ptr;  // [n_nonwall + 1] surface nodes per non-wall particle
+  const int* surf_idx;
   // node-update block maxima of |v|
   unsigned long long* blockmax;
   int nblockmax;
@@ -332,6 +345,7 @@ __device__ void ctrl_decide(Ctrl* cg) {
     c->n_rebuilds++;
     c->nactive = 0u;
     c->nlight = 0u;
+    c->npairs = 0u;
   }
   *cg = h;
 }
@@ -371,12 +385,17 @@ __global__ void __launch_bounds__(1024) k_ctrl(DS s) {
       s.pcv[d * nn + q] = s.v[d] ? s.v[d][cn] : 0.;
     }
   }
+  if (s.cl_on)
+    for (int q = t; q < nn; q += blockDim.x) s.plain_any[q] = 0;
   __syncthreads();
-  if (reb)
+  if (reb) {
     for (int64_t k = t; k < s.ncells; k += blockDim.x) {
       s.cell_count[k] = 0;
       s.cell_owner[k] = -2;
     }
+    if (s.cl_on)
+      for (int q = t; q < nn; q += blockDim.x) s.wall_near[q] = 0;
+  }
 }
 
 // ------------------------------------------------------------ node update
@@ -871,6 +890,27 @@ __device__ __forceinline__ void contact_pair(const DS& s, const double* vi, int 
   }
 }
 
+// Closure gating seen from the node-level contact loops: a node is visited
+// only if its particle has a plain pair or a wall candidate, and a candidate
+// in another non-wall particle only if that pair is plain this step.
+__device__ __forceinline__ bool node_needs_contact(const DS& s, int nwi) {
+  return !s.cl_on || nwi < 0 || s.plain_any[nwi] != 0 || s.wall_near[nwi] != 0;
+}
+__device__ __forceinline__ bool pair_plain(const DS& s, int a, int b) {
+  const int* dl = s.dlist + (int64_t)a * s.dcap;
+  int lo = 0, hi = s.dcount[a] - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int v = dl[mid];
+    if (v == b) return s.cmode[(int64_t)a * s.dcap + mid] == 2;
+    if (v < b)
+      lo = mid + 1;
+    else
+      hi = mid - 1;
+  }
+  return true;  // not a listed pair: keep the node-level path
+}
+
 // ------------------------------------------------------- contact forces
 // Candidate query on a rebuild: one lane per searching node; nodes that end
 // with candidates (and compute forces) are appended to the active list.
@@ -884,6 +924,12 @@ __global__ void __launch_bounds__(256) k_query(DS s) {
   if (!pi.searches) return;
   query_node<DIM>(s, i, pid);
   const int cnt = s.ccount[i];
+  if (s.cl_on && pi.nonwall_index >= 0)
+    for (int k = 0; k < cnt; k++)
+      if (s.pmeta[s.ptid[s.cand[(int64_t)k * s.N + i]]] >> 16) {
+        atomicOr(&s.wall_near[pi.nonwall_index], 1);
+        break;
+      }
   if (cnt > 0 && pi.compute_force) {
     if (cnt <= s.light_max)
       s.light[atomicAdd(&s.ctrl->nlight, 1u)] = (int)i;
@@ -908,6 +954,7 @@ __global__ void __launch_bounds__(256) k_contact_light(DS s) {
     const int zi = mi & 0xffff;
     const bool wi = (mi >> 16) != 0;
     const int nwi = s.P[pid].nonwall_index;
+    if (!node_needs_contact(s, nwi)) continue;
     double yi[3], vi[3];
     for (int d = 0; d < DIM; d++) {
       yi[d] = s.x[d][i];
@@ -934,6 +981,7 @@ __global__ void __launch_bounds__(256) k_contact_light(DS s) {
       // d^2 >= rc^2 implies R >= rc (1 - 2^-53): isLess(R, rc) is false
       if (d2 >= rc * rc) continue;
       if (!isLess(len3<DIM>(yji), rc)) continue;
+      if (s.cl_on && !wj && !pair_plain(s, nwi, s.P[pjid].nonwall_index)) continue;
       if (wj && nwi >= 0) {
         const int w = s.widx[j];
         atomicOr(&s.wbits[(int64_t)nwi * s.nwords + (w >> 5)], 1u << (w & 31));
@@ -993,6 +1041,7 @@ __global__ void __launch_bounds__(CWAVES * WAVE) k_contact_force(DS s) {
     const int zi = mi & 0xffff;
     const bool wi = (mi >> 16) != 0;
     const int nwi = s.P[pid].nonwall_index;
+    if (!node_needs_contact(s, nwi)) continue;
     double yi[3], vi[3];
     for (int d = 0; d < DIM; d++) {
       yi[d] = s.x[d][i];
@@ -1017,7 +1066,8 @@ __global__ void __launch_bounds__(CWAVES * WAVE) k_contact_force(DS s) {
           const double d2 = dot3<DIM>(yji, yji);
           if (d2 < ct.contact_r * ct.contact_r) {
             const double Rji = len3<DIM>(yji);
-            if (isLess(Rji, ct.contact_r)) {
+            if (isLess(Rji, ct.contact_r) &&
+                !(s.cl_on && !wj && !pair_plain(s, nwi, s.P[pjid].nonwall_index))) {
               on = true;
               if (wj && nwi >= 0) {
                 const int w = s.widx[j];
@@ -1104,7 +1154,18 @@ __global__ void __launch_bounds__(64) k_damp_pairs(DS s) {
     const unsigned long long m = __ballot(on);
     if (on) {
       const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
-      if (pos < s.dcap) s.dlist[(int64_t)q * s.dcap + pos] = qq;
+      if (pos < s.dcap) {
+        s.dlist[(int64_t)q * s.dcap + pos] = qq;
+        if (s.cl_on && qq > q) {
+          const unsigned k = atomicAdd(&s.ctrl->npairs, 1u);
+          if ((int)k < s.cpair_cap) {
+            s.cpairs[2 * (int64_t)k] = q;
+            s.cpairs[2 * (int64_t)k + 1] = pos;
+          } else {
+            s.ctrl->overflow = 5;
+          }
+        }
+      }
     }
     cnt += __popcll(m);
   }
@@ -1203,7 +1264,25 @@ __global__ void __launch_bounds__(64) k_damping(DS s) {
   }
 }
 
-// f += damping force of the node's particle (non-wall), then f += rho g
+// closure forces on a surface node: every closure pair of its particle whose
+// patch holds the node (2D; called from the force finalisation)
+__device__ __forceinline__ void closure_node_force(const DS& s, int a, int64_t i, double* fi) {
+  const int nn = s.n_nonwall;
+  const double r0 = s.x[0][i] - s.pcx[a], r1 = s.x[1][i] - s.pcx[nn + a];
+  const int cnt = s.dcount[a];
+  for (int k = 0; k < cnt; k++) {
+    const int64_t slot = (int64_t)a * s.dcap + k;
+    if (s.cmode[slot] != 1) continue;
+    const double* rec = s.cout + slot * 8;
+    if (r0 * rec[0] + r1 * rec[1] < rec[2]) continue;
+    const double q0 = r0 - rec[6], q1 = r1 - rec[7];
+    fi[0] += rec[3] - rec[5] * q1;
+    fi[1] += rec[4] + rec[5] * q0;
+  }
+}
+
+// f += damping force of the node's particle (non-wall), then the learned
+// closure's pair forces on surface nodes (closure mode only), then f += rho g
 // (computeExternalForces, demModel.cpp:791-812).
 template <int DIM>
 __global__ void k_finalize(DS s, int with_damping) {
@@ -1213,6 +1292,12 @@ __global__ void k_finalize(DS s, int with_damping) {
   const DevParticle& p = s.P[pid];
   if (with_damping && p.type_index == 0)
     for (int d = 0; d < DIM; d++) s.f[d][i] += s.fdamp[3 * pid + d];
+  if (DIM == 2 && s.cl_on && p.type_index == 0 && s.surf[i]) {
+    double fi[2] = {0., 0.};
+    closure_node_force(s, p.nonwall_index, i, fi);
+    s.f[0][i] += fi[0];
+    s.f[1][i] += fi[1];
+  }
   if (s.has_gravity)
     for (int d = 0; d < DIM; d++) s.f[d][i] += p.rho * s.gravity[d];
 }

@@ -15,7 +15,7 @@
 // k%4; B[k][j] in lane j + 16(k/4), element k%4; D[i][j] in lane j + 16(i/4),
 // element i%4.
 //
-// Training (k_mlp_grad): a workgroup of 4 wavefronts takes 64 rows, runs the
+// Training (k_mlp_grad, pdem_mlp.hip): a workgroup of 4 wavefronts takes 64 rows, runs the
 // forward and backward passes per wavefront, stages the activations and the
 // back-propagated pre-activation gradients feature-major in LDS, and forms
 // the weight gradients with the rows as the MFMA K dimension.  Each workgroup
@@ -170,134 +170,6 @@ __device__ __forceinline__ uint64_t splitmix(uint64_t x) {
   x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
   x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
   return x ^ (x >> 31);
-}
-
-// Gradient of loss = (1/B) sum_rows sum_o tw[o] (y_o - t_o)^2 over a batch of
-// B rows.  Rows: sample index = rows in order (shuffle == 0, rows >= n
-// inactive) or drawn with replacement by a hash of (shuffle, row).
-// partial: per workgroup NPARAM gradient entries + 1 loss entry.
-__global__ void __launch_bounds__(256) k_mlp_grad(Net n, const float* X, const float* T, const float* tw,
-                                                  int64_t nsamp, int64_t batch, uint64_t shuffle,
-                                                  float inv_batch, float* partial) {
-  __shared__ short XT[NIN * ROWS_WG];
-  __shared__ short HT[3][NH * ROWS_WG];
-  __shared__ short DYT[NOP * ROWS_WG];
-  __shared__ float lsum[4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
-  const int64_t brow = (int64_t)blockIdx.x * ROWS_WG + wave * 16 + j;
-  const bool active = brow < batch;
-  int64_t srow = 0;
-  if (active) srow = shuffle ? (int64_t)(splitmix(shuffle * 0x100000001b3ull + (uint64_t)brow) % (uint64_t)nsamp) : brow;
-  float x16[NIN];
-  for (int f = 0; f < NIN; f++) x16[f] = active ? X[srow * NIN + f] : n.in_mean[f];
-  const s4 xb = input_frag(n, x16, lane);
-  s4 h1[8], h2[8], h3[8];
-  const f4 y = forward(n, xb, h1, h2, h3, lane);
-  // output gradient (lanes q == 0 hold the 4 outputs)
-  s4 dy;
-  float l = 0.f;
-  for (int r = 0; r < 4; r++) {
-    float d = 0.f;
-    if (q == 0 && active) {
-      const float e = y[r] - T[srow * NOUT + r] * n.out_scale[r];
-      l += tw[r] * e * e;
-      d = 2.f * tw[r] * e * inv_batch;
-    }
-    dy[r] = bf16_rne(d);
-  }
-  for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
-  if (lane == 0) lsum[wave] = l;
-  store_fm(XT, &xb, 1, wave, lane);
-  store_fm(HT[0], h1, 8, wave, lane);
-  store_fm(HT[1], h2, 8, wave, lane);
-  store_fm(HT[2], h3, 8, wave, lane);
-  store_fm(DYT, &dy, 1, wave, lane);
-  __syncthreads();
-  float* g = partial + (size_t)blockIdx.x * (NPARAM + 1);
-  if (threadIdx.x == 0) g[NPARAM] = (lsum[0] + lsum[1] + lsum[2] + lsum[3]) * inv_batch;
-  weight_grad(HT[2], DYT, NH, NOP, NOUT, g + OW4, wave, lane);
-  bias_grad(DYT, NOUT, g + OB4);
-  // dz3 = mask(h3) * W4 dy
-  s4 dz[8], dzn[8];
-  for (int ib = 0; ib < 8; ib++) {
-    f4 acc = {0.f, 0.f, 0.f, 0.f};
-    acc = mfma(frag(n.w + BW4, NOP, ib * 16, 0, lane), dy, acc);
-    s4 o;
-    for (int r = 0; r < 4; r++) o[r] = bf16_f(h3[ib][r]) > 0.f ? bf16_rne(acc[r]) : (short)0;
-    dz[ib] = o;
-  }
-  __syncthreads();  // H3 consumed: its LDS now holds dZ3
-  store_fm(HT[2], dz, 8, wave, lane);
-  __syncthreads();
-  weight_grad(HT[1], HT[2], NH, NH, NH, g + OW3, wave, lane);
-  bias_grad(HT[2], NH, g + OB3);
-  back_hidden(n, BW3, dz, h2, dzn, lane);
-  __syncthreads();
-  store_fm(HT[1], dzn, 8, wave, lane);
-  __syncthreads();
-  weight_grad(HT[0], HT[1], NH, NH, NH, g + OW2, wave, lane);
-  bias_grad(HT[1], NH, g + OB2);
-  back_hidden(n, BW2, dzn, h1, dz, lane);
-  __syncthreads();
-  store_fm(HT[0], dz, 8, wave, lane);
-  __syncthreads();
-  weight_grad(XT, HT[0], NIN, NH, NH, g + OW1, wave, lane);
-  bias_grad(HT[0], NH, g + OB1);
-}
-
-// bf16 copies of one master parameter
-__device__ __forceinline__ void pack_one(int i, float w, short* wb, short* wtb) {
-  int bw, nin, nout_p, nout, k;
-  if (i < OB1) { bw = BW1; nin = NIN; nout_p = NH; nout = NH; k = i - OW1; }
-  else if (i >= OW2 && i < OB2) { bw = BW2; nin = NH; nout_p = NH; nout = NH; k = i - OW2; }
-  else if (i >= OW3 && i < OB3) { bw = BW3; nin = NH; nout_p = NH; nout = NH; k = i - OW3; }
-  else if (i >= OW4 && i < OB4) { bw = BW4; nin = NH; nout_p = NOP; nout = NOUT; k = i - OW4; }
-  else return;  // biases stay FP32
-  const int a = k / nout, b = k % nout;
-  const short h = bf16_rne(w);
-  wb[bw + a * nout_p + b] = h;
-  wtb[bw + b * nin + a] = h;
-}
-
-__global__ void k_mlp_pack(const float* p, short* wb, short* wtb) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < NPARAM) pack_one(i, p[i], wb, wtb);
-}
-
-// Sum of the workgroup partials (fixed order) and one Adam step.
-__global__ void k_mlp_adam(float* p, float* m, float* v, short* wb, short* wtb, const float* partial,
-                           int nwg, float lr, float b1, float b2, float eps, float c1, float c2, float* grad_out,
-                           float* loss_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > NPARAM) return;
-  float g = 0.f;
-  for (int w = 0; w < nwg; w++) g += partial[(size_t)w * (NPARAM + 1) + i];
-  if (i == NPARAM) {
-    if (loss_out) *loss_out = g;
-    return;
-  }
-  if (grad_out) grad_out[i] = g;
-  if (lr == 0.f) return;
-  const float mi = b1 * m[i] + (1.f - b1) * g;
-  const float vi = b2 * v[i] + (1.f - b2) * g * g;
-  m[i] = mi;
-  v[i] = vi;
-  const float w = p[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
-  p[i] = w;
-  pack_one(i, w, wb, wtb);
-}
-
-// Inference on host-provided rows: Y[n][4] de-normalised (y / out_scale).
-__global__ void __launch_bounds__(256) k_mlp_infer(Net n, const float* X, int64_t nrows, float* Y) {
-  const int lane = threadIdx.x & 63, q = lane >> 4, j = lane & 15;
-  const int64_t row = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16 + j;
-  const bool active = row < nrows;
-  float x16[NIN];
-  for (int f = 0; f < NIN; f++) x16[f] = active ? X[row * NIN + f] : n.in_mean[f];
-  s4 h1[8], h2[8], h3[8];
-  const f4 y = forward(n, input_frag(n, x16, lane), h1, h2, h3, lane);
-  if (q == 0 && active)
-    for (int r = 0; r < 4; r++) Y[row * NOUT + r] = y[r] / n.out_scale[r];
 }
 
 }  // namespace mlp

@@ -1,0 +1,122 @@
+"""Learned pair-contact closure on the GPU.
+
+* Gating without a network (empty closure band) must leave the step exactly
+  the plain step: pairs whose geometric gap exceeds the contact radius cannot
+  have a node pair within it (every node pair is at least `gap` apart), so
+  skipping them in the node-level kernels changes no bit.
+* The logged plain pair force of a two-particle collision (no damping,
+  friction or gravity) equals the particle's total force sum_i V_i f_i, since
+  bond forces are pairwise balanced inside a particle.
+* A network trained on ensemble data by the engine's own kernels predicts the
+  plain pair force far better than the untrained network, and the
+  closure-accelerated run follows the plain run.
+"""
+import numpy as np
+import pytest
+
+from peridem_amd import capi, closure, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _ensemble(n=4, seed=5):
+    return configs.ensemble_2d(n_instances=n, n_min=2, n_max=6, ratios=(6,), speed_lo=2.0, speed_hi=4.0,
+                               pitch_factor=1.0, R_lo=1.0e-3, R_hi=1.0e-3, seed=seed)
+
+
+def _compressive():
+    return configs.compressive_2d(n_particles=12, nodes_per_particle=150, seed=3, init_speed=3.0,
+                                  pitch_factor=1.03)
+
+
+@pytest.mark.parametrize("which", ["ensemble", "compressive"])
+def test_gating_without_network_equals_plain(which):
+    spec = _ensemble() if which == "ensemble" else _compressive()
+    K = 400
+    print(f"[closure-gate] {which} nodes={spec.n_nodes} steps={K}", flush=True)
+    a = capi.GpuEngine(spec, capi.PREC_FP64)
+    b = capi.GpuEngine(spec, capi.PREC_FP64)
+    b.closure_setup(closure.particle_codes(spec))
+    b.closure_enable(None, pen_max_h=0.05, check_every=0)
+    for e in (a, b):
+        e.integrate_begin()
+        e.integrate_cd_graph(K)
+    for fid in (capi.FIELD_F, capi.FIELD_U, capi.FIELD_V):
+        np.testing.assert_array_equal(a.field(fid), b.field(fid))
+    st = b.closure_stats()
+    print(f"[closure-gate] {which} stats {st}", flush=True)
+    assert st["closure"] == 0 and st["plain"] > 0 and st["no_contact"] > 0
+    assert np.abs(a.field(capi.FIELD_F)).max() > 0
+
+
+def test_pair_log_matches_particle_force():
+    spec = configs.two_particle(gap=0.09e-3, speed=0.1, dt=1.0e-7)
+    g = capi.GpuEngine(spec, capi.PREC_FP64)
+    g.closure_setup()
+    g.integrate_begin()
+    g.integrate_cd(20)
+    X, T = g.pair_log(band_h=0.5, cap=16)
+    assert len(X) == 1
+    F = g.field(capi.FIELD_F)
+    x = g.field(capi.FIELD_X)
+    vol = spec.vol
+    p0, p1 = spec.particles
+    n0, n1 = p0["glob_start"], p0["glob_end"]
+    Fsum = (F[n0:n1, :2] * vol[n0:n1, None]).sum(0)
+    c0, c1 = x[p0["center_node"], :2], x[p1["center_node"], :2]
+    nvec = (c1 - c0) / np.linalg.norm(c1 - c0)
+    ct = spec.contact[p0["zone"]][p1["zone"]]
+    Kn, Rc = ct["Kn"], ct["contact_r"]
+    vbar0 = vol[n0:n1].mean()
+    vbar1 = vol[p1["glob_start"]:p1["glob_end"]].mean()
+    unit = Kn * Rc * vbar0 * vbar1
+    Flog = unit * (T[0, 0] * nvec + T[0, 1] * np.array([-nvec[1], nvec[0]]))
+    print(f"[closure-log] F particle sum {Fsum}, logged {Flog}, gap/Rc {X[0, 0]:.4f}", flush=True)
+    assert np.linalg.norm(Fsum) > 0
+    assert np.linalg.norm(Flog - Fsum) <= 1e-5 * np.linalg.norm(Fsum)
+    assert X[0, 0] < 1.0  # in contact: gap below the contact radius
+
+
+def test_trained_closure_beats_untrained_and_tracks_plain():
+    train = configs.ensemble_2d(n_instances=48, n_min=2, n_max=9, ratios=(6, 8), speed_lo=0.5, speed_hi=2.0,
+                                pitch_factor=1.0, R_lo=1.0e-3, R_hi=1.0e-3, seed=100)
+    X, T = closure.collect(train, steps=1200, log_every=4, band_h=0.5)
+    Xs, Ts = closure.select(X, T)
+    print(f"[closure-train] rows {len(X)} selected {len(Xs)}", flush=True)
+    assert len(Xs) > 2000
+    rng = np.random.default_rng(0)
+    idx = rng.permutation(len(Xs))
+    cut = int(0.85 * len(Xs))
+    net, curve = closure.fit(Xs[idx[:cut]], Ts[idx[:cut]], iters=2500, batch=512, lr=2e-3, seed=1)
+    rel = closure.fit_report(net, Xs[idx[cut:]], Ts[idx[cut:]])
+    print(f"[closure-train] loss {curve[:10].mean():.4f} -> {curve[-50:].mean():.5f}; held-out rel RMS {rel}",
+          flush=True)
+    assert curve[-50:].mean() < 0.5 * curve[:10].mean()
+    untrained = capi.Mlp(seed=1)
+    nrm = net.normalization()
+    untrained.set_normalization(*nrm)
+
+    test = configs.ensemble_2d(n_instances=16, n_min=2, n_max=9, ratios=(6, 8), speed_lo=0.5, speed_hi=2.0,
+                               pitch_factor=1.0, R_lo=1.0e-3, R_hi=1.0e-3, seed=900)
+    K = 1500
+    res = {}
+    for tag, m in (("plain", None), ("trained", net), ("untrained", untrained)):
+        e = capi.GpuEngine(test, capi.PREC_MIXED)
+        e.closure_setup(closure.particle_codes(test))
+        if m is not None:
+            e.closure_enable(m, pen_max_h=0.05, dz_max=0.5, check_every=10)
+        e.integrate_begin()
+        e.integrate_cd_graph(K)
+        res[tag] = (e.field(capi.FIELD_V), e.closure_stats() if m is not None else None)
+        e.close()
+    st_t, st_u = res["trained"][1], res["untrained"][1]
+    print(f"[closure-run] trained {st_t}", flush=True)
+    print(f"[closure-run] untrained {st_u}", flush=True)
+    assert st_t["closure"] > 0 and st_t["checked"] > 0
+    assert st_t["check_err_mean"] < 0.5 * st_u["check_err_mean"]
+    # centre velocities of the particles: closure run against plain run
+    vp, vt = res["plain"][0], res["trained"][0]
+    cn = np.array([p["center_node"] for p in test.particles])
+    dv = np.abs(vt[cn] - vp[cn]).max() / np.abs(vp[cn]).max()
+    print(f"[closure-run] max centre-velocity deviation {dv:.3e} of max speed", flush=True)
+    assert dv < 0.05
