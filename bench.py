@@ -111,6 +111,70 @@ def cpu_baseline(spec, nthreads, budget_s=12.0, max_steps=-1):
                       f"({t1 - t0:.1f} s) and 2 warm-up steps; FP64 oracle, {nthreads} threads"}
 
 
+def closure_comparison(spec, steps, sample_every, train_instances, train_steps, verbose=True):
+    """Learned closure vs plain method on `spec` (FP64): train the network on a
+    batched ensemble (plain method, mixed precision), then run `steps` steps
+    of both forms; returns step times, the reaction-force-curve error and the
+    closure statistics."""
+    import numpy as np
+    from peridem_amd import capi, closure, configs
+
+    t0 = time.perf_counter()
+    train = configs.ensemble_2d(n_instances=train_instances, seed=1000, pitch_factor=1.0)
+    X, T = closure.collect(train, steps=train_steps, log_every=10, band_h=0.5, cap=1 << 19)
+    Xs, Ts = closure.select(X, T)
+    t1 = time.perf_counter()
+    net, curve = closure.fit(Xs, Ts, iters=4000, batch=1024, lr=2e-3, seed=0)
+    t2 = time.perf_counter()
+    if verbose:
+        print(f"# closure: {len(X)} logged rows, {len(Xs)} near the band ({t1 - t0:.1f} s); "
+              f"training {t2 - t1:.1f} s, loss {curve[:10].mean():.4f} -> {curve[-50:].mean():.5f}",
+              file=sys.stderr, flush=True)
+    out = {}
+    curves = {}
+    for tag in ("plain", "closure"):
+        e = capi.GpuEngine(spec, capi.PREC_FP64)
+        e.closure_setup(closure.particle_codes(spec))
+        if tag == "closure":
+            e.closure_enable(net, pen_max_h=0.05, dz_max=0.5, check_every=100)
+        e.integrate_begin()
+        e.integrate_cd_graph(10)
+        e.synchronize()
+        rc, tt = [], 0.0
+        done = 0
+        while done < steps:
+            k = min(sample_every, steps - done)
+            ta = time.perf_counter()
+            e.integrate_cd_graph(k)
+            e.synchronize()
+            tt += time.perf_counter() - ta
+            done += k
+            pen, reac = e.pp_compressive_test(spec.wall_id, spec.f_dir)
+            cn = [p["center_node"] for p in spec.particles if p["type_index"] == 0]
+            v = e.field(capi.FIELD_V)[cn, :2]
+            rc.append((done, reac, float(np.sqrt((v ** 2).sum(1).mean()))))
+        out[tag] = {"ms_per_step": round(tt / steps * 1e3, 5)}
+        if tag == "closure":
+            out[tag]["stats"] = e.closure_stats()
+        curves[tag] = np.array(rc)
+        e.close()
+    rp, rcl = curves["plain"], curves["closure"]
+    reac_err = float(np.abs(rcl[:, 1] - rp[:, 1]).max() / max(np.abs(rp[:, 1]).max(), 1e-300))
+    vrms_err = float(np.abs(rcl[:, 2] - rp[:, 2]).max() / max(np.abs(rp[:, 2]).max(), 1e-300))
+    return {
+        "workload": spec.name, "steps": steps,
+        "speedup": round(out["plain"]["ms_per_step"] / out["closure"]["ms_per_step"], 4),
+        "ms_per_step_plain": out["plain"]["ms_per_step"],
+        "ms_per_step_closure": out["closure"]["ms_per_step"],
+        "reaction_curve_err": reac_err, "centre_speed_rms_curve_err": vrms_err,
+        "pairs": out["closure"]["stats"],
+        "train": {"ensemble_instances": train_instances, "ensemble_steps": train_steps,
+                  "rows_logged": int(len(X)), "rows_used": int(len(Xs)),
+                  "held_out_rel_rms": [round(float(r), 4) for r in closure.fit_report(net, Xs[::7], Ts[::7])],
+                  "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +188,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--prof-steps", type=int, default=300)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-closure", action="store_true")
+    ap.add_argument("--closure-steps", type=int, default=50000)
+    ap.add_argument("--closure-train-instances", type=int, default=512)
+    ap.add_argument("--closure-train-steps", type=int, default=5000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU (particle-sharded) "
@@ -188,6 +256,12 @@ def main():
     broken, total = eng.broken_bonds()
     eng.close()
 
+    closure_res = None
+    if not args.no_closure and args.workload == "compressive_2d":
+        closure_res = closure_comparison(spec, args.closure_steps, 500, args.closure_train_instances,
+                                         args.closure_train_steps)
+        print(f"# closure: {closure_res}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -204,7 +278,7 @@ def main():
                    "sim_time_end": t, "contact_rebuilds": st["n_rebuilds"],
                    "contact_interval": st["interval"], "contact_candidates": n_cand,
                    "broken_bond_fraction": broken / max(1, total), "launch": tuned},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "closure": closure_res,
     }
     print(json.dumps(out), flush=True)
 

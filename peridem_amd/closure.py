@@ -20,8 +20,9 @@ import numpy as np
 from . import capi
 
 # rows used for training: pairs whose penetration Rc - gap is below this many
-# mesh sizes (wider than the closure band so the fit is smooth at its edge)
-TRAIN_PEN_H = 0.3
+# mesh sizes (twice the closure band, so the fit is smooth at its edge and the
+# target scale is that of the band)
+TRAIN_PEN_H = 0.1
 OUT_WEIGHT = (1.0, 1.0, 1.0, 0.1)
 
 
@@ -55,11 +56,16 @@ def collect(spec, steps, log_every=10, band_h=0.5, precision=capi.PREC_MIXED, ca
     return X, T
 
 
+# feature columns (pdem_closure.hpp, pair_features)
+F_GAP, F_DMIN, F_OVL = 0, 14, 15
+
+
 def select(X, T, pen_h=TRAIN_PEN_H):
-    """Rows near the closure band: penetration below pen_h mesh sizes of the
-    finer particle, i.e. gap / Rc >= 1 - pen_h * min(h_a, h_b) / Rc."""
+    """Rows the network would see: node pairs in contact (nearest patch pair
+    closer than Rc, extent gap >= 0) with penetration Rc - dmin below pen_h
+    mesh sizes of the finer particle."""
     hmin_rc = np.minimum(X[:, 1], X[:, 2])
-    keep = X[:, 0] >= 1.0 - pen_h * hmin_rc
+    keep = (X[:, F_GAP] >= 0.0) & (X[:, F_DMIN] < 1.0) & (X[:, F_DMIN] >= 1.0 - pen_h * hmin_rc)
     return X[keep], T[keep]
 
 

@@ -8,17 +8,19 @@
 //             along -n; gap = |c_b - c_a| - ext_a - ext_b.  Every node pair
 //             (i in a, j in b) is at least `gap` apart, so gap >= Rc means no
 //             contact force is possible (mode 0, exact).
-//   gating    penetration Rc - gap <= pen_max_h * min(h_a, h_b), no damage in
-//             either particle (max Z < 1) and a predicted local damage below
-//             dz_max: closure (mode 1); otherwise the plain node-level contact
-//             kernels handle the pair (mode 2).
+//   patches   surface nodes within Rc of the extents; for gap >= 0 every node
+//             pair closer than Rc is a patch pair, so the nearest patch pair
+//             dmin >= Rc also means no contact force (mode 0, exact).
+//   gating    penetration Rc - dmin <= pen_max_h * min(h_a, h_b), gap >= 0,
+//             no damage in either particle (max Z < 1) and a predicted local
+//             damage below dz_max: closure (mode 1); otherwise the plain
+//             node-level contact kernels handle the pair (mode 2).
 //   network   16 features -> (F.n, F.t, M_a, local damage) in units of
 //             Kn Rc Vbar_a Vbar_b (and h_a for M), BF16 MFMA (pdem_mlp.hpp).
-//   transfer  the pair force F on a (-F on b) and the moments M_a, M_b are
-//             spread over the patch of surface nodes within Rc of the extent:
-//             a uniform force density F / sum V plus a torque term
-//             k perp(x - xbar) with zero net force, so the patch receives
-//             exactly F and M.
+//   transfer  the pair force F on a (-F on b) and the moments M_a, M_b go to
+//             the patch nodes in proportion to their overlap weights (the
+//             spatial pattern of the plain node-level forces) plus a torque
+//             term with zero net force, so each patch receives exactly F and M.
 //
 // A sampled check every `check_every` steps evaluates the plain node-level
 // pair force of every closure pair from the candidate lists and accumulates
@@ -131,8 +133,62 @@ __device__ __forceinline__ double omega(const DS& s, const ClosureArgs& ca, int 
   return r2 > 0. ? (rx * vy - ry * vx) / r2 : 0.;
 }
 
-__device__ __forceinline__ void pair_features(const DS& s, const ClosureArgs& ca, const PairGeo& g, double za,
-                                              double zb, float* x16) {
+// Nearest node pair and overlap of the contact patches (surface nodes within
+// Rc of the extents).  With gap >= 0 every node pair closer than Rc is a
+// patch pair (a node outside a's patch is more than gap + Rc from every node
+// of b along n), so dmin >= Rc means no contact force (exact).
+// Overlap weight of a patch pair: c_ij = (1 - R_ij / Rc) V_i V_j for
+// R_ij < Rc.  ovl = sum c_ij / (Vbar_a Vbar_b); the first moments of the
+// weights about each centre (P.sa, P.sb) and second moments (P.saa, P.sbb)
+// give the weighted patch centroids and polar moments of the force transfer.
+struct PatchSums {
+  double dmin, W, sa[2], saa, sb[2], sbb;
+};
+
+__device__ __forceinline__ void patch_contact(const DS& s, const PairGeo& g, int sub, bool on, PatchSums& P) {
+  double dm = 1e300, w = 0., a0 = 0., a1 = 0., aa = 0., b0s = 0., b1s = 0., bb = 0.;
+  if (on) {
+    const double thr_a = g.ext_a - g.rc, thr_b = g.ext_b - g.rc;
+    const int b0 = s.surf_ptr[g.b], b1 = s.surf_ptr[g.b + 1];
+    for (int k = s.surf_ptr[g.a] + sub; k < s.surf_ptr[g.a + 1]; k += 4) {
+      const int i = s.surf_idx[k];
+      const double xi0 = s.x[0][i], xi1 = s.x[1][i];
+      const double ri0 = xi0 - g.ca[0], ri1 = xi1 - g.ca[1];
+      if (ri0 * g.n[0] + ri1 * g.n[1] < thr_a) continue;
+      const double Vi = s.vol[i];
+      for (int m = b0; m < b1; m++) {
+        const int j = s.surf_idx[m];
+        const double xj0 = s.x[0][j], xj1 = s.x[1][j];
+        const double rj0 = xj0 - g.cb[0], rj1 = xj1 - g.cb[1];
+        if (-(rj0 * g.n[0] + rj1 * g.n[1]) < thr_b) continue;
+        const double dx = xj0 - xi0, dy = xj1 - xi1;
+        const double R = sqrt(dx * dx + dy * dy);
+        dm = fmin(dm, R);
+        if (R < g.rc) {
+          const double c = (1. - R / g.rc) * Vi * s.vol[j];
+          w += c;
+          a0 += c * ri0;
+          a1 += c * ri1;
+          aa += c * (ri0 * ri0 + ri1 * ri1);
+          b0s += c * rj0;
+          b1s += c * rj1;
+          bb += c * (rj0 * rj0 + rj1 * rj1);
+        }
+      }
+    }
+  }
+  P.dmin = -sub_max(-dm);
+  P.W = sub_sum(w);
+  P.sa[0] = sub_sum(a0);
+  P.sa[1] = sub_sum(a1);
+  P.saa = sub_sum(aa);
+  P.sb[0] = sub_sum(b0s);
+  P.sb[1] = sub_sum(b1s);
+  P.sbb = sub_sum(bb);
+}
+
+__device__ __forceinline__ void pair_features(const DS& s, const ClosureArgs& ca, const PairGeo& g, double dmin,
+                                              double ovl, float* x16) {
   const int nn = s.n_nonwall;
   const double t[2] = {-g.n[1], g.n[0]};
   const double dvx = s.pcv[g.b] - s.pcv[g.a], dvy = s.pcv[nn + g.b] - s.pcv[nn + g.a];
@@ -151,8 +207,8 @@ __device__ __forceinline__ void pair_features(const DS& s, const ClosureArgs& ca
   x16[11] = ca.code[g.a];
   x16[12] = ca.code[g.b];
   x16[13] = (float)g.ct.mu;
-  x16[14] = (float)za;
-  x16[15] = (float)zb;
+  x16[14] = (float)(dmin / g.rc);
+  x16[15] = (float)ovl;
 }
 
 // force unit of a pair: Kn Rc Vbar_a Vbar_b
@@ -205,49 +261,36 @@ __device__ __forceinline__ void plain_pair(const DS& s, const PairGeo& g, int su
   zloc = sub_max(z);
 }
 
-// patch of q's surface nodes with (x - c).dir >= thr: sum V, sum V r, sum V |r|^2 (r = x - c)
-__device__ __forceinline__ void patch_sums(const DS& s, int q, const double* c, const double* dir, double thr,
-                                           int sub, bool on, double& sv, double* sx, double& sxx) {
-  double a = 0., b0 = 0., b1 = 0., e2 = 0.;
-  if (on)
-    for (int k = s.surf_ptr[q] + sub; k < s.surf_ptr[q + 1]; k += 4) {
-      const int i = s.surf_idx[k];
-      const double r0 = s.x[0][i] - c[0], r1 = s.x[1][i] - c[1];
-      if (r0 * dir[0] + r1 * dir[1] < thr) continue;
-      const double V = s.vol[i];
-      a += V;
-      b0 += V * r0;
-      b1 += V * r1;
-      e2 += V * (r0 * r0 + r1 * r1);
-    }
-  sv = sub_sum(a);
-  sx[0] = sub_sum(b0);
-  sx[1] = sub_sum(b1);
-  sxx = sub_sum(e2);
-}
-
-// record of one side of a closure pair: dir, thr, uniform force density,
-// torque coefficient, patch centroid (relative to the particle centre)
-__device__ __forceinline__ void closure_record(double* rec, const double* dir, double thr, const double* F,
-                                               double Mc, double sv, const double* sx, double sxx) {
-  double xb[2] = {0., 0.}, f0[2] = {0., 0.}, k = 0.;
-  if (sv > 0.) {
-    xb[0] = sx[0] / sv;
-    xb[1] = sx[1] / sv;
-    f0[0] = F[0] / sv;
-    f0[1] = F[1] / sv;
-    const double I = sxx - sv * (xb[0] * xb[0] + xb[1] * xb[1]);
-    const double dM = Mc - (xb[0] * F[1] - xb[1] * F[0]);
-    k = I > 0. ? dM / I : 0.;
+// Record of one side of a closure pair (CL_REC doubles): the pair direction
+// (own centre to partner), own and partner patch thresholds, the force per
+// unit overlap weight g = F / W, the torque coefficient k and the weighted
+// patch centroid (relative to the own centre).  A patch node i receives
+// w_i (g + k perp(r_i - xbar)), w_i = sum_j (1 - R_ij / Rc) V_j over the
+// partner's patch nodes closer than Rc: the patch then carries exactly F and,
+// about the centre, M (the torque term has no net force).
+__device__ __forceinline__ void closure_record(double* rec, const double* dir, double thr_own, double thr_partner,
+                                               const double* F, double M, double W, const double* s1, double s2,
+                                               double rc) {
+  double g[2] = {0., 0.}, xb[2] = {0., 0.}, k = 0.;
+  if (W > 0.) {
+    g[0] = F[0] / W;
+    g[1] = F[1] / W;
+    xb[0] = s1[0] / W;
+    xb[1] = s1[1] / W;
+    const double I = s2 - W * (xb[0] * xb[0] + xb[1] * xb[1]);
+    const double dM = M - (xb[0] * F[1] - xb[1] * F[0]);
+    k = I > 1e-12 * s2 ? dM / I : 0.;
   }
   rec[0] = dir[0];
   rec[1] = dir[1];
-  rec[2] = thr;
-  rec[3] = f0[0];
-  rec[4] = f0[1];
-  rec[5] = k;
-  rec[6] = xb[0];
-  rec[7] = xb[1];
+  rec[2] = thr_own;
+  rec[3] = thr_partner;
+  rec[4] = g[0];
+  rec[5] = g[1];
+  rec[6] = k;
+  rec[7] = xb[0];
+  rec[8] = xb[1];
+  rec[9] = rc;
 }
 
 __device__ __forceinline__ int find_slot(const DS& s, int q, int target) {
@@ -273,17 +316,25 @@ __global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
     const bool valid = k < npairs;
     PairGeo g;
     pair_geometry(s, k, valid, sub, g);
-    const bool near = valid && g.gap < g.rc;
+    bool near = valid && g.gap < g.rc;
+    // patches: nearest node pair (exact no-contact test), overlap weights
+    PatchSums P;
+    patch_contact(s, g, sub, near && g.gap >= 0., P);
+    if (near && g.gap >= 0. && P.dmin >= g.rc) near = false;  // no node pair within Rc
+    const double dmin = (near && g.gap >= 0.) ? P.dmin : g.gap;
+    const double ovl = P.W / (ca.vbar[g.a] * ca.vbar[g.b]);
     const double za = particle_zmax(s, g.pa, sub, near);
     const double zb = particle_zmax(s, g.pb, sub, near);
     int mode = 0;
     if (near)
-      mode = (ca.use_net && g.rc - g.gap <= ca.pen_max_h * fmin(g.ha, g.hb) && za < 1. && zb < 1.) ? 1 : 2;
+      mode = (ca.use_net && g.gap >= 0. && g.rc - dmin <= ca.pen_max_h * fmin(g.ha, g.hb) && za < 1. && zb < 1.)
+                 ? 1
+                 : 2;
     double Fa[2] = {0., 0.}, Ma = 0.;
     const double unit = pair_unit(ca, g);
     if (ca.use_net) {
       float x16[CL_NFEAT];
-      pair_features(s, ca, g, za, zb, x16);
+      pair_features(s, ca, g, dmin, ovl, x16);
       mlp::s4 h1[8], h2[8], h3[8];
       const mlp::f4 y = mlp::forward(ca.net, mlp::input_frag(ca.net, x16, lane), h1, h2, h3, lane);
       double ys[4];
@@ -295,9 +346,6 @@ __global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
       Ma = ys[2] * unit * g.ha;
     }
     const double nb[2] = {-g.n[0], -g.n[1]};
-    double sva, sxa[2], sxxa, svb, sxb[2], sxxb;
-    patch_sums(s, g.a, g.ca, g.n, g.ext_a - g.rc, sub, mode == 1, sva, sxa, sxxa);
-    patch_sums(s, g.b, g.cb, nb, g.ext_b - g.rc, sub, mode == 1, svb, sxb, sxxb);
     const int sb = valid ? find_slot(s, g.b, g.a) : -1;
     if (valid && sub == 0) {
       const int64_t ia = (int64_t)g.a * s.dcap + g.sa;
@@ -311,8 +359,10 @@ __global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
         const double Fb[2] = {-Fa[0], -Fa[1]};
         // torque balance: M_b = -M_a + (c_b - c_a) x F_a
         const double Mb = -Ma + ((g.cb[0] - g.ca[0]) * Fa[1] - (g.cb[1] - g.ca[1]) * Fa[0]);
-        closure_record(s.cout + ia * 8, g.n, g.ext_a - g.rc, Fa, Ma, sva, sxa, sxxa);
-        closure_record(s.cout + ((int64_t)g.b * s.dcap + sb) * 8, nb, g.ext_b - g.rc, Fb, Mb, svb, sxb, sxxb);
+        const double ta = g.ext_a - g.rc, tb = g.ext_b - g.rc;
+        closure_record(s.cout + ia * CL_REC, g.n, ta, tb, Fa, Ma, P.W, P.sa, P.saa, g.rc);
+        closure_record(s.cout + ((int64_t)g.b * s.dcap + sb) * CL_REC, nb, tb, ta, Fb, Mb, P.W, P.sb, P.sbb,
+                       g.rc);
       }
       if (sb < 0) s.ctrl->overflow = 6;
       atomicAdd(&ca.stats[mode], 1ull);
@@ -344,15 +394,17 @@ __global__ void __launch_bounds__(256) k_pair_log(DS s, ClosureArgs ca, double b
     PairGeo g;
     pair_geometry(s, k, valid, sub, g);
     const bool take = valid && g.gap < g.rc + band_h * fmin(g.ha, g.hb);
-    const double za = particle_zmax(s, g.pa, sub, take);
-    const double zb = particle_zmax(s, g.pb, sub, take);
+    PatchSums P;
+    patch_contact(s, g, sub, take && g.gap >= 0., P);
+    const double dmin = (take && g.gap >= 0.) ? P.dmin : g.gap;
+    const double ovl = P.W / (ca.vbar[g.a] * ca.vbar[g.b]);
     double Fp[2], Mp, zl;
     plain_pair(s, g, sub, take, Fp, Mp, zl);
     if (take && sub == 0) {
       const unsigned row = atomicAdd(count, 1u);
       if ((int)row < cap) {
         float x16[CL_NFEAT];
-        pair_features(s, ca, g, za, zb, x16);
+        pair_features(s, ca, g, dmin, ovl, x16);
         for (int f = 0; f < CL_NFEAT; f++) feat[(int64_t)row * CL_NFEAT + f] = x16[f];
         const double unit = pair_unit(ca, g);
         targ[(int64_t)row * 4 + 0] = (float)((Fp[0] * g.n[0] + Fp[1] * g.n[1]) / unit);

@@ -1177,7 +1177,7 @@ void closure_setup(pdem_ctx* c, const float* particle_code) {
   c->ca.vbar = c->upload(vbar.data(), nn);
   c->ca.code = c->upload(code.data(), nn);
   s.cmode = c->dalloc<int8_t>((size_t)nn * s.dcap);
-  s.cout = c->dalloc<double>((size_t)nn * s.dcap * 8);
+  s.cout = c->dalloc<double>((size_t)nn * s.dcap * CL_REC);
   s.plain_any = c->dalloc<int>(nn);
   s.wall_near = c->dalloc<int>(nn);
   s.cpair_cap = (int)std::min<int64_t>((int64_t)nn * 16, (int64_t)nn * s.dcap);

@@ -24,6 +24,7 @@ namespace pdem {
 constexpr int WAVE = 64;
 constexpr int LIGHT_MAX_C = 16;  // candidate count up to which one lane takes a node
 constexpr double COMPARE_EPS = 1e-5;
+constexpr int CL_REC = 10;       // doubles per closure force record
 
 struct DevParticle {
   int64_t glob_start, glob_end, center;
@@ -136,7 +137,7 @@ struct DS {
   int* cpairs;          // [cpair_cap][2] (a, slot): pair a < dlist[a][slot], listed at rebuilds
   int cpair_cap;
   int8_t* cmode;        // [n_nonwall][dcap] 0: no contact possible, 1: closure, 2: plain
-  double* cout;         // [n_nonwall][dcap][8] closure force records
+  double* cout;         // [n_nonwall][dcap][CL_REC] closure force records
   int* plain_any;       // [n_nonwall] some pair of the particle is plain this step
   int* wall_near;       // [n_nonwall] a node has a wall candidate (set at rebuilds)
   const uint8_t* surf;  // [N] surface node flag
@@ -1264,20 +1265,35 @@ __global__ void __launch_bounds__(64) k_damping(DS s) {
   }
 }
 
-// closure forces on a surface node: every closure pair of its particle whose
-// patch holds the node (2D; called from the force finalisation)
+// closure forces on a surface node (2D; called from the force finalisation):
+// for every closure pair of its particle whose patch holds the node, the
+// node's overlap weight w_i = sum_j (1 - R_ij / Rc) V_j over the partner's
+// patch nodes closer than Rc times (g + k perp(r_i - xbar)).
 __device__ __forceinline__ void closure_node_force(const DS& s, int a, int64_t i, double* fi) {
   const int nn = s.n_nonwall;
-  const double r0 = s.x[0][i] - s.pcx[a], r1 = s.x[1][i] - s.pcx[nn + a];
+  const double xi0 = s.x[0][i], xi1 = s.x[1][i];
+  const double r0 = xi0 - s.pcx[a], r1 = xi1 - s.pcx[nn + a];
   const int cnt = s.dcount[a];
   for (int k = 0; k < cnt; k++) {
     const int64_t slot = (int64_t)a * s.dcap + k;
     if (s.cmode[slot] != 1) continue;
-    const double* rec = s.cout + slot * 8;
+    const double* rec = s.cout + slot * CL_REC;
     if (r0 * rec[0] + r1 * rec[1] < rec[2]) continue;
-    const double q0 = r0 - rec[6], q1 = r1 - rec[7];
-    fi[0] += rec[3] - rec[5] * q1;
-    fi[1] += rec[4] + rec[5] * q0;
+    const int b = s.dlist[slot];
+    const double cb0 = s.pcx[b], cb1 = s.pcx[nn + b], rc = rec[9];
+    double w = 0.;
+    for (int m = s.surf_ptr[b]; m < s.surf_ptr[b + 1]; m++) {
+      const int j = s.surf_idx[m];
+      const double xj0 = s.x[0][j], xj1 = s.x[1][j];
+      if ((cb0 - xj0) * rec[0] + (cb1 - xj1) * rec[1] < rec[3]) continue;
+      const double dx = xj0 - xi0, dy = xj1 - xi1;
+      const double R = sqrt(dx * dx + dy * dy);
+      if (R < rc) w += (1. - R / rc) * s.vol[j];
+    }
+    if (w == 0.) continue;
+    const double q0 = r0 - rec[7], q1 = r1 - rec[8];
+    fi[0] += w * (rec[4] - rec[6] * q1);
+    fi[1] += w * (rec[5] + rec[6] * q0);
   }
 }
 
