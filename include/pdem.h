@@ -289,10 +289,13 @@ int pdem_mlp_train(pdem_mlp *m, const float *x, const float *t, int64_t n, int32
  *  stats:   pairs per mode since enable (0 no contact possible, 1 closure,
  *           2 plain), checked pairs, mean and max |F_closure - F_plain| and
  *           mean |F_plain|, in units of Kn Rc Vbar_a Vbar_b.
- *  log:     training rows of the current state: 16 features and 4 targets
- *           (F.n, F.t, M_a / h_a in those units, local max Z) for every pair
- *           with gap < Rc + band_h * min(h_a, h_b); *n gets the number of
- *           rows found (rows beyond cap are dropped).
+ *  log:     training rows of the current state: 16 features, 4 targets
+ *           (plain F.n, F.t, M_a / h_a in those units, local max Z) and 3
+ *           baseline values (the normal-contact part of the first three
+ *           computed from the contact patches; the network predicts target
+ *           minus baseline; baseline may be NULL) for every pair with
+ *           gap < Rc + band_h * min(h_a, h_b); *n gets the number of rows
+ *           found (rows beyond cap are dropped).
  * ------------------------------------------------------------------- */
 int pdem_closure_setup(pdem_ctx *ctx, const float *particle_code);
 int pdem_closure_enable(pdem_ctx *ctx, const pdem_mlp *net, double pen_max_h, double dz_max,
@@ -300,7 +303,8 @@ int pdem_closure_enable(pdem_ctx *ctx, const pdem_mlp *net, double pen_max_h, do
 int pdem_closure_disable(pdem_ctx *ctx);
 int pdem_closure_stats(pdem_ctx *ctx, int64_t *pairs_by_mode3, int64_t *checked, double *err_mean,
                        double *err_max, double *plain_mean);
-int pdem_pair_log(pdem_ctx *ctx, double band_h, int64_t cap, float *features, float *targets, int64_t *n);
+int pdem_pair_log(pdem_ctx *ctx, double band_h, int64_t cap, float *features, float *targets, float *baseline,
+                  int64_t *n);
 
 /* number of HIP devices visible to the engine (0 when none); no device work */
 int pdem_device_count(void);

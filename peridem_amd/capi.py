@@ -342,7 +342,7 @@ def engine_library():
                                            C.POINTER(C.c_double)]
         lib.pdem_pair_log.restype = C.c_int
         lib.pdem_pair_log.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.POINTER(C.c_float),
-                                      C.POINTER(C.c_float), C.POINTER(C.c_int64)]
+                                      C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int64)]
         lib.pdem_autotune.restype = C.c_int
         lib.pdem_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
         lib.pdem_get_launch_config.restype = C.c_int
@@ -417,14 +417,19 @@ class GpuEngine(Engine):
         return {"no_contact": modes[0], "closure": modes[1], "plain": modes[2], "checked": chk.value,
                 "check_err_mean": em.value, "check_err_max": ex.value, "check_plain_mean": pm.value}
 
-    def pair_log(self, band_h=0.5, cap=1 << 20):
-        """Training rows of the current state: (features [n][16], targets [n][4])."""
+    def pair_log(self, band_h=0.5, cap=1 << 20, baseline=False):
+        """Training rows of the current state: features [n][16], plain targets
+        [n][4] and, with baseline=True, the normal-contact baseline [n][3]."""
         f = np.empty((cap, 16), np.float32)
         t = np.empty((cap, 4), np.float32)
+        b = np.empty((cap, 3), np.float32)
         n = C.c_int64()
-        self._check(self.lib.pdem_pair_log(self.ctx, float(band_h), int(cap), f.ctypes.data_as(C.POINTER(C.c_float)),
-                                           t.ctypes.data_as(C.POINTER(C.c_float)), C.byref(n)), "pair_log")
+        P = C.POINTER(C.c_float)
+        self._check(self.lib.pdem_pair_log(self.ctx, float(band_h), int(cap), f.ctypes.data_as(P),
+                                           t.ctypes.data_as(P), b.ctypes.data_as(P), C.byref(n)), "pair_log")
         m = min(n.value, cap)
+        if baseline:
+            return f[:m].copy(), t[:m].copy(), b[:m].copy()
         return f[:m].copy(), t[:m].copy()
 
     def update_contact_neighborlist(self):

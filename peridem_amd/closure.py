@@ -35,7 +35,9 @@ def particle_codes(spec):
 
 
 def collect(spec, steps, log_every=10, band_h=0.5, precision=capi.PREC_MIXED, cap=1 << 20, verbose=False):
-    """Run `spec` with the plain method and log pair rows every `log_every` steps."""
+    """Run `spec` with the plain method and log pair rows every `log_every`
+    steps.  Returns features X and the network targets T: the plain pair
+    force and moment minus their normal-contact baseline, and the local Z."""
     eng = capi.GpuEngine(spec, precision)
     eng.closure_setup(particle_codes(spec))
     eng.integrate_begin()
@@ -43,7 +45,8 @@ def collect(spec, steps, log_every=10, band_h=0.5, precision=capi.PREC_MIXED, ca
     t0 = time.perf_counter()
     for k in range(steps // log_every):
         eng.integrate_cd_graph(log_every)
-        x, t = eng.pair_log(band_h, cap)
+        x, t, b = eng.pair_log(band_h, cap, baseline=True)
+        t[:, :3] -= b
         X.append(x)
         T.append(t)
         if verbose and (k + 1) % max(1, (steps // log_every) // 10) == 0:
@@ -57,15 +60,16 @@ def collect(spec, steps, log_every=10, band_h=0.5, precision=capi.PREC_MIXED, ca
 
 
 # feature columns (pdem_closure.hpp, pair_features)
-F_GAP, F_DMIN, F_OVL = 0, 14, 15
+F_GAP, F_HMIN, F_DMIN = 0, 1, 14
 
 
 def select(X, T, pen_h=TRAIN_PEN_H):
     """Rows the network would see: node pairs in contact (nearest patch pair
     closer than Rc, extent gap >= 0) with penetration Rc - dmin below pen_h
     mesh sizes of the finer particle."""
-    hmin_rc = np.minimum(X[:, 1], X[:, 2])
+    hmin_rc = X[:, F_HMIN]
     keep = (X[:, F_GAP] >= 0.0) & (X[:, F_DMIN] < 1.0) & (X[:, F_DMIN] >= 1.0 - pen_h * hmin_rc)
+    keep &= T[:, 3] < 1.0  # damaged pairs stay with the plain method
     return X[keep], T[keep]
 
 

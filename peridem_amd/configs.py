@@ -249,7 +249,7 @@ def single_particle_3d(R=1.0e-3, nodes=1000, state=True, stretch=0.0):
 
 def ensemble_2d(n_instances=512, seed=0, dt=5.0e-8, n_steps=5000, R_lo=0.8e-3, R_hi=1.2e-3,
                 n_min=2, n_max=64, ratios=(6, 8, 10), speed_lo=0.05, speed_hi=1.0,
-                angle_max_deg=60.0, kn_factor=1.0e-2, pitch_factor=1.06):
+                angle_max_deg=60.0, kn_factor=1.0e-2, pitch_factor=1.06, mu=0.5):
     """Config 3: a batch of independent 2D simulations in one model.
 
     Instance i (seed + i): particle count ~ U[n_min, n_max], mesh ratio
@@ -289,7 +289,7 @@ def ensemble_2d(n_instances=512, seed=0, dt=5.0e-8, n_steps=5000, R_lo=0.8e-3, R
             delta = 3.0 * h
             contact[(a, b)] = ContactPairDeck(contact_radius_factor=None, contact_radius=0.95 * h,
                                               Kn=_kn(mat, delta), Kn_factor=kn_factor, epsilon=0.7,
-                                              beta_n_factor=1.0, friction_coeff=0.5)
+                                              beta_n_factor=1.0, friction_coeff=mu)
     b = ModelBuilder(2, dt, zones, contact, search_factor=5.0, neigh_update_interval=40)
     # start every pair out of contact: gaps of at least 1.2 contact radii
     rc_max = max(c.contact_radius for c in contact.values())
@@ -311,7 +311,7 @@ def ensemble_2d(n_instances=512, seed=0, dt=5.0e-8, n_steps=5000, R_lo=0.8e-3, R
         first = len(b.parts)
         for q in range(n_p):
             rad = rng.uniform(R_lo, R_hi)
-            jit = (pitch / 2 - rad - 0.6 * rc_max) * 0.9
+            jit = max(0.0, (pitch / 2 - rad - 0.6 * rc_max) * 0.9)
             cx = ox + (q % ncol + 0.5) * pitch + rng.uniform(-jit, jit)
             cy = oy + (q // ncol + 0.5) * pitch + rng.uniform(-jit, jit)
             z = 2 * ri + (q % 2)

@@ -138,15 +138,19 @@ __device__ __forceinline__ double omega(const DS& s, const ClosureArgs& ca, int 
 // patch pair (a node outside a's patch is more than gap + Rc from every node
 // of b along n), so dmin >= Rc means no contact force (exact).
 // Overlap weight of a patch pair: c_ij = (1 - R_ij / Rc) V_i V_j for
-// R_ij < Rc.  ovl = sum c_ij / (Vbar_a Vbar_b); the first moments of the
-// weights about each centre (P.sa, P.sb) and second moments (P.saa, P.sbb)
-// give the weighted patch centroids and polar moments of the force transfer.
+// R_ij < Rc.  W = sum c_ij; Wn, Wt: the weights times the pair direction
+// e_ij (i to j) along n and t, Wm = sum c_ij (r_i x e_ij): the normal part of
+// the plain contact force on a is -Kn Rc (Wn n + Wt t), its moment about c_a
+// -Kn Rc Wm (the network predicts the remainder: friction and corrections);
+// the first moments of the weights about each centre (P.sa, P.sb) and second
+// moments (P.saa, P.sbb) give the weighted patch centroids and polar moments
+// of the force transfer.
 struct PatchSums {
-  double dmin, W, sa[2], saa, sb[2], sbb;
+  double dmin, W, Wn, Wt, Wm, sa[2], saa, sb[2], sbb;
 };
 
 __device__ __forceinline__ void patch_contact(const DS& s, const PairGeo& g, int sub, bool on, PatchSums& P) {
-  double dm = 1e300, w = 0., a0 = 0., a1 = 0., aa = 0., b0s = 0., b1s = 0., bb = 0.;
+  double dm = 1e300, w = 0., wn = 0., wt = 0., wm = 0., a0 = 0., a1 = 0., aa = 0., b0s = 0., b1s = 0., bb = 0.;
   if (on) {
     const double thr_a = g.ext_a - g.rc, thr_b = g.ext_b - g.rc;
     const int b0 = s.surf_ptr[g.b], b1 = s.surf_ptr[g.b + 1];
@@ -167,6 +171,10 @@ __device__ __forceinline__ void patch_contact(const DS& s, const PairGeo& g, int
         if (R < g.rc) {
           const double c = (1. - R / g.rc) * Vi * s.vol[j];
           w += c;
+          // direction of the node pair (i to j) in the pair frame
+          wn += c * (dx * g.n[0] + dy * g.n[1]) / R;
+          wt += c * (-dx * g.n[1] + dy * g.n[0]) / R;
+          wm += c * (ri0 * dy - ri1 * dx) / R;
           a0 += c * ri0;
           a1 += c * ri1;
           aa += c * (ri0 * ri0 + ri1 * ri1);
@@ -179,6 +187,9 @@ __device__ __forceinline__ void patch_contact(const DS& s, const PairGeo& g, int
   }
   P.dmin = -sub_max(-dm);
   P.W = sub_sum(w);
+  P.Wn = sub_sum(wn);
+  P.Wt = sub_sum(wt);
+  P.Wm = sub_sum(wm);
   P.sa[0] = sub_sum(a0);
   P.sa[1] = sub_sum(a1);
   P.saa = sub_sum(aa);
@@ -188,14 +199,15 @@ __device__ __forceinline__ void patch_contact(const DS& s, const PairGeo& g, int
 }
 
 __device__ __forceinline__ void pair_features(const DS& s, const ClosureArgs& ca, const PairGeo& g, double dmin,
-                                              double ovl, float* x16) {
+                                              const PatchSums& P, float* x16) {
+  const double vv = ca.vbar[g.a] * ca.vbar[g.b];
   const int nn = s.n_nonwall;
   const double t[2] = {-g.n[1], g.n[0]};
   const double dvx = s.pcv[g.b] - s.pcv[g.a], dvy = s.pcv[nn + g.b] - s.pcv[nn + g.a];
   const double Ra = s.prad[g.a], Rb = s.prad[g.b];
   x16[0] = (float)(g.gap / g.rc);
-  x16[1] = (float)(g.ha / g.rc);
-  x16[2] = (float)(g.hb / g.rc);
+  x16[1] = (float)(fmin(g.ha, g.hb) / g.rc);
+  x16[2] = (float)(P.Wt / vv);
   x16[3] = (float)(Ra / g.ha);
   x16[4] = (float)(Rb / g.hb);
   x16[5] = (float)(g.ext_a / Ra);
@@ -208,7 +220,17 @@ __device__ __forceinline__ void pair_features(const DS& s, const ClosureArgs& ca
   x16[12] = ca.code[g.b];
   x16[13] = (float)g.ct.mu;
   x16[14] = (float)(dmin / g.rc);
-  x16[15] = (float)ovl;
+  x16[15] = (float)(P.Wn / vv);
+}
+
+// normal-contact baseline of a pair in network units (F.n, F.t, M_a / h_a):
+// the network's targets are the plain values minus this baseline
+__device__ __forceinline__ void pair_baseline(const ClosureArgs& ca, const PairGeo& g, const PatchSums& P,
+                                              double* b3) {
+  const double vv = ca.vbar[g.a] * ca.vbar[g.b];
+  b3[0] = -P.Wn / vv;
+  b3[1] = -P.Wt / vv;
+  b3[2] = -P.Wm / (vv * g.ha);
 }
 
 // force unit of a pair: Kn Rc Vbar_a Vbar_b
@@ -270,16 +292,20 @@ __device__ __forceinline__ void plain_pair(const DS& s, const PairGeo& g, int su
 // about the centre, M (the torque term has no net force).
 __device__ __forceinline__ void closure_record(double* rec, const double* dir, double thr_own, double thr_partner,
                                                const double* F, double M, double W, const double* s1, double s2,
-                                               double rc) {
+                                               double rc, double h) {
   double g[2] = {0., 0.}, xb[2] = {0., 0.}, k = 0.;
   if (W > 0.) {
     g[0] = F[0] / W;
     g[1] = F[1] / W;
     xb[0] = s1[0] / W;
     xb[1] = s1[1] / W;
+    // torque term only for patches wider than h / 4 (weighted spread), with
+    // the extra moment limited to |F| h: a narrow patch cannot carry a couple
     const double I = s2 - W * (xb[0] * xb[0] + xb[1] * xb[1]);
-    const double dM = M - (xb[0] * F[1] - xb[1] * F[0]);
-    k = I > 1e-12 * s2 ? dM / I : 0.;
+    double dM = M - (xb[0] * F[1] - xb[1] * F[0]);
+    const double lim = sqrt(F[0] * F[0] + F[1] * F[1]) * h;
+    dM = fmax(-lim, fmin(lim, dM));
+    k = I > 0.0625 * W * h * h ? dM / I : 0.;
   }
   rec[0] = dir[0];
   rec[1] = dir[1];
@@ -322,7 +348,6 @@ __global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
     patch_contact(s, g, sub, near && g.gap >= 0., P);
     if (near && g.gap >= 0. && P.dmin >= g.rc) near = false;  // no node pair within Rc
     const double dmin = (near && g.gap >= 0.) ? P.dmin : g.gap;
-    const double ovl = P.W / (ca.vbar[g.a] * ca.vbar[g.b]);
     const double za = particle_zmax(s, g.pa, sub, near);
     const double zb = particle_zmax(s, g.pb, sub, near);
     int mode = 0;
@@ -334,16 +359,18 @@ __global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
     const double unit = pair_unit(ca, g);
     if (ca.use_net) {
       float x16[CL_NFEAT];
-      pair_features(s, ca, g, dmin, ovl, x16);
+      pair_features(s, ca, g, dmin, P, x16);
       mlp::s4 h1[8], h2[8], h3[8];
       const mlp::f4 y = mlp::forward(ca.net, mlp::input_frag(ca.net, x16, lane), h1, h2, h3, lane);
       double ys[4];
       for (int r = 0; r < 4; r++) ys[r] = (double)(__shfl(y[r], pl) / ca.net.out_scale[r]);
       if (mode == 1 && ys[3] > (double)ca.dz_max) mode = 2;
-      const double Fn = ys[0] * unit, Ft = ys[1] * unit;
+      double b3[3];
+      pair_baseline(ca, g, P, b3);
+      const double Fn = (ys[0] + b3[0]) * unit, Ft = (ys[1] + b3[1]) * unit;
       Fa[0] = Fn * g.n[0] - Ft * g.n[1];
       Fa[1] = Fn * g.n[1] + Ft * g.n[0];
-      Ma = ys[2] * unit * g.ha;
+      Ma = (ys[2] + b3[2]) * unit * g.ha;
     }
     const double nb[2] = {-g.n[0], -g.n[1]};
     const int sb = valid ? find_slot(s, g.b, g.a) : -1;
@@ -360,9 +387,9 @@ __global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
         // torque balance: M_b = -M_a + (c_b - c_a) x F_a
         const double Mb = -Ma + ((g.cb[0] - g.ca[0]) * Fa[1] - (g.cb[1] - g.ca[1]) * Fa[0]);
         const double ta = g.ext_a - g.rc, tb = g.ext_b - g.rc;
-        closure_record(s.cout + ia * CL_REC, g.n, ta, tb, Fa, Ma, P.W, P.sa, P.saa, g.rc);
+        closure_record(s.cout + ia * CL_REC, g.n, ta, tb, Fa, Ma, P.W, P.sa, P.saa, g.rc, g.ha);
         closure_record(s.cout + ((int64_t)g.b * s.dcap + sb) * CL_REC, nb, tb, ta, Fb, Mb, P.W, P.sb, P.sbb,
-                       g.rc);
+                       g.rc, g.hb);
       }
       if (sb < 0) s.ctrl->overflow = 6;
       atomicAdd(&ca.stats[mode], 1ull);
@@ -383,9 +410,11 @@ __global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
 }
 
 // Training rows: every listed pair with gap < Rc + band_h * min(h_a, h_b):
-// features and the plain targets (F.n, F.t, M_a / h_a in force units, local Z).
+// features, the plain targets (F.n, F.t, M_a / h_a in force units, local Z)
+// and the normal-contact baseline of the patches in the same units (zero when
+// no patch pair is within Rc); the network learns targets minus baseline.
 __global__ void __launch_bounds__(256) k_pair_log(DS s, ClosureArgs ca, double band_h, float* feat, float* targ,
-                                                  unsigned* count, int cap) {
+                                                  float* basel, unsigned* count, int cap) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, sub = lane >> 4, pl = lane & 15;
   const int npairs = min((int)s.ctrl->npairs, s.cpair_cap);
   for (int base = (blockIdx.x * 4 + wave) * 16; base < npairs; base += gridDim.x * 64) {
@@ -397,19 +426,22 @@ __global__ void __launch_bounds__(256) k_pair_log(DS s, ClosureArgs ca, double b
     PatchSums P;
     patch_contact(s, g, sub, take && g.gap >= 0., P);
     const double dmin = (take && g.gap >= 0.) ? P.dmin : g.gap;
-    const double ovl = P.W / (ca.vbar[g.a] * ca.vbar[g.b]);
     double Fp[2], Mp, zl;
     plain_pair(s, g, sub, take, Fp, Mp, zl);
     if (take && sub == 0) {
       const unsigned row = atomicAdd(count, 1u);
       if ((int)row < cap) {
         float x16[CL_NFEAT];
-        pair_features(s, ca, g, dmin, ovl, x16);
+        pair_features(s, ca, g, dmin, P, x16);
         for (int f = 0; f < CL_NFEAT; f++) feat[(int64_t)row * CL_NFEAT + f] = x16[f];
         const double unit = pair_unit(ca, g);
+        double b3[3];
+        pair_baseline(ca, g, P, b3);
         targ[(int64_t)row * 4 + 0] = (float)((Fp[0] * g.n[0] + Fp[1] * g.n[1]) / unit);
         targ[(int64_t)row * 4 + 1] = (float)((-Fp[0] * g.n[1] + Fp[1] * g.n[0]) / unit);
         targ[(int64_t)row * 4 + 2] = (float)(Mp / (unit * g.ha));
+        if (basel)
+          for (int r = 0; r < 3; r++) basel[(int64_t)row * 3 + r] = (float)b3[r];
         targ[(int64_t)row * 4 + 3] = (float)zl;
       }
     }
@@ -419,8 +451,8 @@ __global__ void __launch_bounds__(256) k_pair_log(DS s, ClosureArgs ca, double b
 constexpr int CL_BLOCKS = 256;  // closure kernel grid (grid-stride over pair chunks)
 
 // Closure lists from the current candidate and partner lists (set-up and
-// logging; afterwards the rebuilds maintain them): wall_near per particle and
-// the pair list.
+// logging; afterwards the rebuilds maintain them): wall_near per particle
+// (stamped with the rebuild count) and the pair list.
 __global__ void k_cl_wall_near(DS s) {
   const int64_t i = gid();
   if (i >= s.N) return;
@@ -429,7 +461,7 @@ __global__ void k_cl_wall_near(DS s) {
   const int cnt = s.ccount[i];
   for (int k = 0; k < cnt; k++)
     if (s.pmeta[s.ptid[s.cand[(int64_t)k * s.N + i]]] >> 16) {
-      atomicOr(&s.wall_near[P.nonwall_index], 1);
+      s.wall_near[P.nonwall_index] = (int)s.ctrl->n_rebuilds;
       return;
     }
 }

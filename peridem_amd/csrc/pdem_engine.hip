@@ -190,7 +190,7 @@ struct Launch {
   static void contact_search(pdem_ctx* c) {
     hipStream_t st = c->stream;
     const unsigned nb = blocks(c->N);
-    hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, st, c->ds);
+    hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, st, c->ds);
     hipLaunchKernelGGL(k_bin<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
     hipLaunchKernelGGL(k_query<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
     damp_pairs(c);
@@ -205,10 +205,12 @@ struct Launch {
     if (c->ds.n_nonwall > 0)
       hipLaunchKernelGGL(k_damping<DIM>, dim3(c->ds.n_nonwall), dim3(64), 0, c->stream, c->ds);
   }
+  // rebuild steps: damping partner lists, then the cell table is emptied
   static void damp_pairs(pdem_ctx* c) {
-    if (c->ds.n_nonwall > 0)
-      hipLaunchKernelGGL(k_damp_pairs<DIM>, dim3(c->ds.n_nonwall), dim3(64), 0, c->stream, c->ds);
+    const unsigned nb = (unsigned)std::min<int64_t>(std::max(c->ds.n_nonwall, 1), 2048);
+    hipLaunchKernelGGL(k_damp_pairs<DIM>, dim3(nb), dim3(64), 0, c->stream, c->ds);
   }
+  static unsigned query_blocks(pdem_ctx* c) { return (unsigned)std::min<int64_t>(blocks(c->N), 2048); }
   // closure pairs: modes, network, patch records (2D, closure mode only)
   static void closure(pdem_ctx* c) {
     if (DIM == 2 && c->ds.cl_on)
@@ -245,7 +247,7 @@ struct Launch {
   static void forces_after_ctrl(pdem_ctx* c) {
     pd_forces(c, c->multi ? 1 : 0);
     if (c->multi) {
-      hipLaunchKernelGGL(k_query<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds);
+      hipLaunchKernelGGL(k_query<DIM>, dim3(query_blocks(c)), dim3(TPB), 0, c->stream, c->ds);
       damp_pairs(c);
       closure(c);
       contact_force_kernel(c);
@@ -254,13 +256,13 @@ struct Launch {
     external_forces(c, c->multi != 0);
   }
   static void compute_forces(pdem_ctx* c) {
-    if (c->multi) hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, c->stream, c->ds);
+    if (c->multi) hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, c->stream, c->ds);
     forces_after_ctrl(c);
   }
   static void integrate_step(pdem_ctx* c) {
     hipLaunchKernelGGL(k_integrate<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds, c->multi);
     displacement_bc(c);
-    if (c->multi) hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, c->stream, c->ds);
+    if (c->multi) hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, c->stream, c->ds);
     forces_after_ctrl(c);
   }
 };
@@ -602,6 +604,8 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     }
     s.cell_count = c->dalloc<int>(ncells + 1);
     s.cell_owner = c->dalloc<int>(ncells + 1);
+    hipLaunchKernelGGL(k_cells_init, dim3((unsigned)std::min<int64_t>(blocks(ncells), 4096)), dim3(TPB), 0,
+                       c->stream, s);
     s.cell_nodes = c->dalloc<int>((size_t)ncells * s.cell_cap, false);
     // candidate capacity: nodes of a ball of the largest search radius at the
     // finest mesh spacing
@@ -1247,18 +1251,19 @@ int pdem_closure_stats(pdem_ctx* c, int64_t* pairs_by_mode, int64_t* checked, do
   });
 }
 
-int pdem_pair_log(pdem_ctx* c, double band_h, int64_t cap, float* feat, float* targ, int64_t* n) {
+int pdem_pair_log(pdem_ctx* c, double band_h, int64_t cap, float* feat, float* targ, float* base, int64_t* n) {
   return guarded(c, [&] {
     if (!c->cl_ready) throw HipError{"call pdem_closure_setup before pdem_pair_log"};
     if (cap < 0) throw HipError{"negative capacity"};
     closure_lists(c);
     float* df = c->dalloc<float>((size_t)std::max<int64_t>(cap, 1) * CL_NFEAT);
     float* dt = c->dalloc<float>((size_t)std::max<int64_t>(cap, 1) * 4);
+    float* db = c->dalloc<float>((size_t)std::max<int64_t>(cap, 1) * 3);
     unsigned* dn = c->dalloc<unsigned>(1, false);
     HIPCHK(hipMemsetAsync(dn, 0, sizeof(unsigned), c->stream));
     DS t = c->ds;
     t.cl_on = 1;
-    hipLaunchKernelGGL(k_pair_log, dim3(CL_BLOCKS), dim3(256), 0, c->stream, t, c->ca, band_h, df, dt, dn,
+    hipLaunchKernelGGL(k_pair_log, dim3(CL_BLOCKS), dim3(256), 0, c->stream, t, c->ca, band_h, df, dt, db, dn,
                        (int)std::min<int64_t>(cap, INT_MAX));
     check_launch(c);
     unsigned cnt = 0;
@@ -1268,10 +1273,12 @@ int pdem_pair_log(pdem_ctx* c, double band_h, int64_t cap, float* feat, float* t
     if (m > 0) {
       HIPCHK(hipMemcpy(feat, df, m * CL_NFEAT * sizeof(float), hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(targ, dt, m * 4 * sizeof(float), hipMemcpyDeviceToHost));
+      if (base) HIPCHK(hipMemcpy(base, db, m * 3 * sizeof(float), hipMemcpyDeviceToHost));
     }
     *n = (int64_t)cnt;
     c->dfree(df);
     c->dfree(dt);
+    c->dfree(db);
     c->dfree(dn);
   });
 }

@@ -139,7 +139,7 @@ struct DS {
   int8_t* cmode;        // [n_nonwall][dcap] 0: no contact possible, 1: closure, 2: plain
   double* cout;         // [n_nonwall][dcap][CL_REC] closure force records
   int* plain_any;       // [n_nonwall] some pair of the particle is plain this step
-  int* wall_near;       // [n_nonwall] a node has a wall candidate (set at rebuilds)
+  int* wall_near;       // [n_nonwall] rebuild count at which a node had a wall candidate
   const uint8_t* surf;  // [N] surface node flag
   const int* surf_ptr;  // [n_nonwall + 1] surface nodes per non-wall particle
   const int* surf_idx;
@@ -351,53 +351,53 @@ __device__ void ctrl_decide(Ctrl* cg) {
   *cg = h;
 }
 
-// One workgroup: reduce the node-update block maxima of |v|, run the
-// controller, pack the damping particles' centre data, and clear the cell
-// table when this step rebuilds the candidate lists.
+// Controller step.  Workgroup 0 reduces the node-update block maxima of |v|
+// and runs the controller; all workgroups pack the non-wall particles' centre
+// data (and clear the closure's per-step flags).  The cell table is cleared
+// after each rebuild's query (k_damp_pairs), so no step clears it here.
+constexpr int CTRL_BLOCKS = 64;
+
 __global__ void __launch_bounds__(1024) k_ctrl(DS s) {
   __shared__ unsigned long long red[1024 / WAVE];
-  __shared__ int reb;
   Ctrl* c = s.ctrl;
   const int t = threadIdx.x;
-  if (c->vdirty) {
-    unsigned long long m = 0;
-    for (int k = t; k < s.nblockmax; k += blockDim.x) m = s.blockmax[k] > m ? s.blockmax[k] : m;
-    for (int off = WAVE / 2; off > 0; off >>= 1) {
-      unsigned long long o = __shfl_xor(m, off);
-      m = o > m ? o : m;
+  if (blockIdx.x == 0) {
+    if (c->vdirty) {
+      unsigned long long m = 0;
+      for (int k = t; k < s.nblockmax; k += blockDim.x) m = s.blockmax[k] > m ? s.blockmax[k] : m;
+      for (int off = WAVE / 2; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(m, off);
+        m = o > m ? o : m;
+      }
+      if ((t & (WAVE - 1)) == 0) red[t / WAVE] = m;
+      __syncthreads();
+      if (t == 0) {
+        unsigned long long mm = 0;
+        for (int k = 0; k < (int)(blockDim.x / WAVE); k++) mm = red[k] > mm ? red[k] : mm;
+        c->vmax_acc = mm;
+      }
     }
-    if ((t & (WAVE - 1)) == 0) red[t / WAVE] = m;
-    __syncthreads();
-    if (t == 0) {
-      unsigned long long mm = 0;
-      for (int k = 0; k < (int)(blockDim.x / WAVE); k++) mm = red[k] > mm ? red[k] : mm;
-      c->vmax_acc = mm;
-    }
-  }
-  if (t == 0) {
-    ctrl_decide(c);
-    reb = c->rebuild;
+    if (t == 0) ctrl_decide(c);
   }
   const int nn = s.n_nonwall;
-  for (int q = t; q < nn; q += blockDim.x) {
+  for (int q = blockIdx.x * blockDim.x + t; q < nn; q += gridDim.x * blockDim.x) {
     const int64_t cn = s.P[s.nonwall_pid[q]].center;
     for (int d = 0; d < 3; d++) {
       s.pcx[d * nn + q] = s.x[d] ? s.x[d][cn] : 0.;
       s.pcv[d * nn + q] = s.v[d] ? s.v[d][cn] : 0.;
     }
-  }
-  if (s.cl_on)
-    for (int q = t; q < nn; q += blockDim.x) s.plain_any[q] = 0;
-  __syncthreads();
-  if (reb) {
-    for (int64_t k = t; k < s.ncells; k += blockDim.x) {
-      s.cell_count[k] = 0;
-      s.cell_owner[k] = -2;
-    }
-    if (s.cl_on)
-      for (int q = t; q < nn; q += blockDim.x) s.wall_near[q] = 0;
+    if (s.cl_on) s.plain_any[q] = 0;
   }
 }
+
+// empty the cell table (all cells, also at set-up)
+__device__ __forceinline__ void clear_cells(const DS& s, int64_t first, int64_t stride) {
+  for (int64_t k = first; k < s.ncells; k += stride) {
+    s.cell_count[k] = 0;
+    s.cell_owner[k] = -2;
+  }
+}
+__global__ void k_cells_init(DS s) { clear_cells(s, gid(), (int64_t)gridDim.x * blockDim.x); }
 
 // ------------------------------------------------------------ node update
 // integrateCD (demModel.cpp:349-390): semi-implicit Euler on free dofs.
@@ -895,7 +895,7 @@ __device__ __forceinline__ void contact_pair(const DS& s, const double* vi, int 
 // only if its particle has a plain pair or a wall candidate, and a candidate
 // in another non-wall particle only if that pair is plain this step.
 __device__ __forceinline__ bool node_needs_contact(const DS& s, int nwi) {
-  return !s.cl_on || nwi < 0 || s.plain_any[nwi] != 0 || s.wall_near[nwi] != 0;
+  return !s.cl_on || nwi < 0 || s.plain_any[nwi] != 0 || s.wall_near[nwi] == (int)s.ctrl->n_rebuilds;
 }
 __device__ __forceinline__ bool pair_plain(const DS& s, int a, int b) {
   const int* dl = s.dlist + (int64_t)a * s.dcap;
@@ -918,24 +918,24 @@ __device__ __forceinline__ bool pair_plain(const DS& s, int a, int b) {
 template <int DIM>
 __global__ void __launch_bounds__(256) k_query(DS s) {
   if (!s.ctrl->rebuild) return;
-  int64_t i = gid();
-  if (i >= s.N) return;
-  const int pid = s.ptid[i];
-  const DevParticle& pi = s.P[pid];
-  if (!pi.searches) return;
-  query_node<DIM>(s, i, pid);
-  const int cnt = s.ccount[i];
-  if (s.cl_on && pi.nonwall_index >= 0)
-    for (int k = 0; k < cnt; k++)
-      if (s.pmeta[s.ptid[s.cand[(int64_t)k * s.N + i]]] >> 16) {
-        atomicOr(&s.wall_near[pi.nonwall_index], 1);
-        break;
-      }
-  if (cnt > 0 && pi.compute_force) {
-    if (cnt <= s.light_max)
-      s.light[atomicAdd(&s.ctrl->nlight, 1u)] = (int)i;
-    else
-      s.active[atomicAdd(&s.ctrl->nactive, 1u)] = (int)i;
+  for (int64_t i = gid(); i < s.N; i += (int64_t)gridDim.x * blockDim.x) {
+    const int pid = s.ptid[i];
+    const DevParticle& pi = s.P[pid];
+    if (!pi.searches) continue;
+    query_node<DIM>(s, i, pid);
+    const int cnt = s.ccount[i];
+    if (s.cl_on && pi.nonwall_index >= 0)
+      for (int k = 0; k < cnt; k++)
+        if (s.pmeta[s.ptid[s.cand[(int64_t)k * s.N + i]]] >> 16) {
+          s.wall_near[pi.nonwall_index] = (int)s.ctrl->n_rebuilds;
+          break;
+        }
+    if (cnt > 0 && pi.compute_force) {
+      if (cnt <= s.light_max)
+        s.light[atomicAdd(&s.ctrl->nlight, 1u)] = (int)i;
+      else
+        s.active[atomicAdd(&s.ctrl->nactive, 1u)] = (int)i;
+    }
   }
 }
 
@@ -1128,11 +1128,8 @@ __global__ void __launch_bounds__(CWAVES * WAVE) k_contact_force(DS s) {
 // Damping partners on a rebuild: one wavefront per non-wall particle, lanes
 // over the other particles, ballot-ordered append (ascending index).
 template <int DIM>
-__global__ void __launch_bounds__(64) k_damp_pairs(DS s) {
-  if (!s.ctrl->rebuild) return;
-  const int q = blockIdx.x;
+__device__ void damp_pairs_one(const DS& s, int q) {
   const int nn = s.n_nonwall;
-  if (q >= nn) return;
   const int lane = threadIdx.x;
   const double Ri = s.prad[q];
   const double margin = 2.0 * s.ctrl->radius;
@@ -1177,6 +1174,15 @@ __global__ void __launch_bounds__(64) k_damp_pairs(DS s) {
     }
     s.dcount[q] = cnt;
   }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(64) k_damp_pairs(DS s) {
+  if (!s.ctrl->rebuild) return;
+  // the query of this rebuild is done: leave the cell table empty for the next
+  clear_cells(s, gid(), (int64_t)gridDim.x * blockDim.x);
+  const int nn = s.n_nonwall;
+  for (int q = blockIdx.x; q < nn; q += gridDim.x) damp_pairs_one<DIM>(s, q);
 }
 
 // Damping (demModel.cpp:937-1052): one wavefront per non-wall particle.
