@@ -111,68 +111,75 @@ def cpu_baseline(spec, nthreads, budget_s=12.0, max_steps=-1):
                       f"({t1 - t0:.1f} s) and 2 warm-up steps; FP64 oracle, {nthreads} threads"}
 
 
-def closure_comparison(spec, steps, sample_every, train_instances, train_steps, verbose=True):
-    """Learned closure vs plain method on `spec` (FP64): train the network on a
-    batched ensemble (plain method, mixed precision), then run `steps` steps
-    of both forms; returns step times, the reaction-force-curve error and the
-    closure statistics."""
+def _closure_run(spec, precision, net, steps, sample_every, reaction):
+    """Plain (net None) or closure run of `steps` steps; returns ms/step, the
+    sampled observable curve and the closure statistics."""
+    import numpy as np
+    from peridem_amd import capi, closure
+
+    e = capi.GpuEngine(spec, precision)
+    e.closure_setup(closure.particle_codes(spec))
+    if net is not None:
+        e.closure_enable(net, pen_max_h=0.05, dz_max=0.5, check_every=100)
+    e.integrate_begin()
+    e.integrate_cd_graph(10)
+    e.synchronize()
+    curve, tt, done = [], 0.0, 0
+    mass = spec.vol * np.array([p["density"] for p in spec.particles])[spec.pt_id]
+    while done < steps:
+        k = min(sample_every, steps - done)
+        ta = time.perf_counter()
+        e.integrate_cd_graph(k)
+        e.synchronize()
+        tt += time.perf_counter() - ta
+        done += k
+        if reaction:
+            curve.append(e.pp_compressive_test(spec.wall_id, spec.f_dir)[1])
+        else:  # kinetic energy of the model
+            v = e.field(capi.FIELD_V)
+            curve.append(float(0.5 * (v ** 2).sum(1) @ mass))
+    st = e.closure_stats() if net is not None else None
+    e.close()
+    return tt / steps * 1e3, np.array(curve), st
+
+
+def closure_comparison(c1_spec, steps, sample_every, train_instances, train_steps, eval_steps, verbose=True):
+    """Learned closure vs plain method: train the network on a batched ensemble
+    (plain method, mixed precision), then time both forms on the 2D
+    compressive test (FP64; reaction-force curve compared) and on a fresh
+    ensemble (mixed; kinetic-energy curve compared)."""
     import numpy as np
     from peridem_amd import capi, closure, configs
 
     t0 = time.perf_counter()
     train = configs.ensemble_2d(n_instances=train_instances, seed=1000, pitch_factor=1.0)
-    X, T = closure.collect(train, steps=train_steps, log_every=10, band_h=0.5, cap=1 << 19)
+    X, T = closure.collect(train, steps=train_steps, log_every=10, band_h=0.5, cap=1 << 18)
     Xs, Ts = closure.select(X, T)
     t1 = time.perf_counter()
     net, curve = closure.fit(Xs, Ts, iters=4000, batch=1024, lr=2e-3, seed=0)
     t2 = time.perf_counter()
     if verbose:
-        print(f"# closure: {len(X)} logged rows, {len(Xs)} near the band ({t1 - t0:.1f} s); "
+        print(f"# closure: {len(X)} logged rows, {len(Xs)} in the band ({t1 - t0:.1f} s); "
               f"training {t2 - t1:.1f} s, loss {curve[:10].mean():.4f} -> {curve[-50:].mean():.5f}",
               file=sys.stderr, flush=True)
-    out = {}
-    curves = {}
-    for tag in ("plain", "closure"):
-        e = capi.GpuEngine(spec, capi.PREC_FP64)
-        e.closure_setup(closure.particle_codes(spec))
-        if tag == "closure":
-            e.closure_enable(net, pen_max_h=0.05, dz_max=0.5, check_every=100)
-        e.integrate_begin()
-        e.integrate_cd_graph(10)
-        e.synchronize()
-        rc, tt = [], 0.0
-        done = 0
-        while done < steps:
-            k = min(sample_every, steps - done)
-            ta = time.perf_counter()
-            e.integrate_cd_graph(k)
-            e.synchronize()
-            tt += time.perf_counter() - ta
-            done += k
-            pen, reac = e.pp_compressive_test(spec.wall_id, spec.f_dir)
-            cn = [p["center_node"] for p in spec.particles if p["type_index"] == 0]
-            v = e.field(capi.FIELD_V)[cn, :2]
-            rc.append((done, reac, float(np.sqrt((v ** 2).sum(1).mean()))))
-        out[tag] = {"ms_per_step": round(tt / steps * 1e3, 5)}
-        if tag == "closure":
-            out[tag]["stats"] = e.closure_stats()
-        curves[tag] = np.array(rc)
-        e.close()
-    rp, rcl = curves["plain"], curves["closure"]
-    reac_err = float(np.abs(rcl[:, 1] - rp[:, 1]).max() / max(np.abs(rp[:, 1]).max(), 1e-300))
-    vrms_err = float(np.abs(rcl[:, 2] - rp[:, 2]).max() / max(np.abs(rp[:, 2]).max(), 1e-300))
-    return {
-        "workload": spec.name, "steps": steps,
-        "speedup": round(out["plain"]["ms_per_step"] / out["closure"]["ms_per_step"], 4),
-        "ms_per_step_plain": out["plain"]["ms_per_step"],
-        "ms_per_step_closure": out["closure"]["ms_per_step"],
-        "reaction_curve_err": reac_err, "centre_speed_rms_curve_err": vrms_err,
-        "pairs": out["closure"]["stats"],
-        "train": {"ensemble_instances": train_instances, "ensemble_steps": train_steps,
-                  "rows_logged": int(len(X)), "rows_used": int(len(Xs)),
-                  "held_out_rel_rms": [round(float(r), 4) for r in closure.fit_report(net, Xs[::7], Ts[::7])],
-                  "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2)},
-    }
+    out = {"train": {"ensemble_instances": train_instances, "ensemble_steps": train_steps,
+                     "rows_logged": int(len(X)), "rows_used": int(len(Xs)),
+                     "held_out_rel_rms": [round(float(r), 4) for r in closure.fit_report(net, Xs[::7], Ts[::7])],
+                     "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2)}}
+    cases = (("compressive_2d", c1_spec, capi.PREC_FP64, steps, sample_every, True),
+             ("ensemble_2d", configs.ensemble_2d(n_instances=train_instances, seed=2000, pitch_factor=1.0),
+              capi.PREC_MIXED, eval_steps, max(1, eval_steps // 20), False))
+    for name, spec, prec, n, every, reaction in cases:
+        mp, cp, _ = _closure_run(spec, prec, None, n, every, reaction)
+        mc, cc, st = _closure_run(spec, prec, net, n, every, reaction)
+        err = float(np.abs(cc - cp).max() / max(np.abs(cp).max(), 1e-300))
+        out[name] = {"workload": spec.name, "steps": n, "speedup": round(mp / mc, 4),
+                     "ms_per_step_plain": round(mp, 5), "ms_per_step_closure": round(mc, 5),
+                     ("reaction_curve_err" if reaction else "kinetic_energy_curve_err"): err,
+                     "pairs": st}
+        if verbose:
+            print(f"# closure {name}: {out[name]}", file=sys.stderr, flush=True)
+    return out
 
 
 def main():
@@ -192,6 +199,7 @@ def main():
     ap.add_argument("--closure-steps", type=int, default=50000)
     ap.add_argument("--closure-train-instances", type=int, default=512)
     ap.add_argument("--closure-train-steps", type=int, default=5000)
+    ap.add_argument("--closure-eval-steps", type=int, default=5000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU (particle-sharded) "
@@ -259,8 +267,7 @@ def main():
     closure_res = None
     if not args.no_closure and args.workload == "compressive_2d":
         closure_res = closure_comparison(spec, args.closure_steps, 500, args.closure_train_instances,
-                                         args.closure_train_steps)
-        print(f"# closure: {closure_res}", file=sys.stderr, flush=True)
+                                         args.closure_train_steps, args.closure_eval_steps)
 
     cpu = None
     if not args.no_cpu:

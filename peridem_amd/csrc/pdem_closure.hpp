@@ -61,6 +61,32 @@ __device__ __forceinline__ double sub_sum(double v) {
   return v + __shfl_xor(v, 32);
 }
 
+// Per non-wall particle: largest node distance from the centre node and
+// largest Z, one wavefront per particle (coalesced node loops).
+__global__ void __launch_bounds__(256) k_particle_radius(DS s) {
+  const int lane = threadIdx.x & 63;
+  const int nwaves = gridDim.x * 4;
+  for (int q = blockIdx.x * 4 + (threadIdx.x >> 6); q < s.n_nonwall; q += nwaves) {
+    const DevParticle& P = s.P[s.nonwall_pid[q]];
+    const double c0 = s.x[0][P.center], c1 = s.x[1][P.center];
+    double r2 = 0.;
+    float z = 0.f;
+    for (int64_t i = P.glob_start + lane; i < P.glob_end; i += 64) {
+      const double d0 = s.x[0][i] - c0, d1 = s.x[1][i] - c1;
+      r2 = fmax(r2, d0 * d0 + d1 * d1);
+      z = fmaxf(z, s.Z[i]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      r2 = fmax(r2, __shfl_xor(r2, off));
+      z = fmaxf(z, __shfl_xor(z, off));
+    }
+    if (lane == 0) {
+      s.prmax[q] = sqrt(r2);
+      s.pzmax[q] = z;
+    }
+  }
+}
+
 struct PairGeo {
   int a, b, pa, pb, sa;
   double ca[2], cb[2], n[2], d;
@@ -73,6 +99,7 @@ __device__ __forceinline__ double extent(const DS& s, int q, const double* c, co
                                          bool on) {
   double m = -1e300;
   if (on)
+#pragma unroll 4
     for (int k = s.surf_ptr[q] + sub; k < s.surf_ptr[q + 1]; k += 4) {
       const int i = s.surf_idx[k];
       const double e = (s.x[0][i] - c[0]) * dir[0] + (s.x[1][i] - c[1]) * dir[1];
@@ -102,26 +129,16 @@ __device__ __forceinline__ void pair_geometry(const DS& s, int k, bool valid, in
   g.rc = g.ct.contact_r;
   g.ha = s.P[g.pa].h;
   g.hb = s.P[g.pb].h;
-  // nodes lie within the bounding radius of the geometric centre, which is
-  // within h of the centre node; 1.5 h per particle leaves room for strain
-  const double reach = s.prad[g.a] + s.prad[g.b] + 1.5 * (g.ha + g.hb);
-  const bool far = ok && g.d - reach >= g.rc;
+  // every node lies within prmax of its centre node (this step's positions):
+  // d - prmax_a - prmax_b >= Rc means no node pair within Rc
+  const bool far = ok && g.d - s.prmax[g.a] - s.prmax[g.b] >= g.rc;
   g.ext_a = extent(s, g.a, g.ca, g.n, sub, ok && !far);
   g.ext_b = extent(s, g.b, g.cb, nb, sub, ok && !far);
   if (far) {
-    g.ext_a = s.prad[g.a] + 1.5 * g.ha;
-    g.ext_b = s.prad[g.b] + 1.5 * g.hb;
+    g.ext_a = s.prmax[g.a];
+    g.ext_b = s.prmax[g.b];
   }
   g.gap = ok ? g.d - g.ext_a - g.ext_b : -1e300;  // coincident centres: plain
-}
-
-__device__ __forceinline__ double particle_zmax(const DS& s, int pid, int sub, bool on) {
-  double z = 0.;
-  if (on) {
-    const DevParticle& P = s.P[pid];
-    for (int64_t i = P.glob_start + sub; i < P.glob_end; i += 4) z = fmax(z, (double)s.Z[i]);
-  }
-  return sub_max(z);
 }
 
 __device__ __forceinline__ double omega(const DS& s, const ClosureArgs& ca, int q, const double* c) {
@@ -160,6 +177,7 @@ __device__ __forceinline__ void patch_contact(const DS& s, const PairGeo& g, int
       const double ri0 = xi0 - g.ca[0], ri1 = xi1 - g.ca[1];
       if (ri0 * g.n[0] + ri1 * g.n[1] < thr_a) continue;
       const double Vi = s.vol[i];
+#pragma unroll 4
       for (int m = b0; m < b1; m++) {
         const int j = s.surf_idx[m];
         const double xj0 = s.x[0][j], xj1 = s.x[1][j];
@@ -348,8 +366,7 @@ __global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
     patch_contact(s, g, sub, near && g.gap >= 0., P);
     if (near && g.gap >= 0. && P.dmin >= g.rc) near = false;  // no node pair within Rc
     const double dmin = (near && g.gap >= 0.) ? P.dmin : g.gap;
-    const double za = particle_zmax(s, g.pa, sub, near);
-    const double zb = particle_zmax(s, g.pb, sub, near);
+    const double za = s.pzmax[g.a], zb = s.pzmax[g.b];
     int mode = 0;
     if (near)
       mode = (ca.use_net && g.gap >= 0. && g.rc - dmin <= ca.pen_max_h * fmin(g.ha, g.hb) && za < 1. && zb < 1.)
@@ -357,7 +374,7 @@ __global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
                  : 2;
     double Fa[2] = {0., 0.}, Ma = 0.;
     const double unit = pair_unit(ca, g);
-    if (ca.use_net) {
+    if (ca.use_net && __ballot(mode == 1)) {
       float x16[CL_NFEAT];
       pair_features(s, ca, g, dmin, P, x16);
       mlp::s4 h1[8], h2[8], h3[8];

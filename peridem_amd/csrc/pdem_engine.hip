@@ -196,7 +196,10 @@ struct Launch {
     damp_pairs(c);
   }
   static void contact_force_kernel(pdem_ctx* c) {
-    hipLaunchKernelGGL(k_contact_light<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds);
+    // lane-per-node list (grid-stride); empty by construction when light_max == 0
+    if (c->ds.light_max > 0)
+      hipLaunchKernelGGL(k_contact_light<DIM>, dim3((unsigned)std::min<int64_t>(blocks(c->N), 1024)), dim3(TPB), 0,
+                         c->stream, c->ds);
     const int64_t waves = std::min<int64_t>(c->N, 8192);
     hipLaunchKernelGGL(k_contact_force<DIM>, dim3((unsigned)((waves + CWAVES - 1) / CWAVES)),
                        dim3(CWAVES * WAVE), 0, c->stream, c->ds);
@@ -207,14 +210,21 @@ struct Launch {
   }
   // rebuild steps: damping partner lists, then the cell table is emptied
   static void damp_pairs(pdem_ctx* c) {
-    const unsigned nb = (unsigned)std::min<int64_t>(std::max(c->ds.n_nonwall, 1), 2048);
+    const unsigned nb = (unsigned)std::min<int64_t>(std::max(c->ds.n_nonwall, 1), 1024);
     hipLaunchKernelGGL(k_damp_pairs<DIM>, dim3(nb), dim3(64), 0, c->stream, c->ds);
   }
-  static unsigned query_blocks(pdem_ctx* c) { return (unsigned)std::min<int64_t>(blocks(c->N), 2048); }
+  // grid-stride query: a smaller grid keeps the no-op launch of non-rebuild
+  // steps short
+  static unsigned query_blocks(pdem_ctx* c) { return (unsigned)std::min<int64_t>(blocks(c->N), 1024); }
   // closure pairs: modes, network, patch records (2D, closure mode only)
   static void closure(pdem_ctx* c) {
-    if (DIM == 2 && c->ds.cl_on)
+    if (DIM == 2 && c->ds.cl_on) {
+      hipLaunchKernelGGL(k_particle_radius, dim3(radius_blocks(c)), dim3(256), 0, c->stream, c->ds);
       hipLaunchKernelGGL(k_closure, dim3(CL_BLOCKS), dim3(256), 0, c->stream, c->ds, c->ca);
+    }
+  }
+  static unsigned radius_blocks(pdem_ctx* c) {
+    return (unsigned)std::min<int64_t>((std::max(c->ds.n_nonwall, 1) + 3) / 4, 1024);
   }
   static void contact_forces(pdem_ctx* c, bool apply_damping) {
     hipStream_t st = c->stream;
@@ -1184,6 +1194,8 @@ void closure_setup(pdem_ctx* c, const float* particle_code) {
   s.cout = c->dalloc<double>((size_t)nn * s.dcap * CL_REC);
   s.plain_any = c->dalloc<int>(nn);
   s.wall_near = c->dalloc<int>(nn);
+  s.prmax = c->dalloc<double>(nn);
+  s.pzmax = c->dalloc<float>(nn);
   s.cpair_cap = (int)std::min<int64_t>((int64_t)nn * 16, (int64_t)nn * s.dcap);
   s.cpair_cap = std::max(s.cpair_cap, 1);
   s.cpairs = c->dalloc<int>(2 * (size_t)s.cpair_cap);
@@ -1263,6 +1275,8 @@ int pdem_pair_log(pdem_ctx* c, double band_h, int64_t cap, float* feat, float* t
     HIPCHK(hipMemsetAsync(dn, 0, sizeof(unsigned), c->stream));
     DS t = c->ds;
     t.cl_on = 1;
+    hipLaunchKernelGGL(k_particle_radius, dim3((unsigned)std::min<int64_t>((t.n_nonwall + 3) / 4, 1024)),
+                       dim3(256), 0, c->stream, t);
     hipLaunchKernelGGL(k_pair_log, dim3(CL_BLOCKS), dim3(256), 0, c->stream, t, c->ca, band_h, df, dt, db, dn,
                        (int)std::min<int64_t>(cap, INT_MAX));
     check_launch(c);

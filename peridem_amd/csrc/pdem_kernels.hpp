@@ -143,6 +143,8 @@ struct DS {
   const uint8_t* surf;  // [N] surface node flag
   const int* surf_ptr;  // [n_nonwall + 1] surface nodes per non-wall particle
   const int* surf_idx;
+  double* prmax;        // [n_nonwall] largest node distance from the centre node (this step)
+  float* pzmax;         // [n_nonwall] largest damage Z of the particle (this step)
   // node-update block maxima of |v|
   unsigned long long* blockmax;
   int nblockmax;
