@@ -210,12 +210,12 @@ struct Launch {
   }
   // rebuild steps: damping partner lists, then the cell table is emptied
   static void damp_pairs(pdem_ctx* c) {
-    const unsigned nb = (unsigned)std::min<int64_t>(std::max(c->ds.n_nonwall, 1), 1024);
+    const unsigned nb = (unsigned)std::min<int64_t>(std::max(c->ds.n_nonwall, 1), 4096);
     hipLaunchKernelGGL(k_damp_pairs<DIM>, dim3(nb), dim3(64), 0, c->stream, c->ds);
   }
   // grid-stride query: a smaller grid keeps the no-op launch of non-rebuild
   // steps short
-  static unsigned query_blocks(pdem_ctx* c) { return (unsigned)std::min<int64_t>(blocks(c->N), 1024); }
+  static unsigned query_blocks(pdem_ctx* c) { return (unsigned)std::min<int64_t>(blocks(c->N), 4096); }
   // closure pairs: modes, network, patch records (2D, closure mode only)
   static void closure(pdem_ctx* c) {
     if (DIM == 2 && c->ds.cl_on) {
