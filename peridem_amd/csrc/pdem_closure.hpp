@@ -395,9 +395,10 @@ __global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
       const int64_t ia = (int64_t)g.a * s.dcap + g.sa;
       s.cmode[ia] = (int8_t)mode;
       if (sb >= 0) s.cmode[(int64_t)g.b * s.dcap + sb] = (int8_t)mode;
-      if (mode == 2) {
-        atomicOr(&s.plain_any[g.a], 1);
-        atomicOr(&s.plain_any[g.b], 1);
+      if (mode == 2 || (mode == 1 && sb >= 0)) {
+        const int bit = mode == 2 ? 1 : 2;
+        atomicOr(&s.pflags[g.a], bit);
+        atomicOr(&s.pflags[g.b], bit);
       }
       if (mode == 1 && sb >= 0) {
         const double Fb[2] = {-Fa[0], -Fa[1]};

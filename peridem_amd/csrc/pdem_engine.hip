@@ -1192,7 +1192,7 @@ void closure_setup(pdem_ctx* c, const float* particle_code) {
   c->ca.code = c->upload(code.data(), nn);
   s.cmode = c->dalloc<int8_t>((size_t)nn * s.dcap);
   s.cout = c->dalloc<double>((size_t)nn * s.dcap * CL_REC);
-  s.plain_any = c->dalloc<int>(nn);
+  s.pflags = c->dalloc<int>(nn);
   s.wall_near = c->dalloc<int>(nn);
   s.prmax = c->dalloc<double>(nn);
   s.pzmax = c->dalloc<float>(nn);

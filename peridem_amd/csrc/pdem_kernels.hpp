@@ -138,7 +138,7 @@ struct DS {
   int cpair_cap;
   int8_t* cmode;        // [n_nonwall][dcap] 0: no contact possible, 1: closure, 2: plain
   double* cout;         // [n_nonwall][dcap][CL_REC] closure force records
-  int* plain_any;       // [n_nonwall] some pair of the particle is plain this step
+  int* pflags;          // [n_nonwall] this step: bit 0 some pair is plain, bit 1 some pair is closure
   int* wall_near;       // [n_nonwall] rebuild count at which a node had a wall candidate
   const uint8_t* surf;  // [N] surface node flag
   const int* surf_ptr;  // [n_nonwall + 1] surface nodes per non-wall particle
@@ -388,7 +388,7 @@ __global__ void __launch_bounds__(1024) k_ctrl(DS s) {
       s.pcx[d * nn + q] = s.x[d] ? s.x[d][cn] : 0.;
       s.pcv[d * nn + q] = s.v[d] ? s.v[d][cn] : 0.;
     }
-    if (s.cl_on) s.plain_any[q] = 0;
+    if (s.cl_on) s.pflags[q] = 0;
   }
 }
 
@@ -897,7 +897,7 @@ __device__ __forceinline__ void contact_pair(const DS& s, const double* vi, int 
 // only if its particle has a plain pair or a wall candidate, and a candidate
 // in another non-wall particle only if that pair is plain this step.
 __device__ __forceinline__ bool node_needs_contact(const DS& s, int nwi) {
-  return !s.cl_on || nwi < 0 || s.plain_any[nwi] != 0 || s.wall_near[nwi] == (int)s.ctrl->n_rebuilds;
+  return !s.cl_on || nwi < 0 || (s.pflags[nwi] & 1) != 0 || s.wall_near[nwi] == (int)s.ctrl->n_rebuilds;
 }
 __device__ __forceinline__ bool pair_plain(const DS& s, int a, int b) {
   const int* dl = s.dlist + (int64_t)a * s.dcap;
@@ -1316,7 +1316,7 @@ __global__ void k_finalize(DS s, int with_damping) {
   const DevParticle& p = s.P[pid];
   if (with_damping && p.type_index == 0)
     for (int d = 0; d < DIM; d++) s.f[d][i] += s.fdamp[3 * pid + d];
-  if (DIM == 2 && s.cl_on && p.type_index == 0 && s.surf[i]) {
+  if (DIM == 2 && s.cl_on && p.type_index == 0 && s.surf[i] && (s.pflags[p.nonwall_index] & 2)) {
     double fi[2] = {0., 0.};
     closure_node_force(s, p.nonwall_index, i, fi);
     s.f[0][i] += fi[0];
