@@ -47,7 +47,8 @@ class ContactPairDeck:
 
 @dataclass
 class BcDeck:
-    """One Displacement_BC / Force_BC set restricted to a particle list."""
+    """One Displacement_BC / Force_BC set (inp::PBCData, input.cpp:487-597):
+    a particle list, a region, or a region with include / exclude lists."""
     particles: Sequence[int]
     directions: Sequence[int]        # 1-based, as in the deck
     time_fn: int
@@ -55,6 +56,45 @@ class BcDeck:
     spatial_fn: str = "constant"
     spatial_params: Sequence[float] = ()
     is_zero: bool = False
+    region: Optional[object] = None  # object with inside(x_ref) -> bool
+    exclude: Sequence[int] = ()
+
+    def selection(self) -> str:
+        if self.region is None:
+            return "particle" if len(self.particles) else ""
+        s = "region"
+        if len(self.particles):
+            s += "_with_include_list"
+        if len(self.exclude):
+            s += "_with_exclude_list"
+        return s
+
+    def process_particle(self, pid: int) -> bool:
+        """ParticleULoading::needToProcessParticle (particleULoading.cpp:40-57)."""
+        sel = self.selection()
+        if sel in ("particle", "region_with_include_list") and pid not in self.particles:
+            return False
+        if sel == "region_with_exclude_list" and pid in self.exclude:
+            return False
+        if sel == "region_with_include_list_with_exclude_list" and (pid in self.exclude or
+                                                                    pid not in self.particles):
+            return False
+        return True
+
+    def compute_dof(self, x, pid: int) -> bool:
+        """ParticleULoading::needToComputeDof (particleULoading.cpp:59-88)."""
+        sel = self.selection()
+        if self.region is None:
+            return sel == "particle" and pid in self.particles
+        if not self.region.inside(x):
+            return False
+        if sel == "region":
+            return True
+        if sel == "region_with_include_list":
+            return pid in self.particles
+        if sel == "region_with_exclude_list":
+            return pid not in self.exclude
+        return pid in self.particles and pid not in self.exclude
 
 
 @dataclass
@@ -143,7 +183,7 @@ class ModelBuilder:
         for pid, (z, x, vv, cl, br) in enumerate(self.parts):
             zd = self.zones[z]
             md = zd.material
-            h = h_override[pid] if h_override is not None else mesh_size(x)
+            h = h_override[pid] if h_override is not None and h_override[pid] is not None else mesh_size(x)
             horizon = md.horizon
             if md.horizon_mesh_ratio > 0.0:
                 horizon = md.horizon_mesh_ratio * h
@@ -206,8 +246,7 @@ class ModelBuilder:
                 mask |= 1 << (d - 1)
             nodes, amp = self._bc_nodes(bc, starts, x_ref, is_force=False)
             # setFixity marks every selected node, zero sets included
-            for pid in bc.particles:
-                fix[starts[pid]:starts[pid + 1]] |= np.uint8(mask)
+            fix[nodes] |= np.uint8(mask)
             if bc.is_zero:
                 # ParticleULoading::apply marks a zero set applied on its first call,
                 # which is for particle 0; only particle 0's nodes are ever written
@@ -233,8 +272,15 @@ class ModelBuilder:
 
     def _bc_nodes(self, bc: BcDeck, starts, x_ref, is_force):
         nodes, amp = [], []
-        for pid in bc.particles:
+        for pid in range(len(self.parts)):
+            if not bc.process_particle(pid):
+                continue
             idx = np.arange(starts[pid], starts[pid + 1], dtype=np.int64)
+            if bc.region is not None or pid not in bc.particles:
+                keep = np.array([bc.compute_dof(x_ref[i], pid) for i in idx], bool)
+                idx = idx[keep]
+            if len(idx) == 0:
+                continue
             x = x_ref[idx]
             base = 1.0 if is_force else float(bc.time_params[0])
             a = np.full(len(idx), base)
