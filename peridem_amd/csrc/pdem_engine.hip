@@ -192,8 +192,9 @@ struct Launch {
     const unsigned nb = blocks(c->N);
     hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, st, c->ds);
     hipLaunchKernelGGL(k_bin<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
-    hipLaunchKernelGGL(k_query<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
-    damp_pairs(c);
+    rebuild_lists(c);
+    hipLaunchKernelGGL(k_cells_init, dim3((unsigned)std::min<int64_t>(blocks(c->ds.ncells), 4096)), dim3(TPB), 0,
+                       st, c->ds);
   }
   static void contact_force_kernel(pdem_ctx* c) {
     // lane-per-node list (grid-stride); empty by construction when light_max == 0
@@ -208,14 +209,13 @@ struct Launch {
     if (c->ds.n_nonwall > 0)
       hipLaunchKernelGGL(k_damping<DIM>, dim3(c->ds.n_nonwall), dim3(64), 0, c->stream, c->ds);
   }
-  // rebuild steps: damping partner lists, then the cell table is emptied
-  static void damp_pairs(pdem_ctx* c) {
-    const unsigned nb = (unsigned)std::min<int64_t>(std::max(c->ds.n_nonwall, 1), 4096);
-    hipLaunchKernelGGL(k_damp_pairs<DIM>, dim3(nb), dim3(64), 0, c->stream, c->ds);
+  // rebuild steps: candidate query and damping partner lists in one launch
+  // (grid-stride, capped grids keep the no-op launch of other steps short)
+  static void rebuild_lists(pdem_ctx* c) {
+    const int qb = (int)std::min<int64_t>(blocks(c->N), 4096);
+    const int db = (int)std::min<int64_t>((std::max(c->ds.n_nonwall, 1) + 3) / 4, 1024);
+    hipLaunchKernelGGL(k_rebuild_lists<DIM>, dim3((unsigned)(qb + db)), dim3(TPB), 0, c->stream, c->ds, qb);
   }
-  // grid-stride query: a smaller grid keeps the no-op launch of non-rebuild
-  // steps short
-  static unsigned query_blocks(pdem_ctx* c) { return (unsigned)std::min<int64_t>(blocks(c->N), 4096); }
   // closure pairs: modes, network, patch records (2D, closure mode only)
   static void closure(pdem_ctx* c) {
     if (DIM == 2 && c->ds.cl_on) {
@@ -257,8 +257,7 @@ struct Launch {
   static void forces_after_ctrl(pdem_ctx* c) {
     pd_forces(c, c->multi ? 1 : 0);
     if (c->multi) {
-      hipLaunchKernelGGL(k_query<DIM>, dim3(query_blocks(c)), dim3(TPB), 0, c->stream, c->ds);
-      damp_pairs(c);
+      rebuild_lists(c);
       closure(c);
       contact_force_kernel(c);
       damping(c);

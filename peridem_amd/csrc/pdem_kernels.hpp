@@ -356,7 +356,7 @@ __device__ void ctrl_decide(Ctrl* cg) {
 // Controller step.  Workgroup 0 reduces the node-update block maxima of |v|
 // and runs the controller; all workgroups pack the non-wall particles' centre
 // data (and clear the closure's per-step flags).  The cell table is cleared
-// after each rebuild's query (k_damp_pairs), so no step clears it here.
+// after each rebuild's query (k_contact_force), so no step clears it here.
 constexpr int CTRL_BLOCKS = 64;
 
 __global__ void __launch_bounds__(1024) k_ctrl(DS s) {
@@ -918,26 +918,23 @@ __device__ __forceinline__ bool pair_plain(const DS& s, int a, int b) {
 // Candidate query on a rebuild: one lane per searching node; nodes that end
 // with candidates (and compute forces) are appended to the active list.
 template <int DIM>
-__global__ void __launch_bounds__(256) k_query(DS s) {
-  if (!s.ctrl->rebuild) return;
-  for (int64_t i = gid(); i < s.N; i += (int64_t)gridDim.x * blockDim.x) {
-    const int pid = s.ptid[i];
-    const DevParticle& pi = s.P[pid];
-    if (!pi.searches) continue;
-    query_node<DIM>(s, i, pid);
-    const int cnt = s.ccount[i];
-    if (s.cl_on && pi.nonwall_index >= 0)
-      for (int k = 0; k < cnt; k++)
-        if (s.pmeta[s.ptid[s.cand[(int64_t)k * s.N + i]]] >> 16) {
-          s.wall_near[pi.nonwall_index] = (int)s.ctrl->n_rebuilds;
-          break;
-        }
-    if (cnt > 0 && pi.compute_force) {
-      if (cnt <= s.light_max)
-        s.light[atomicAdd(&s.ctrl->nlight, 1u)] = (int)i;
-      else
-        s.active[atomicAdd(&s.ctrl->nactive, 1u)] = (int)i;
-    }
+__device__ __forceinline__ void query_one(const DS& s, int64_t i) {
+  const int pid = s.ptid[i];
+  const DevParticle& pi = s.P[pid];
+  if (!pi.searches) return;
+  query_node<DIM>(s, i, pid);
+  const int cnt = s.ccount[i];
+  if (s.cl_on && pi.nonwall_index >= 0)
+    for (int k = 0; k < cnt; k++)
+      if (s.pmeta[s.ptid[s.cand[(int64_t)k * s.N + i]]] >> 16) {
+        s.wall_near[pi.nonwall_index] = (int)s.ctrl->n_rebuilds;
+        break;
+      }
+  if (cnt > 0 && pi.compute_force) {
+    if (cnt <= s.light_max)
+      s.light[atomicAdd(&s.ctrl->nlight, 1u)] = (int)i;
+    else
+      s.active[atomicAdd(&s.ctrl->nactive, 1u)] = (int)i;
   }
 }
 
@@ -1032,6 +1029,8 @@ template <int DIM>
 __global__ void __launch_bounds__(CWAVES * WAVE) k_contact_force(DS s) {
   __shared__ int sj[CWAVES][MAXACT];
   __shared__ double st[CWAVES][MAXACT][6];
+  // this rebuild's query is done: leave the cell table empty for the next
+  if (s.ctrl->rebuild) clear_cells(s, gid(), (int64_t)gridDim.x * blockDim.x);
   const int lane = threadIdx.x & (WAVE - 1);
   const int wv = threadIdx.x / WAVE;
   const int nwaves = gridDim.x * CWAVES;
@@ -1130,9 +1129,8 @@ __global__ void __launch_bounds__(CWAVES * WAVE) k_contact_force(DS s) {
 // Damping partners on a rebuild: one wavefront per non-wall particle, lanes
 // over the other particles, ballot-ordered append (ascending index).
 template <int DIM>
-__device__ void damp_pairs_one(const DS& s, int q) {
+__device__ void damp_pairs_one(const DS& s, int q, int lane) {
   const int nn = s.n_nonwall;
-  const int lane = threadIdx.x;
   const double Ri = s.prad[q];
   const double margin = 2.0 * s.ctrl->radius;
   const int zi = s.pzone[q];
@@ -1178,13 +1176,24 @@ __device__ void damp_pairs_one(const DS& s, int q) {
   }
 }
 
+// Rebuild steps only: the candidate query (radiusSearchExcludeTag per
+// searching node, demModel.cpp:1717-1756) on workgroups [0, qblocks) and the
+// damping partner lists (a wavefront per particle) on the others; the two are
+// independent.  The cell table is emptied afterwards by the contact-force
+// kernel (or k_cells_init outside the step).
 template <int DIM>
-__global__ void __launch_bounds__(64) k_damp_pairs(DS s) {
+__global__ void __launch_bounds__(256) k_rebuild_lists(DS s, int qblocks) {
   if (!s.ctrl->rebuild) return;
-  // the query of this rebuild is done: leave the cell table empty for the next
-  clear_cells(s, gid(), (int64_t)gridDim.x * blockDim.x);
-  const int nn = s.n_nonwall;
-  for (int q = blockIdx.x; q < nn; q += gridDim.x) damp_pairs_one<DIM>(s, q);
+  if ((int)blockIdx.x < qblocks) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < s.N; i += (int64_t)qblocks * blockDim.x)
+      query_one<DIM>(s, i);
+    return;
+  }
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int nw = ((int)gridDim.x - qblocks) * (int)(blockDim.x / WAVE);
+  for (int q = ((int)blockIdx.x - qblocks) * (int)(blockDim.x / WAVE) + (int)(threadIdx.x / WAVE);
+       q < s.n_nonwall; q += nw)
+    damp_pairs_one<DIM>(s, q, lane);
 }
 
 // Damping (demModel.cpp:937-1052): one wavefront per non-wall particle.
