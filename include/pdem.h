@@ -306,6 +306,17 @@ int pdem_closure_stats(pdem_ctx *ctx, int64_t *pairs_by_mode3, int64_t *checked,
 int pdem_pair_log(pdem_ctx *ctx, double band_h, int64_t cap, float *features, float *targets, float *baseline,
                   int64_t *n);
 
+/* Checkpoint / restore of the complete step state (node fields, bond bits,
+ * damage, the cadence controller's counters, the candidate and partner lists
+ * of the last rebuild, zero-set flags): resuming from a checkpoint continues
+ * bit for bit.  The reference's restart (vtkParticleReader.cpp:41-115) reads
+ * positions and velocities only.  The buffer is host memory of
+ * pdem_checkpoint_size() bytes; load requires a context built from the same
+ * descriptor (and the same closure set-up). */
+int pdem_checkpoint_size(pdem_ctx *ctx, int64_t *bytes);
+int pdem_checkpoint_save(pdem_ctx *ctx, void *buf, int64_t bytes);
+int pdem_checkpoint_load(pdem_ctx *ctx, const void *buf, int64_t bytes);
+
 /* number of HIP devices visible to the engine (0 when none); no device work */
 int pdem_device_count(void);
 

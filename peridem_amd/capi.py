@@ -343,6 +343,12 @@ def engine_library():
         lib.pdem_pair_log.restype = C.c_int
         lib.pdem_pair_log.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.POINTER(C.c_float),
                                       C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int64)]
+        lib.pdem_checkpoint_size.restype = C.c_int
+        lib.pdem_checkpoint_size.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        lib.pdem_checkpoint_save.restype = C.c_int
+        lib.pdem_checkpoint_save.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        lib.pdem_checkpoint_load.restype = C.c_int
+        lib.pdem_checkpoint_load.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
         lib.pdem_autotune.restype = C.c_int
         lib.pdem_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
         lib.pdem_get_launch_config.restype = C.c_int
@@ -376,6 +382,18 @@ class GpuEngine(Engine):
         n = C.c_int64()
         self._check(self.lib.pdem_profile_get(self.ctx, C.byref(ms), C.byref(n)), "profile_get")
         return ms.value, n.value
+
+    def checkpoint(self) -> bytes:
+        """The complete step state; restore() on a context of the same model
+        continues bit for bit."""
+        n = C.c_int64()
+        self._check(self.lib.pdem_checkpoint_size(self.ctx, C.byref(n)), "checkpoint_size")
+        buf = C.create_string_buffer(n.value)
+        self._check(self.lib.pdem_checkpoint_save(self.ctx, buf, n.value), "checkpoint_save")
+        return buf.raw
+
+    def restore(self, blob: bytes):
+        self._check(self.lib.pdem_checkpoint_load(self.ctx, blob, len(blob)), "checkpoint_load")
 
     def autotune(self, reps=5):
         """Pick the fastest launch shape of the force pass; returns its ms."""

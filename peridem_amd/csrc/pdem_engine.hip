@@ -39,6 +39,8 @@ struct pdem_ctx {
   std::vector<DevBc> ubc, fbc;
   std::vector<int> ubc_zero;
   std::vector<char> zero_done;
+  // device arrays that carry state from step to step (checkpoint / restore)
+  std::vector<std::pair<void*, size_t>> persist;
   int64_t nslices = 0, nnz_padded = 0, nbitwords = 0, nnz = 0;
   std::vector<int64_t> h_sptr, h_bptr;
   std::vector<int> h_deg;
@@ -683,6 +685,26 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     s.nblockmax = (int)blocks(N);
     s.blockmax = c->dalloc<unsigned long long>(s.nblockmax);
   }
+  // state carried between steps: node fields, bond bits, damage, controller,
+  // candidate and partner lists of the last rebuild (everything else is
+  // constant or recomputed inside every step)
+  for (int k = 0; k < dim; k++)
+    for (double* a : {s.x[k], s.u[k], s.v[k], s.f[k]}) c->persist.push_back({a, (size_t)N * 8});
+  c->persist.push_back({s.vmag, (size_t)N * 8});
+  c->persist.push_back({s.theta, (size_t)N * 8});
+  c->persist.push_back({s.Z, (size_t)N * 4});
+  c->persist.push_back({s.bits, (size_t)c->nbitwords * 4});
+  c->persist.push_back({s.ctrl, sizeof(Ctrl)});
+  c->persist.push_back({s.blockmax, (size_t)s.nblockmax * 8});
+  if (c->multi) {
+    c->persist.push_back({s.ccount, (size_t)N * 4});
+    c->persist.push_back({s.cand, (size_t)c->cap * N * 4});
+    c->persist.push_back({s.active, (size_t)N * 4});
+    c->persist.push_back({s.light, (size_t)N * 4});
+    const size_t nn = (size_t)std::max(1, s.n_nonwall);
+    c->persist.push_back({s.dcount, nn * 4});
+    c->persist.push_back({s.dlist, nn * s.dcap * 4});
+  }
   // weighted volume for every node (computeStateMx), then the init-time
   // contact search (DEMModel::init calls updateContactNeighborlist once)
   dispatch(
@@ -1200,6 +1222,8 @@ void closure_setup(pdem_ctx* c, const float* particle_code) {
   s.cpairs = c->dalloc<int>(2 * (size_t)s.cpair_cap);
   c->ca.stats = c->dalloc<unsigned long long>(8);
   c->ca.err_sum = c->dalloc<double>(2);
+  c->persist.push_back({s.cpairs, 2 * (size_t)s.cpair_cap * 4});
+  c->persist.push_back({s.wall_near, (size_t)nn * 4});
   c->cl_ready = true;
 }
 
@@ -1293,6 +1317,75 @@ int pdem_pair_log(pdem_ctx* c, double band_h, int64_t cap, float* feat, float* t
     c->dfree(dt);
     c->dfree(db);
     c->dfree(dn);
+  });
+}
+
+// Checkpoint layout: "PDEMCKP1", int64 N, int32 dim, int32 number of
+// blocks, int64 number of zero-set flags, int64 size of every block, the
+// zero-set flags (one byte each), then the blocks.
+static int64_t ckp_bytes(const pdem_ctx* c) {
+  int64_t b = 8 + 8 + 4 + 4 + 8 + 8 * (int64_t)c->persist.size() + (int64_t)c->zero_done.size();
+  for (auto& q : c->persist) b += (int64_t)q.second;
+  return b;
+}
+
+int pdem_checkpoint_size(pdem_ctx* c, int64_t* bytes) {
+  return guarded(c, [&] { *bytes = ckp_bytes(c); });
+}
+
+int pdem_checkpoint_save(pdem_ctx* c, void* buf, int64_t bytes) {
+  return guarded(c, [&] {
+    if (bytes < ckp_bytes(c)) throw HipError{"checkpoint buffer too small"};
+    char* o = (char*)buf;
+    std::memcpy(o, "PDEMCKP1", 8);
+    o += 8;
+    const int64_t N = c->N, nz = (int64_t)c->zero_done.size();
+    const int32_t dim = c->dim, nb = (int32_t)c->persist.size();
+    std::memcpy(o, &N, 8);
+    std::memcpy(o + 8, &dim, 4);
+    std::memcpy(o + 12, &nb, 4);
+    std::memcpy(o + 16, &nz, 8);
+    o += 24;
+    for (auto& q : c->persist) {
+      const int64_t sz = (int64_t)q.second;
+      std::memcpy(o, &sz, 8);
+      o += 8;
+    }
+    for (char z : c->zero_done) *o++ = z;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& q : c->persist) {
+      HIPCHK(hipMemcpy(o, q.first, q.second, hipMemcpyDeviceToHost));
+      o += q.second;
+    }
+  });
+}
+
+int pdem_checkpoint_load(pdem_ctx* c, const void* buf, int64_t bytes) {
+  return guarded(c, [&] {
+    if (bytes < 32 || std::memcmp(buf, "PDEMCKP1", 8) != 0) throw HipError{"not a checkpoint"};
+    const char* o = (const char*)buf + 8;
+    int64_t N, nz;
+    int32_t dim, nb;
+    std::memcpy(&N, o, 8);
+    std::memcpy(&dim, o + 8, 4);
+    std::memcpy(&nb, o + 12, 4);
+    std::memcpy(&nz, o + 16, 8);
+    o += 24;
+    if (N != c->N || dim != c->dim || nb != (int32_t)c->persist.size() || nz != (int64_t)c->zero_done.size() ||
+        bytes != ckp_bytes(c))
+      throw HipError{"checkpoint does not match this model"};
+    for (auto& q : c->persist) {
+      int64_t sz;
+      std::memcpy(&sz, o, 8);
+      if (sz != (int64_t)q.second) throw HipError{"checkpoint does not match this model"};
+      o += 8;
+    }
+    for (auto& z : c->zero_done) z = *o++;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto& q : c->persist) {
+      HIPCHK(hipMemcpy(q.first, o, q.second, hipMemcpyHostToDevice));
+      o += q.second;
+    }
   });
 }
 

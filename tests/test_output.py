@@ -1,0 +1,35 @@
+"""VTU and compressive-test CSV output (peridem_amd/output.py), on the oracle's
+state: the arrays read back equal the state bit for bit (17 significant
+digits), with the reference's array names and component counts."""
+import numpy as np
+
+from peridem_amd import capi, configs, output
+import oracle
+
+
+def test_vtu_round_trip(tmp_path):
+    spec = configs.two_particle(gap=0.0, speed=0.1, dt=1.0e-7)
+    o = oracle.OracleEngine(spec, 4)
+    o.integrate_begin()
+    o.integrate_cd(15)
+    path = tmp_path / "out_0.vtu"
+    output.write_vtu(str(path), o, spec)
+    pts, data = output.read_vtu(str(path))
+    np.testing.assert_array_equal(pts, o.field(capi.FIELD_X))
+    np.testing.assert_array_equal(data["Displacement"], o.field(capi.FIELD_U))
+    np.testing.assert_array_equal(data["Velocity"], o.field(capi.FIELD_V))
+    np.testing.assert_array_equal(data["Force_Density"], o.field(capi.FIELD_F))
+    np.testing.assert_array_equal(data["Force"], o.field(capi.FIELD_F) * spec.vol[:, None])
+    np.testing.assert_array_equal(data["Damage_Z"], o.field(capi.FIELD_Z).astype(np.float64))
+    np.testing.assert_array_equal(data["Particle_ID"], spec.pt_id)
+    np.testing.assert_array_equal(data["Nodal_Volume"], spec.vol)
+    assert set(data) == set(output.DEFAULT_TAGS)
+    assert data["Displacement"].shape == (spec.n_nodes, 3) and data["Theta"].shape == (spec.n_nodes,)
+
+
+def test_compressive_csv(tmp_path):
+    rows = [(1e-7 * k, -1e-9 * k, 0.5 * k) for k in range(5)]
+    path = tmp_path / "pp.csv"
+    output.write_compressive_csv(str(path), rows)
+    back = np.loadtxt(str(path), delimiter=",", skiprows=1)
+    np.testing.assert_array_equal(back, np.array(rows))
