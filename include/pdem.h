@@ -35,6 +35,8 @@ extern "C" {
 /* material laws (reference src/material/mparticle/material.h) */
 #define PDEM_MAT_RNP 0       /* RnpMaterial      material.h:251-659   */
 #define PDEM_MAT_PDSTATE 1   /* PdState          material.h:1330-1724 */
+#define PDEM_MAT_PMB 2       /* PmbMaterial      material.h:665-1023  */
+#define PDEM_MAT_PDELASTIC 3 /* PdElastic        material.h:1025-1324 */
 
 /* influence functions (reference src/material/mparticle/influenceFn.cpp:14-122) */
 #define PDEM_INF_CONST 0
@@ -96,7 +98,8 @@ typedef struct pdem_particle {
   double Kn;                 /* broken-bond contact stiffness 18 K / (pi delta^5) */
   double Rc;                 /* broken-bond contact radius 0.95 h */
   /* material parameters: RNP   {C, beta, rbar, inv_factor, check_sc_factor}
-                          PDState {K, G, s0} */
+                          PDState {K, G, s0}
+                          PMB {c, s0}   PdElastic {c} */
   double mat[6];
 } pdem_particle;
 

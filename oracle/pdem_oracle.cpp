@@ -9,7 +9,8 @@
  * -ffp-contract=off, the x86-64 -O3 arithmetic of the reference):
  *   node update            src/model/dem/demModel.cpp:349-390
  *   computeForces          demModel.cpp:462-600
- *   PD forces (both laws)  demModel.cpp:602-789, material.h:342-368,405-417,1415-1451
+ *   PD forces              demModel.cpp:602-789; RNP material.h:342-368,405-417,
+ *                          PMB :732-793, PdElastic :1090-1136, PdState :1415-1451
  *   external forces        demModel.cpp:791-812, particleFLoading.cpp:85-173
  *   displacement BC        particleULoading.cpp:115-230
  *   contact + damping      demModel.cpp:820-1066
@@ -34,6 +35,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <limits>
 #include <condition_variable>
 #include <cstdio>
 #include <cstring>
@@ -433,6 +435,15 @@ inline double rnpBondF(double J, double r, double S, const double *mat, int irre
   if (fs) return 0.;
   return J * 4. * S * Cc * beta * std::exp(-beta * r * S * S) / invF;
 }
+// PmbMaterial::getBondEF(r, s, fs, break_bonds = true), force part
+// (material.h:732-757); PdElastic::getBondEF (material.h:1090-1097) never
+// breaks.  J = getInfFn(r); fs updated in place.
+inline double pmbBondF(double J, double S, double cc, double s0, bool &fs) {
+  if (!fs && isGreater(std::abs(S), s0 + 1.0e-10)) fs = true;
+  if (fs) return 0.;
+  return J * cc * S;
+}
+inline double pdelasticBondF(double J, double S, double cc) { return J * cc * S; }
 // PdState::getBondEF(r, s, fs, mx, thetax), force part (material.h:1415-1429)
 inline double pdstateBondF(double J, double r, double S, double K, double G, double mx,
                            double thetax) {
@@ -489,6 +500,9 @@ void computePeridynamicForces(oracle_ctx *c) {
       const double mi = c->mX[i], thetai = c->theta[i];
       const double check_up = horizon + 0.5 * h, check_low = horizon - 0.5 * h;
       const bool state = p.material_type == PDEM_MAT_PDSTATE;
+      // PMB and PdElastic: actual stretch and deformed direction like PdState,
+      // pairwise force like RNP (demModel.cpp:741-756)
+      const bool deformed = p.material_type == PDEM_MAT_PMB || p.material_type == PDEM_MAT_PDELASTIC;
       for (int64_t k = c->prow[i]; k < c->prow[i + 1]; k++) {
         int64_t j = c->pcol[k];
         bool fs = c->bits[k];
@@ -497,9 +511,9 @@ void computePeridynamicForces(oracle_ctx *c) {
         const P3 dx = sub(xj, xi), du = sub(uj, ui);
         double rji = len(dx);
         double Sji;
-        if (state) {
+        if (state || deformed) {
           const P3 dy = add(dx, du);
-          Sji = (len(dy) - len(dx)) / len(dx);  // PdState::getS, material.h:1449-1451
+          Sji = (len(dy) - len(dx)) / len(dx);  // getS, material.h:791-793, 1134-1136, 1449-1451
         } else {
           Sji = dot(dx, du) / dot(dx, dx);  // RnpMaterial::getS, material.h:405-407
         }
@@ -513,6 +527,13 @@ void computePeridynamicForces(oracle_ctx *c) {
             double ef_i = pdstateBondF(J, rji, Sji, K, G, mi, thetai);
             double ef_j = pdstateBondF(J, rji, Sji, K, G, mj, thetaj);
             double scalar_f = (ef_i + ef_j) * volj;
+            const P3 dy = add(dx, du);
+            force = add(force, scale(scalar_f, divs(dy, len(dy))));
+          } else if (deformed) {
+            double ef = p.material_type == PDEM_MAT_PMB ? pmbBondF(J, Sji, p.mat[0], p.mat[1], fs)
+                                                         : pdelasticBondF(J, Sji, p.mat[0]);
+            c->bits[k] = fs;
+            double scalar_f = ef * volj;
             const P3 dy = add(dx, du);
             force = add(force, scale(scalar_f, divs(dy, len(dy))));
           } else {
@@ -531,6 +552,8 @@ void computePeridynamicForces(oracle_ctx *c) {
           force = add(force, scale(scalar_f, yji));
         }
         double Sc = state ? p.mat[2] : p.mat[2] / std::sqrt(rji);
+        if (p.material_type == PDEM_MAT_PMB) Sc = p.mat[1];
+        if (p.material_type == PDEM_MAT_PDELASTIC) Sc = std::numeric_limits<double>::max();
         if (isGreater(std::abs(Sji / Sc), Zi)) Zi = (float)std::abs(Sji / Sc);
       }
       c->f[i] = force;
@@ -1001,6 +1024,12 @@ int oracle_rnp_bond_f(double J, double r, double S, const double *mat, int irrev
                       double *f) {
   bool fs = fs_in != 0;
   *f = rnpBondF(J, r, S, mat, irrev, fs);
+  return fs ? 1 : 0;
+}
+/* PMB (law 2) / PdElastic (law 3) force part; returns fs after the call */
+int oracle_bond_f(int law, double J, double S, double cc, double s0, int fs_in, double *f) {
+  bool fs = fs_in != 0;
+  *f = law == PDEM_MAT_PMB ? pmbBondF(J, S, cc, s0, fs) : pdelasticBondF(J, S, cc);
   return fs ? 1 : 0;
 }
 double oracle_pdstate_bond_f(double J, double r, double S, double K, double G, double mx,

@@ -6,7 +6,7 @@
  * the reference's own code.
  *
  * Reference sources used as they lie:
- *   src/material/mparticle/material.h      RnpMaterial, PdState
+ *   src/material/mparticle/material.h      RnpMaterial, PmbMaterial, PdElastic, PdState
  *   src/material/mparticle/influenceFn.cpp influence functions
  *   src/util/function.cpp                  isGreater / isLess
  *   src/nsearch/nsearch.h + nanoflann      kd-tree radius searches
@@ -116,6 +116,61 @@ double ref_pdstate_get_s(const double *dx, const double *du) {
   inp::MaterialDeck deck = makeDeck("PDState", -1., 1e9, 5e8, -1., 1., 1., 0, nullptr, 0);
   material::PdState m(deck, 3, 1.);
   return m.getS(util::Point(dx[0], dx[1], dx[2]), util::Point(du[0], du[1], du[2]));
+}
+
+/* PmbMaterial / PdElastic (bond-based, actual stretch, deformed direction):
+ * out: c, s0 (PMB; 0 for PdElastic), K_from_material, J(0.5 horizon).
+ * c and s0 are private: s0 = getSc(r) for PMB, c from the unbroken force
+ * getBondEF(r, 1, fs, false).second / J(r). */
+void ref_bond_params(int law, double E, double Gc, double rho, double horizon, int dim, int inf_type,
+                     const double *inf_params, int n_inf, double *out) {
+  const double r = 0.5 * horizon;
+  bool fs = false;
+  if (law == 2) {
+    auto deck = makeDeck("PMBBond", E, -1., -1., -1., Gc, rho, inf_type, inf_params, n_inf);
+    material::PmbMaterial m(deck, (size_t)dim, horizon);
+    const double J = m.getInfFn(r);
+    out[0] = m.getBondEF(r, 1.0, fs, false).second / J;
+    out[1] = m.getSc(r);
+    out[2] = m.computeMaterialProperties((size_t)dim).d_K;
+    out[3] = J;
+  } else {
+    auto deck = makeDeck("PDElasticBond", E, -1., -1., -1., Gc, rho, inf_type, inf_params, n_inf);
+    material::PdElastic m(deck, (size_t)dim, horizon);
+    const double J = m.getInfFn(r);
+    out[0] = m.getBondEF(r, 1.0, fs, false).second / J;
+    out[1] = 0.;
+    out[2] = m.computeMaterialProperties((size_t)dim).d_K;
+    out[3] = J;
+  }
+}
+
+/* getBondEF(r, s, fs, true) of PMB (law 2) or PdElastic (law 3): returns fs
+ * after the call, e and f in out[0..1]; out[2] = getS(dx, du) of the same
+ * class for the given vectors, out[3] = getSc(r) */
+int ref_bond_ef(int law, double E, double Gc, double rho, double horizon, int dim, int inf_type,
+                const double *inf_params, int n_inf, double r, double s, int fs_in, const double *dx,
+                const double *du, double *out) {
+  bool fs = fs_in != 0;
+  const util::Point a(dx[0], dx[1], dx[2]), b(du[0], du[1], du[2]);
+  if (law == 2) {
+    auto deck = makeDeck("PMBBond", E, -1., -1., -1., Gc, rho, inf_type, inf_params, n_inf);
+    material::PmbMaterial m(deck, (size_t)dim, horizon);
+    auto ef = m.getBondEF(r, s, fs, true);
+    out[0] = ef.first;
+    out[1] = ef.second;
+    out[2] = m.getS(a, b);
+    out[3] = m.getSc(r);
+  } else {
+    auto deck = makeDeck("PDElasticBond", E, -1., -1., -1., Gc, rho, inf_type, inf_params, n_inf);
+    material::PdElastic m(deck, (size_t)dim, horizon);
+    auto ef = m.getBondEF(r, s, fs, true);
+    out[0] = ef.first;
+    out[1] = ef.second;
+    out[2] = m.getS(a, b);
+    out[3] = m.getSc(r);
+  }
+  return fs ? 1 : 0;
 }
 
 /* Influence-function value J(r) for a type */

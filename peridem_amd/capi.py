@@ -12,6 +12,8 @@ import numpy as np
 
 MAT_RNP = 0
 MAT_PDSTATE = 1
+MAT_PMB = 2
+MAT_PDELASTIC = 3
 
 INF_CONST = 0
 INF_LINEAR = 1

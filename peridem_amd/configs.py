@@ -28,6 +28,18 @@ def pdstate_material(K=2.16e7, G=1.296e7, rho=1200.0, Gc=50.0, ratio=3.0, inf=ca
                         influence_fn_type=inf)
 
 
+def pmb_material(E=2.0e9, rho=1200.0, Gc=500.0, ratio=3.0, inf=capi.INF_CONST):
+    """Bond-based PMB (nu fixed at 1/4 by the model)."""
+    return MaterialDeck(type="PMBBond", E=E, K=E / 1.5, Gc=Gc, density=rho, horizon_mesh_ratio=ratio,
+                        influence_fn_type=inf)
+
+
+def pdelastic_material(E=2.0e9, rho=1200.0, ratio=3.0, inf=capi.INF_CONST):
+    """Bond-based linear elastic PD (no fracture; nu fixed at 1/4)."""
+    return MaterialDeck(type="PDElasticBond", E=E, K=E / 1.5, density=rho, horizon_mesh_ratio=ratio,
+                        influence_fn_type=inf)
+
+
 def _kn(mat: MaterialDeck, horizon: float) -> float:
     """Deck Kn = 18 K / (pi delta^5), as the reference's deck generators write it
     (test/test_data/peridem/twop_circ/inp/problem_setup.py:195-197)."""

@@ -44,7 +44,7 @@ struct pdem_ctx {
   int64_t nslices = 0, nnz_padded = 0, nbitwords = 0, nnz = 0;
   std::vector<int64_t> h_sptr, h_bptr;
   std::vector<int> h_deg;
-  bool state_any = false, rnp_any = false;
+  bool state_any = false, rnp_any = false, pmb_any = false, elastic_any = false;
   int64_t cap = 0;
   MixedArgs mixed{};
   int ntiles = 0, ntiles_global = 0;
@@ -155,6 +155,8 @@ struct Launch {
     if (c->precision == PDEM_PREC_MIXED && c->ntiles > 0) {
       hipLaunchKernelGGL(k_bin_zero<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds, bin);
       if (c->rnp_any) hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(nb), dim3(TPB), 0, st, c->ds, 0);
+      if (c->pmb_any) hipLaunchKernelGGL((k_pd_bond<DIM, INF, 2>), dim3(nb), dim3(TPB), 0, st, c->ds, 0);
+      if (c->elastic_any) hipLaunchKernelGGL((k_pd_bond<DIM, INF, 3>), dim3(nb), dim3(TPB), 0, st, c->ds, 0);
       if (c->mixed_tpb == 256)
         mixed_dilation<INF, 256>(c);
       else
@@ -173,8 +175,16 @@ struct Launch {
       bin = 0;
     }
     auto ev = ev_begin(c);
-    if (c->rnp_any || !c->state_any) {
+    if (c->rnp_any || !(c->state_any || c->pmb_any || c->elastic_any)) {
       hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(pnb), dim3(pb), 0, st, c->ds, bin);
+      bin = 0;
+    }
+    if (c->pmb_any) {
+      hipLaunchKernelGGL((k_pd_bond<DIM, INF, 2>), dim3(pnb), dim3(pb), 0, st, c->ds, bin);
+      bin = 0;
+    }
+    if (c->elastic_any) {
+      hipLaunchKernelGGL((k_pd_bond<DIM, INF, 3>), dim3(pnb), dim3(pb), 0, st, c->ds, bin);
       bin = 0;
     }
     if (c->state_any) hipLaunchKernelGGL((k_pd_state<DIM, INF>), dim3(pnb), dim3(pb), 0, st, c->ds, bin);
@@ -436,6 +446,8 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     for (int k = 0; k < 6; k++) o.mat[k] = q.mat[k];
     if (q.material_type == PDEM_MAT_PDSTATE) c->state_any = true;
     if (q.material_type == PDEM_MAT_RNP) c->rnp_any = true;
+    if (q.material_type == PDEM_MAT_PMB) c->pmb_any = true;
+    if (q.material_type == PDEM_MAT_PDELASTIC) c->elastic_any = true;
   }
   s.P = c->upload(dp.data(), dp.size());
   std::vector<DevContact> dc((size_t)c->nz * c->nz);
@@ -806,7 +818,7 @@ int pdem_create(const pdem_desc* d, pdem_ctx** out) {
     if (q.center_node < q.glob_start || q.center_node >= q.glob_end)
       return set_err(nullptr, "center node outside its particle");
     if (q.zone < 0 || (d->multi_particle && q.zone >= d->n_zones)) return set_err(nullptr, "bad zone id");
-    if (q.material_type != PDEM_MAT_RNP && q.material_type != PDEM_MAT_PDSTATE)
+    if (q.material_type < PDEM_MAT_RNP || q.material_type > PDEM_MAT_PDELASTIC)
       return set_err(nullptr, "material type not supported on the GPU path");
     if (q.glob_end - q.glob_start > INT_MAX / 2) return set_err(nullptr, "particle too large");
     expect = q.glob_end;

@@ -17,6 +17,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <cfloat>
 #include <climits>
 
 namespace pdem {
@@ -639,6 +640,82 @@ __global__ void __launch_bounds__(256) k_pd_rnp(DS s, int bin) {
     } else {
       double y[3];
       for (int d = 0; d < DIM; d++) y[d] = (xj[d] + uj[d]) - (xi[d] + ui[d]);
+      const double R = len3<DIM>(y);
+      double scalar_f = Kn * volj * (R - Rc) / R;
+      if (scalar_f > 0.) scalar_f = 0.;
+      for (int d = 0; d < DIM; d++) force[d] += scalar_f * y[d];
+    }
+    if (isGreater(fabs(Sji / Sc), (double)Zi)) Zi = (float)fabs(Sji / Sc);
+  }
+  if (n > 0 && wnew != word) bw[(int64_t)((n - 1) >> 5) * WAVE] = wnew;
+  for (int d = 0; d < DIM; d++) s.f[d][i] = force[d];
+  s.Z[i] = Zi;
+}
+
+// Bond-based PMB (LAW 2) and PdElastic (LAW 3) force pass
+// (demModel.cpp:682-788 with PmbMaterial / PdElastic, material.h:665-1324):
+// actual stretch S = (|xi + eta| - |xi|) / |xi|, force J c S along the
+// deformed bond (xi + eta) / |xi + eta|; PMB bonds break once
+// |S| > s0 + 1e-10 (relative compare), PdElastic bonds never break; broken
+// bonds give the short-range repulsion; damage max |S / Sc| with Sc = s0
+// (PMB) or the largest double (PdElastic).
+template <int DIM, int INF, int LAW>
+__global__ void __launch_bounds__(256) k_pd_bond(DS s, int bin) {
+  int64_t i = gid();
+  if (i >= s.N) return;
+  if (bin && s.ctrl->rebuild) bin_node<DIM>(s, i);
+  const DevParticle& p = s.P[s.ptid[i]];
+  if (!p.compute_force) {
+    for (int d = 0; d < DIM; d++) s.f[d][i] = 0.;
+    return;
+  }
+  if (p.material_type != LAW) return;
+  const double horizon = p.horizon, h = p.h;
+  const double check_up = horizon + 0.5 * h, check_low = horizon - 0.5 * h;
+  const double cc = p.mat[0], s0 = p.mat[1];
+  const double Sc = LAW == 2 ? s0 : DBL_MAX;
+  const double Kn = p.Kn, Rc = p.Rc;
+  const int64_t sl = i / WAVE, lane = i % WAVE;
+  const int* __restrict__ c = s.col + s.sptr[sl] + lane;
+  unsigned* bw = s.bits + s.bptr[sl] + lane;
+  const int n = s.deg[i];
+  double xi[3], ui[3];
+  for (int d = 0; d < DIM; d++) {
+    xi[d] = s.xr[d][i];
+    ui[d] = s.u[d][i];
+  }
+  double force[3] = {0., 0., 0.};
+  float Zi = 0.f;
+  unsigned word = 0, wnew = 0;
+  for (int k = 0; k < n; k++) {
+    if ((k & 31) == 0) {
+      if (k > 0 && wnew != word) bw[(int64_t)((k >> 5) - 1) * WAVE] = wnew;
+      word = bw[(int64_t)(k >> 5) * WAVE];
+      wnew = word;
+    }
+    const int j = c[(int64_t)k * WAVE];
+    double dx[3], du[3], dy[3];
+    for (int d = 0; d < DIM; d++) {
+      dx[d] = s.xr[d][j] - xi[d];
+      du[d] = s.u[d][j] - ui[d];
+      dy[d] = dx[d] + du[d];
+    }
+    double volj = s.vol[j];
+    bool fs = (word >> (k & 31)) & 1u;
+    const double rji = len3<DIM>(dx);
+    const double Sji = (len3<DIM>(dy) - len3<DIM>(dx)) / len3<DIM>(dx);
+    if (!fs) {
+      if (isGreater(rji, check_low)) volj *= (check_up - rji) / h;
+      const double J = infJ<INF>(s, 0., rji, horizon);
+      if (LAW == 2 && isGreater(fabs(Sji), s0 + 1.0e-10)) fs = true;
+      const double ef = fs ? 0. : J * cc * Sji;
+      if (fs) wnew |= 1u << (k & 31);
+      const double scalar_f = ef * volj;
+      const double ly = len3<DIM>(dy);
+      for (int d = 0; d < DIM; d++) force[d] += scalar_f * (dy[d] / ly);
+    } else {
+      double y[3];
+      for (int d = 0; d < DIM; d++) y[d] = (s.xr[d][j] + s.u[d][j]) - (xi[d] + ui[d]);
       const double R = len3<DIM>(y);
       double scalar_f = Kn * volj * (R - Rc) / R;
       if (scalar_f > 0.) scalar_f = 0.;

@@ -12,7 +12,7 @@ written for the reference run unmodified:
   Particle_Generation  From_File (loc_rad / loc_rad_orient CSV) or one particle
                    per zone from the zone's own mesh
   Mesh             Zone_k.File, Mesh_Size
-  Material         Zone_k: Type (RNPBond / PDState), E K G Poisson_Ratio Gc KIc,
+  Material         Zone_k: Type (RNPBond / PMBBond / PDElasticBond / PDState), E K G Poisson_Ratio Gc KIc,
                    Density, Horizon / Horizon_Mesh_Ratio, Influence_Function,
                    Bond_Potential.{Check_Sc_Factor, Irreversible_Bond_Fracture},
                    Compute_From_Classical, Copy_Material_Data
@@ -208,8 +208,9 @@ _FFN = {"constant": capi.FFN_CONSTANT, "linear": capi.FFN_LINEAR, "linear_step":
 
 def _material(e: dict) -> MaterialDeck:
     t = e.get("Type")
-    if t not in ("RNPBond", "PDState"):
-        raise ValueError(f"material type {t}: the step supports RNPBond and PDState")
+    if t not in ("RNPBond", "PMBBond", "PDElasticBond", "PDState"):
+        raise ValueError(f"material type {t}: the step supports RNPBond, PMBBond, PDElasticBond and PDState "
+                         "(baseParticle.cpp:156-166)")
     if "Density" not in e:
         raise ValueError("material density missing")
     m = MaterialDeck(type=t, density=float(e["Density"]))

@@ -14,7 +14,7 @@ import math
 from dataclasses import dataclass, field
 from typing import List, Optional
 
-from .capi import INF_CONST, INF_GAUSSIAN, INF_LINEAR, MAT_PDSTATE, MAT_RNP
+from .capi import INF_CONST, INF_GAUSSIAN, INF_LINEAR, MAT_PDELASTIC, MAT_PDSTATE, MAT_PMB, MAT_RNP
 
 
 @dataclass
@@ -208,12 +208,65 @@ def pdstate_params(deck: MaterialDeck, dim: int, horizon: float) -> MaterialPara
     return MaterialParams(MAT_PDSTATE, [K, G, s0, 0.0, 0.0, 0.0], K, True)
 
 
+def _bond_elastic(deck: MaterialDeck):
+    """E (or K) with nu = 1/4 and lambda, as PmbMaterial / PdElastic
+    computeParameters set them (material.h:925-1000, 1260-1306)."""
+    E, K = deck.E, deck.K
+    if E < 0 and K < 0:
+        raise ValueError("bond-based material needs E or K")
+    nu = 0.25
+    if E > 0.0:
+        K = _toK(E, nu)
+    if K > 0.0 and E < 0.0:
+        E = _toE(K, nu)
+    return E, nu, _toLambdaE(E, nu)
+
+
+def pmb_params(deck: MaterialDeck, dim: int, horizon: float) -> MaterialParams:
+    """PmbMaterial ctor + computeParameters + computeMaterialProperties
+    (material.h:674-716, 846-870, 925-1000)."""
+    if deck.compute_from_classical:
+        E, nu, lam = _bond_elastic(deck)
+        Gc, KIc = deck.Gc, deck.KIc
+        if Gc < 0 and KIc < 0:
+            raise ValueError("PMB needs Gc or KIc")
+        if KIc > 0.0 and Gc < 0.0:
+            Gc = KIc * KIc / E
+        if dim == 2:
+            c = 24.0 * E / (math.pi * math.pow(horizon, 3.0) * (1.0 - nu))
+        else:
+            c = 24.0 * lam / (math.pi * math.pow(horizon, 3.0))
+        s0 = math.sqrt(5.0 * math.pi * Gc / (9.0 * E * horizon))
+    else:
+        c, s0 = deck.bond_potential_params[0], deck.bond_potential_params[1]
+    lam_p = c * (math.pi * math.pow(horizon, 3.0)) / 24.0
+    Kp = _toK(_toELambda(lam_p, 0.25), 0.25)
+    return MaterialParams(MAT_PMB, [c, s0, 0.0, 0.0, 0.0, 0.0], Kp, True)
+
+
+def pdelastic_params(deck: MaterialDeck, dim: int, horizon: float) -> MaterialParams:
+    """PdElastic ctor + computeParameters + computeMaterialProperties
+    (material.h:1034-1076, 1189-1213, 1260-1306)."""
+    if deck.compute_from_classical:
+        E, nu, lam = _bond_elastic(deck)
+        c = 24.0 * lam / (math.pi * math.pow(horizon, 3.0))
+    else:
+        c = deck.bond_potential_params[0]
+    lam_p = c * (math.pi * math.pow(horizon, 3.0)) / 24.0
+    Kp = _toK(_toELambda(lam_p, 0.25), 0.25)
+    return MaterialParams(MAT_PDELASTIC, [c, 0.0, 0.0, 0.0, 0.0, 0.0], Kp, False)
+
+
 def material_params(deck: MaterialDeck, dim: int, horizon: float, inf: InfluenceFn) -> MaterialParams:
     if deck.type == "RNPBond":
         return rnp_params(deck, dim, horizon, inf)
     if deck.type == "PDState":
         return pdstate_params(deck, dim, horizon)
-    raise ValueError(f"material type {deck.type} is not on the GPU path (RNPBond, PDState)")
+    if deck.type == "PMBBond":
+        return pmb_params(deck, dim, horizon)
+    if deck.type == "PDElasticBond":
+        return pdelastic_params(deck, dim, horizon)
+    raise ValueError(f"material type {deck.type} is not supported (RNPBond, PMBBond, PDElasticBond, PDState)")
 
 
 def particle_contact_constants(K: float, horizon: float, h: float):

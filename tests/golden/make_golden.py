@@ -43,6 +43,9 @@ def lib():
     L.ref_is_greater.argtypes = [D, D]
     L.ref_is_less.argtypes = [D, D]
     L.ref_neighbors.argtypes = [PD, C.c_int64, PI64, PD, C.POINTER(C.c_uint8), C.c_int, PI64, PI64]
+    L.ref_bond_params.argtypes = [C.c_int, D, D, D, D, C.c_int, C.c_int, PD, C.c_int, PD]
+    L.ref_bond_ef.argtypes = [C.c_int, D, D, D, D, C.c_int, C.c_int, PD, C.c_int, D, D, C.c_int, PD, PD, PD]
+    L.ref_bond_ef.restype = C.c_int
     return L
 
 
@@ -94,6 +97,29 @@ def main():
                     rows_b.append([dim, E, K, G, nu, hor, r, s, mx, th, o[0], o[1], f])
     out["pdstate_params"] = np.array(rows_p)
     out["pdstate_bond"] = np.array(rows_b)
+    # ---- PMB (law 2) and PdElastic (law 3) parameters and bond laws
+    rows_p, rows_b = [], []
+    for law in (2, 3):
+        for dim in (2, 3):
+            for inf in (0, 1):
+                for (E, Gc, hor) in ((2.0e9, 500.0, 3.0e-4), (5.0e7, 50.0, 6.0e-4)):
+                    o = np.zeros(4)
+                    L.ref_bond_params(law, E, Gc, 1200.0, hor, dim, inf, p(empty), 0, p(o))
+                    rows_p.append([law, dim, inf, E, Gc, hor] + list(o))
+                    s0 = o[1]
+                    for r_frac in (0.2, 0.7, 0.999):
+                        r = r_frac * hor
+                        for s in (0.0, 1e-6, -2e-4, 0.5 * s0, s0 + 0.5e-10, s0 * 1.00002, -1.3 * s0, 3e-2):
+                            for fs_in in (0, 1):
+                                dx = rng.normal(size=3) * hor * 0.5
+                                du = rng.normal(size=3) * hor * 1e-3
+                                ef = np.zeros(4)
+                                fs = L.ref_bond_ef(law, E, Gc, 1200.0, hor, dim, inf, p(empty), 0, r, s, fs_in,
+                                                   p(dx), p(du), p(ef))
+                                rows_b.append([law, dim, inf, E, Gc, hor, r, s, fs_in, fs, ef[0], ef[1]]
+                                              + list(dx) + list(du) + [ef[2], ef[3]])
+    out["bond_params"] = np.array(rows_p)
+    out["bond_law"] = np.array(rows_b)
     # ---- strains
     rows = []
     for _ in range(40):

@@ -101,3 +101,36 @@ def test_decks_run_on_the_oracle(path):
     o.integrate_cd(20)
     u = o.field(capi.FIELD_U)
     assert np.isfinite(u).all() and np.abs(u).max() > 0
+
+
+def test_bond_based_material_types(tmp_path):
+    """The demo deck with its zones switched to PDElasticBond and PMBBond:
+    material ids and constants (PdElastic c = 24 lambda / (pi delta^3); 2D
+    PMB c = 24 E / (pi delta^3 (1 - nu)), s0 = sqrt(5 pi Gc / (9 E delta)),
+    nu = 1/4, E from K: material.h:925-1000, 1260-1306), and an oracle run."""
+    cfg = yaml.safe_load(open(DEMO))
+    d = os.path.dirname(DEMO)
+    for z in ("Zone_1", "Zone_2"):
+        cfg["Mesh"][z]["File"] = os.path.join(d, os.path.basename(cfg["Mesh"][z]["File"].strip()))
+    cfg["Particle_Generation"]["From_File"] = os.path.join(d, "particle_locations_0.csv")
+    cfg["Material"]["Zone_1"]["Type"] = "PDElasticBond"
+    cfg["Material"]["Zone_2"]["Type"] = "PMBBond"
+    path = tmp_path / "bond.yaml"
+    path.write_text(yaml.safe_dump(cfg))
+    spec = deck.load_deck(str(path))
+    p0, p1 = spec.particles
+    assert p0["material_type"] == capi.MAT_PDELASTIC and p1["material_type"] == capi.MAT_PMB
+    delta, nu = 6e-4, 0.25
+    for p, m in ((p0, cfg["Material"]["Zone_1"]), (p1, cfg["Material"]["Zone_2"])):
+        E = 3.0 * m["K"] * (1.0 - 2.0 * nu)
+        lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+        if p is p0:
+            assert p["mat"][0] == pytest.approx(24.0 * lam / (math.pi * delta ** 3), rel=1e-14)
+        else:
+            assert p["mat"][0] == pytest.approx(24.0 * E / (math.pi * delta ** 3 * (1.0 - nu)), rel=1e-14)
+            assert p["mat"][1] == pytest.approx(math.sqrt(5.0 * math.pi * m["Gc"] / (9.0 * E * delta)), rel=1e-14)
+    o = oracle.OracleEngine(spec, 4)
+    o.integrate_begin()
+    o.integrate_cd(20)
+    u = o.field(capi.FIELD_U)
+    assert np.isfinite(u).all() and np.abs(u).max() > 0

@@ -1,5 +1,7 @@
 """FP64 parity on the code paths the configuration workloads leave out:
 3D bond-based RNP, the Gaussian influence function, reversible RNP bonds,
+bond-based PMB (with breaking) and PdElastic in 2D and 3D, a PDState and a
+PMB particle in contact (FP64 and mixed precision),
 every displacement time function (constant, linear, quadratic, sin,
 rotation) and force time function (constant, linear, linear_step,
 linear_slow_fast, sin) on region selections, and the capacity error path.
@@ -14,7 +16,7 @@ import pytest
 from peridem_amd import capi, configs
 from peridem_amd.deck import _Region
 from peridem_amd.geometry import ball_mesh, center_node, disk_mesh
-from peridem_amd.model import BcDeck, ModelBuilder, ZoneDeck
+from peridem_amd.model import BcDeck, ContactPairDeck, ModelBuilder, ZoneDeck
 import oracle
 
 from parity_util import compare_state
@@ -68,10 +70,48 @@ def _cases():
         fbc=[BcDeck([], [2], capi.FFN_LINEAR_SLOW_FAST, [1e9, 2e-7, 1.0, 4.0],
                     region=_box(2, (-0.2 * R, -0.2 * R), (0.2 * R, 0.2 * R))),
              BcDeck([], [1], capi.FFN_CONSTANT, [1e8], region=_box(2, (-0.2 * R, 0.5 * R), (0.2 * R, R)))])
+    # 2D PMB pulled fast on the left cap: bonds break near the pull
+    mp = configs.pmb_material(Gc=1.0)
+    out["pmb2d_break"] = _single(
+        2, mp,
+        ubc=[BcDeck([], [1], capi.UFN_LINEAR, [-20.0], region=_box(2, (-R, -R), (-0.7 * R, R))),
+             BcDeck([], [1, 2], capi.UFN_CONSTANT, [0.0], region=_box(2, (0.7 * R, -R), (R, R)), is_zero=True)])
+    # 3D PMB, Gaussian influence, sin displacement and sin force
+    mg = configs.pmb_material(Gc=20.0, inf=capi.INF_GAUSSIAN)
+    out["pmb3d_gauss_sin"] = _single(
+        3, mg,
+        ubc=[BcDeck([], [3], capi.UFN_SIN, [4e-6, 3e4], region=_box(3, (-R, -R, -R), (R, R, -0.7 * R)))],
+        fbc=[BcDeck([], [1], capi.FFN_SIN, [1e9, 3e4], region=_box(3, (-R, -R, 0.7 * R), (R, R, R)))])
+    # 2D and 3D PdElastic (never breaks), linear influence, fast pull
+    me = configs.pdelastic_material(inf=capi.INF_LINEAR)
+    out["pdelastic2d_linear"] = _single(
+        2, me,
+        ubc=[BcDeck([], [1], capi.UFN_LINEAR, [-20.0], region=_box(2, (-R, -R), (-0.7 * R, R)))],
+        fbc=[BcDeck([], [2], capi.FFN_CONSTANT, [1e9], region=_box(2, (0.6 * R, -R), (R, R)))])
+    out["pdelastic3d_const"] = _single(
+        3, configs.pdelastic_material(), gravity=(0.0, 0.0, -9.81),
+        ubc=[BcDeck([], [1, 2, 3], capi.UFN_LINEAR, [-10.0], region=_box(3, (-R, -R, -R), (R, R, -0.7 * R)))])
     return out
 
 
 CASES = _cases()
+
+
+def _state_pmb_pair(speed=3.0):
+    """A PDState disk (zone 0, first particle) hits a PMB disk (zone 1)."""
+    R, h = 1.0e-3, 1.0e-4
+    ms = configs.pdstate_material()
+    mp = configs.pmb_material(Gc=50.0)
+    cp = ContactPairDeck(contact_radius_factor=0.95, Kn=18.0 * ms.K / (math.pi * (3 * h) ** 5), Kn_factor=1e-3,
+                         damping_on=True, friction_on=True, friction_coeff=0.3, epsilon=0.8)
+    b = ModelBuilder(2, 2e-8, [ZoneDeck(ms), ZoneDeck(mp)], {(0, 0): cp, (0, 1): cp, (1, 1): cp},
+                     search_factor=5.0, neigh_update_interval=20)
+    x, vol = disk_mesh(R, h)
+    cl = center_node(x, (0, 0, 0), R)
+    b.add_particle(0, x, vol, cl, R)
+    b.add_particle(1, x + np.array([0.0, 2 * R, 0.0]), vol, cl, R)
+    b.ic.append(([1], (0.0, -speed, 0.0)))
+    return b.build(name="state_pmb_pair_2d")
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -88,6 +128,35 @@ def test_path_parity(name):
         out = compare_state(g, r, tag=f"{name} block {k}")
         print(f"[paths] {name} block {k}: {out}", flush=True)
     assert np.abs(r.field(capi.FIELD_F)).max() > 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_state_and_pmb_particles_in_contact(precision):
+    """Two laws in one model: the PDState particle runs the state kernels
+    (FP32 tiles in mixed precision), the PMB particle the FP64 bond kernel."""
+    spec = _state_pmb_pair()
+    prec = capi.PREC_FP64 if precision == "fp64" else capi.PREC_MIXED
+    print(f"[paths] state+pmb {precision} nodes={spec.n_nodes}", flush=True)
+    g = capi.GpuEngine(spec, prec)
+    r = oracle.OracleEngine(spec, 8)
+    g.integrate_begin()
+    r.integrate_begin()
+    g.integrate_cd(600)
+    r.integrate_cd(600)
+    if precision == "fp64":
+        out = compare_state(g, r, tag="state+pmb")
+        print(f"[paths] state+pmb fp64: {out}", flush=True)
+    else:
+        f, fr = g.field(capi.FIELD_F), r.field(capi.FIELD_F)
+        n1 = spec.particles[1]["glob_start"]
+        err = float(np.abs(f - fr).max() / np.abs(fr).max())
+        err_pmb = float(np.abs(f[n1:] - fr[n1:]).max() / np.abs(fr[n1:]).max())
+        print(f"[paths] state+pmb mixed: force err {err:.3e}, PMB particle {err_pmb:.3e}", flush=True)
+        assert err < 1e-4 and err_pmb < 1e-4
+    # the particles touched: contact forces acted on both
+    v = r.field(capi.FIELD_V)
+    cn = [p["center_node"] for p in spec.particles]
+    assert v[cn[0], 1] < 0.0 and v[cn[1], 1] > -3.0
 
 
 def test_capacity_overflow_is_an_error():

@@ -10,11 +10,12 @@ import numpy as np
 import pytest
 
 from peridem_amd import capi
-from peridem_amd.material import InfluenceFn, MaterialDeck, pdstate_params, rnp_params
+from peridem_amd.material import (InfluenceFn, MaterialDeck, pdelastic_params, pdstate_params, pmb_params,
+                                  rnp_params)
 from peridem_amd.model import ContactPairDeck, ModelBuilder, ZoneDeck
 import oracle
 
-from oracle_api import lib, rnp_bond_f
+from oracle_api import bond_f, lib, rnp_bond_f
 
 G = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.npz"))
 
@@ -78,6 +79,39 @@ def test_pdstate_parameters_and_law_match_reference():
         J = float(dim + 1)
         fo = L.oracle_pdstate_bond_f(J, rr, s, K, G_, mx, th)
         assert fo == f_ref, (fo, f_ref)
+
+
+def test_pmb_and_pdelastic_match_reference():
+    """PmbMaterial / PdElastic: parameters from E and Gc, the bond law with
+    fracture (PMB) or without (PdElastic), the actual stretch and Sc."""
+    L = lib()
+    for row in G["bond_params"]:
+        law, dim, inf, E, Gc, hor = int(row[0]), int(row[1]), int(row[2]), row[3], row[4], row[5]
+        c_ref, s0_ref, K_ref, J5 = row[6:10]
+        deck = MaterialDeck(type="PMBBond" if law == 2 else "PDElasticBond", E=E, Gc=Gc, density=1200.0,
+                            influence_fn_type=inf)
+        mp = (pmb_params if law == 2 else pdelastic_params)(deck, dim, hor)
+        f = InfluenceFn(inf, [], dim)
+        assert mp.material_type == law
+        assert ulp_close(mp.mat[0], c_ref, 4), (law, dim, inf)
+        if law == 2:
+            assert ulp_close(mp.mat[1], s0_ref, 0)
+        assert ulp_close(mp.K_for_contact, K_ref, 4)
+        assert ulp_close(f(0.5), J5, 0)
+        sel = G["bond_law"]
+        sel = sel[(sel[:, 0] == law) & (sel[:, 1] == dim) & (sel[:, 2] == inf) & (sel[:, 3] == E)]
+        assert len(sel) > 0
+        for r in sel:
+            rr, s_, fs_in, fs_ref, f_ref = r[6], r[7], int(r[8]), int(r[9]), r[11]
+            fs, fo = bond_f(L, law, f(rr / hor), s_, mp.mat[0], mp.mat[1], fs_in)
+            assert fs == fs_ref, (law, rr, s_, fs_in)
+            assert ulp_close(fo, f_ref, 2), (law, rr, s_, fo, f_ref)
+            dx, du = r[12:15], r[15:18]
+            dy = dx + du
+            ly = math.sqrt(dy[0] * dy[0] + dy[1] * dy[1] + dy[2] * dy[2])
+            lx = math.sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2])
+            assert (ly - lx) / lx == r[18]
+            assert r[19] == (mp.mat[1] if law == 2 else np.finfo(np.float64).max)
 
 
 def test_strain_measures_match_reference():
