@@ -207,6 +207,13 @@ int pdem_compute_external_displacement_bc(pdem_ctx *ctx);
  * Same results as pdem_integrate_cd; fewer launches. */
 int pdem_integrate_cd_graph(pdem_ctx *ctx, int64_t nsteps);
 
+/* nsteps x DEMModel::integrateVerlet (demModel.cpp:392-460), the
+ * "velocity_verlet" time scheme: v += dt/2 f/rho and drift, n++, t += dt,
+ * computeExternalDisplacementBC(), computeForces(), v += dt/2 f/rho. */
+int pdem_integrate_verlet(pdem_ctx *ctx, int64_t nsteps);
+/* The same through a captured step (replaces a captured integrateCD step). */
+int pdem_integrate_verlet_graph(pdem_ctx *ctx, int64_t nsteps);
+
 int pdem_synchronize(pdem_ctx *ctx);
 
 int pdem_get_field(pdem_ctx *ctx, int32_t field, void *host);

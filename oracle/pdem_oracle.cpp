@@ -7,7 +7,8 @@
  *
  * It follows the reference's per-node loops line by line in FP64 (built with
  * -ffp-contract=off, the x86-64 -O3 arithmetic of the reference):
- *   node update            src/model/dem/demModel.cpp:349-390
+ *   node update            src/model/dem/demModel.cpp:349-390 (central difference),
+ *                          :392-460 (velocity Verlet)
  *   computeForces          demModel.cpp:462-600
  *   PD forces              demModel.cpp:602-789; RNP material.h:342-368,405-417,
  *                          PMB :732-793, PdElastic :1090-1136, PdState :1415-1451
@@ -787,6 +788,43 @@ void integrateCD(oracle_ctx *c) {
   computeForces(c);
 }
 
+// integrateVerlet, demModel.cpp:392-460: half kick and drift, n++, t += dt,
+// displacement BC, forces, second half kick.  vMag is set after each kick.
+void integrateVerlet(oracle_ctx *c) {
+  c->currentDt = c->dt;
+  const int dim = c->dim;
+  c->pool->run(c->n, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; i++) {
+      if (!c->computeForceNode(i)) continue;
+      const double rho = c->part(i).density;
+      const uint8_t fx = c->fix[i];
+      for (int dof = 0; dof < dim; dof++) {
+        if (!(fx >> dof & 1)) {
+          double vv = getc(c->v[i], dof) + 0.5 * (c->currentDt / rho) * getc(c->f[i], dof);
+          setc(c->v[i], dof, vv);
+          setc(c->u[i], dof, getc(c->u[i], dof) + c->currentDt * vv);
+          setc(c->x[i], dof, getc(c->x[i], dof) + c->currentDt * vv);
+        }
+      }
+      c->vmag[i] = len(c->v[i]);
+    }
+  });
+  c->step++;
+  c->time += c->currentDt;
+  computeExternalDisplacementBC(c);
+  computeForces(c);
+  c->pool->run(c->n, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; i++) {
+      if (!c->computeForceNode(i)) continue;
+      const double rho = c->part(i).density;
+      const uint8_t fx = c->fix[i];
+      for (int dof = 0; dof < dim; dof++)
+        if (!(fx >> dof & 1)) setc(c->v[i], dof, getc(c->v[i], dof) + 0.5 * (c->currentDt / rho) * getc(c->f[i], dof));
+      c->vmag[i] = len(c->v[i]);
+    }
+  });
+}
+
 }  // namespace
 
 // ======================================================================= ABI
@@ -892,6 +930,11 @@ int oracle_integrate_begin(oracle_ctx *c) {
 
 int oracle_integrate_cd(oracle_ctx *c, int64_t nsteps) {
   for (int64_t s = 0; s < nsteps; s++) integrateCD(c);
+  return 0;
+}
+
+int oracle_integrate_verlet(oracle_ctx *c, int64_t nsteps) {
+  for (int64_t s = 0; s < nsteps; s++) integrateVerlet(c);
   return 0;
 }
 

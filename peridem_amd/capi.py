@@ -185,6 +185,7 @@ def declare(lib: C.CDLL, prefix: str, extra_create_int: bool = False):
     for name in ("integrate_begin", "compute_forces", "compute_peridynamic_forces"):
         f(name, C.c_int, [vp])
     f("integrate_cd", C.c_int, [vp, C.c_int64])
+    f("integrate_verlet", C.c_int, [vp, C.c_int64])
     f("get_field", C.c_int, [vp, C.c_int32, vp])
     f("set_field", C.c_int, [vp, C.c_int32, vp])
     f("get_time", C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_double)])
@@ -239,6 +240,17 @@ class Engine:
 
     def integrate_cd(self, nsteps=1):
         self._check(self._f("integrate_cd")(self.ctx, nsteps), "integrate_cd")
+
+    def integrate_verlet(self, nsteps=1):
+        self._check(self._f("integrate_verlet")(self.ctx, nsteps), "integrate_verlet")
+
+    def integrate(self, nsteps=1):
+        """DEMModel::integrateStep (demModel.cpp:342-347) nsteps times: the
+        model's time scheme, central difference or velocity Verlet."""
+        if self.spec.time_scheme == "velocity_verlet":
+            self.integrate_verlet(nsteps)
+        else:
+            self.integrate_cd(nsteps)
 
     def compute_forces(self):
         self._check(self._f("compute_forces")(self.ctx), "compute_forces")
@@ -319,6 +331,8 @@ def engine_library():
         declare(lib, "pdem_")
         lib.pdem_integrate_cd_graph.restype = C.c_int
         lib.pdem_integrate_cd_graph.argtypes = [C.c_void_p, C.c_int64]
+        lib.pdem_integrate_verlet_graph.restype = C.c_int
+        lib.pdem_integrate_verlet_graph.argtypes = [C.c_void_p, C.c_int64]
         lib.pdem_synchronize.restype = C.c_int
         lib.pdem_synchronize.argtypes = [C.c_void_p]
         for nm in ("update_contact_neighborlist", "compute_contact_forces",
@@ -372,6 +386,16 @@ class GpuEngine(Engine):
 
     def integrate_cd_graph(self, nsteps):
         self._check(self.lib.pdem_integrate_cd_graph(self.ctx, nsteps), "integrate_cd_graph")
+
+    def integrate_verlet_graph(self, nsteps):
+        self._check(self.lib.pdem_integrate_verlet_graph(self.ctx, nsteps), "integrate_verlet_graph")
+
+    def integrate_graph(self, nsteps):
+        """integrate() through the captured step of the model's time scheme."""
+        if self.spec.time_scheme == "velocity_verlet":
+            self.integrate_verlet_graph(nsteps)
+        else:
+            self.integrate_cd_graph(nsteps)
 
     def synchronize(self):
         self._check(self.lib.pdem_synchronize(self.ctx), "synchronize")

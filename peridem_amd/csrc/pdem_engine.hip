@@ -59,6 +59,7 @@ struct pdem_ctx {
   ClosureArgs ca{};
   const pdem_mlp* cl_net = nullptr;
   bool graph_ready = false;
+  bool graph_verlet = false;  // the captured step is integrateVerlet
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   bool profiling = false;
@@ -280,11 +281,15 @@ struct Launch {
     if (c->multi) hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, c->stream, c->ds);
     forces_after_ctrl(c);
   }
-  static void integrate_step(pdem_ctx* c) {
-    hipLaunchKernelGGL(k_integrate<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds, c->multi);
+  // integrateCD, or integrateVerlet when `verlet` (half kick and drift,
+  // forces, second half kick)
+  static void integrate_step(pdem_ctx* c, bool verlet = false) {
+    hipLaunchKernelGGL(k_integrate<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds, c->multi,
+                       verlet ? 1 : 0);
     displacement_bc(c);
     if (c->multi) hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, c->stream, c->ds);
     forces_after_ctrl(c);
+    if (verlet) hipLaunchKernelGGL(k_kick<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds);
   }
 };
 
@@ -757,21 +762,25 @@ int guarded(pdem_ctx* c, F f) {
   }
 }
 
-void ensure_graph(pdem_ctx* c) {
-  if (c->graph_ready) return;
+void drop_graph(pdem_ctx* c);
+
+void ensure_graph(pdem_ctx* c, bool verlet) {
+  if (c->graph_ready && c->graph_verlet == verlet) return;
+  drop_graph(c);
   // zero-displacement sets are applied once, before any capture
   bool pending = false;
   for (size_t s = 0; s < c->ubc.size(); s++)
     if (c->ubc_zero[s] && !c->zero_done[s]) pending = true;
-  if (pending) throw HipError{"call pdem_integrate_begin before pdem_integrate_cd_graph"};
+  if (pending) throw HipError{"call pdem_integrate_begin before replaying the captured step"};
   bool prof = c->profiling;
   c->profiling = false;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  dispatch(c, [&] { Launch<2>::integrate_step(c); }, [&] { Launch<3>::integrate_step(c); });
+  dispatch(c, [&] { Launch<2>::integrate_step(c, verlet); }, [&] { Launch<3>::integrate_step(c, verlet); });
   HIPCHK(hipStreamEndCapture(c->stream, &c->graph));
   c->profiling = prof;
   HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
   c->graph_ready = true;
+  c->graph_verlet = verlet;
 }
 
 }  // namespace
@@ -870,7 +879,24 @@ int pdem_integrate_cd(pdem_ctx* c, int64_t nsteps) {
 
 int pdem_integrate_cd_graph(pdem_ctx* c, int64_t nsteps) {
   return guarded(c, [&] {
-    ensure_graph(c);
+    ensure_graph(c, false);
+    for (int64_t k = 0; k < nsteps; k++) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+    if (c->multi) check_overflow(c);
+  });
+}
+
+int pdem_integrate_verlet(pdem_ctx* c, int64_t nsteps) {
+  return guarded(c, [&] {
+    for (int64_t k = 0; k < nsteps; k++)
+      dispatch(c, [&] { Launch<2>::integrate_step(c, true); }, [&] { Launch<3>::integrate_step(c, true); });
+    check_launch(c);
+    if (c->multi) check_overflow(c);
+  });
+}
+
+int pdem_integrate_verlet_graph(pdem_ctx* c, int64_t nsteps) {
+  return guarded(c, [&] {
+    ensure_graph(c, true);
     for (int64_t k = 0; k < nsteps; k++) HIPCHK(hipGraphLaunch(c->gexec, c->stream));
     if (c->multi) check_overflow(c);
   });

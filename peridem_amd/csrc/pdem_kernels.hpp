@@ -403,9 +403,11 @@ __device__ __forceinline__ void clear_cells(const DS& s, int64_t first, int64_t 
 __global__ void k_cells_init(DS s) { clear_cells(s, gid(), (int64_t)gridDim.x * blockDim.x); }
 
 // ------------------------------------------------------------ node update
-// integrateCD (demModel.cpp:349-390): semi-implicit Euler on free dofs.
+// integrateCD (demModel.cpp:349-390): semi-implicit Euler on free dofs;
+// with `half`, the first half kick and drift of integrateVerlet
+// (demModel.cpp:392-425): v += 0.5 (dt / rho) f.
 template <int DIM>
-__global__ void __launch_bounds__(256) k_integrate(DS s, int multi) {
+__global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
   int64_t i = gid();
   Ctrl* ct = s.ctrl;
   const double dt = ct->dt;
@@ -414,12 +416,13 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi) {
     const DevParticle& p = s.P[s.ptid[i]];
     if (p.compute_force) {
       const double rho = p.rho;
+      const double a = half ? 0.5 * (dt / rho) : dt / rho;
       const uint8_t fx = s.fix[i];
       double vv[3] = {0., 0., 0.};
       for (int d = 0; d < DIM; d++) {
         double vd = s.v[d][i];
         if (!(fx >> d & 1)) {
-          vd += (dt / rho) * s.f[d][i];
+          vd += a * s.f[d][i];
           s.v[d][i] = vd;
           s.u[d][i] += dt * vd;
           s.x[d][i] += dt * vd;
@@ -460,6 +463,28 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi) {
       ct->vdirty = 1;
     }
   }
+}
+
+// second half kick of integrateVerlet (demModel.cpp:437-459) with the new
+// forces: v += 0.5 (dt / rho) f on free dofs, |v|.
+template <int DIM>
+__global__ void __launch_bounds__(256) k_kick(DS s) {
+  int64_t i = gid();
+  if (i >= s.N) return;
+  const DevParticle& p = s.P[s.ptid[i]];
+  if (!p.compute_force) return;
+  const double a = 0.5 * (s.ctrl->dt / p.rho);
+  const uint8_t fx = s.fix[i];
+  double vv[3] = {0., 0., 0.};
+  for (int d = 0; d < DIM; d++) {
+    double vd = s.v[d][i];
+    if (!(fx >> d & 1)) {
+      vd += a * s.f[d][i];
+      s.v[d][i] = vd;
+    }
+    vv[d] = vd;
+  }
+  s.vmag[i] = len3<DIM>(vv);
 }
 
 // ---------------------------------------------------------- boundary data

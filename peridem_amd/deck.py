@@ -5,7 +5,8 @@ src/inp/input.cpp:189-1395) plus its mesh files (gmsh .msh 2.x ASCII or
 .csv `id,x,y[,z],volume`) becomes the ModelSpec the engine takes, so decks
 written for the reference run unmodified:
 
-  Model            Dimension, Final_Time / Time_Steps -> dt, Particle_Sim_Type
+  Model            Dimension, Final_Time / Time_Steps -> dt, Particle_Sim_Type,
+                   Discretization_Type.Time (central_difference / velocity_verlet)
   Zone             Zones, Zone_k.Is_Wall
   Particle         Zone_k geometry (Type, Parameters), All_Dofs_Constrained,
                    Test_Name, Compressive_Test.{Particle_Id, Particle_Force_Direction}
@@ -386,6 +387,12 @@ def load_deck(path: str, name: Optional[str] = None) -> ModelSpec:
     spec = b.build(name=name or os.path.splitext(os.path.basename(path))[0], n_steps=int(model["Time_Steps"]),
                    wall_id=wall_id, f_dir=f_dir, h_override=hover)
     spec.info.update(deck=os.path.abspath(path), test_name=pblock.get("Test_Name", ""))
+    # Model.Discretization_Type.Time (input.cpp:205-217); integrateStep
+    # (demModel.cpp:342-347) steps nothing for any other name, so those are errors here
+    scheme = (model.get("Discretization_Type") or {}).get("Time", "central_difference")
+    if scheme not in ("central_difference", "velocity_verlet"):
+        raise ValueError(f"time discretization {scheme}: central_difference or velocity_verlet")
+    spec.time_scheme = scheme
     return spec
 
 

@@ -126,6 +126,7 @@ class ModelSpec:
     h_min: float = 0.0
     max_contact_r: float = 0.0
     info: dict = field(default_factory=dict)
+    time_scheme: str = "central_difference"   # or "velocity_verlet"
 
     @property
     def n_particles(self):

@@ -1,0 +1,126 @@
+"""The velocity Verlet time scheme (DEMModel::integrateVerlet,
+demModel.cpp:392-460) on the GPU against the FP64 oracle.
+
+Bar: the FP64 parity bar of the central-difference tests (fields within
+1e-12 of their largest entry, bond states identical, damage within its 1e-5
+slack) on a single PMB particle that breaks bonds, the two-particle
+collision and a compressive test with walls, damping and friction; the
+captured step equal to the eager one bit for bit, also after a switch from
+a captured central-difference step; mixed precision within the FP32 bond
+arithmetic bar on the 3D packing; and a reference deck with
+Discretization_Type.Time velocity_verlet run through integrate()."""
+import os
+
+import numpy as np
+import pytest
+import yaml
+
+from peridem_amd import capi, configs, deck
+from peridem_amd.geometry import center_node, disk_mesh
+from peridem_amd.model import BcDeck, ModelBuilder, ZoneDeck
+from peridem_amd.deck import _Region
+import oracle
+
+from parity_util import compare_state
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SINGLE = os.path.join(HERE, "golden", "decks", "single_particle_circle", "input_0.yaml")
+
+
+def _pmb_pull():
+    R, h = 1e-3, 1e-4
+    x, vol = disk_mesh(R, h)
+    b = ModelBuilder(2, 2e-8, [ZoneDeck(configs.pmb_material(Gc=1.0))], {}, multi_particle=False)
+    b.add_particle(0, x, vol, center_node(x, (0, 0, 0), R), R)
+    b.ubc.append(BcDeck([], [1], capi.UFN_LINEAR, [-20.0],
+                        region=_Region("rectangle", [-R, -R, 0.0, -0.7 * R, R, 0.0], 2)))
+    return b.build(name="pmb_pull_2d")
+
+
+def _cases():
+    return {
+        "pmb_pull_2d": (_pmb_pull(), 3, 100),
+        "two_particle": (configs.two_particle(gap=0.0, speed=0.5), 3, 40),
+        "compressive_small": (configs.compressive_2d(n_particles=12, nodes_per_particle=150, seed=3,
+                                                     init_speed=1.0, pitch_factor=1.01, R_lo=1.2e-3,
+                                                     R_hi=1.2e-3), 3, 200),
+    }
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_verlet_parity(name):
+    spec, chunks, steps = CASES[name]
+    print(f"[verlet] {name} nodes={spec.n_nodes} chunks={chunks}x{steps}", flush=True)
+    g = capi.GpuEngine(spec, capi.PREC_FP64)
+    r = oracle.OracleEngine(spec, 8)
+    g.integrate_begin()
+    r.integrate_begin()
+    for k in range(chunks):
+        g.integrate_verlet(steps)
+        r.integrate_verlet(steps)
+        out = compare_state(g, r, tag=f"{name} chunk {k}")
+        print(f"[verlet] {name} chunk {k}: {out}", flush=True)
+    np.testing.assert_allclose(g.field(capi.FIELD_VMAG), r.field(capi.FIELD_VMAG), rtol=0, atol=1e-12 *
+                               max(np.abs(r.field(capi.FIELD_VMAG)).max(), 1e-300))
+    assert g.time()[0] == r.time()[0] == chunks * steps
+    assert np.abs(r.field(capi.FIELD_F)).max() > 0
+
+
+def test_verlet_graph_matches_eager_after_cd_graph():
+    spec = CASES["compressive_small"][0]
+    a = capi.GpuEngine(spec, capi.PREC_FP64)
+    b = capi.GpuEngine(spec, capi.PREC_FP64)
+    for e in (a, b):
+        e.integrate_begin()
+    a.integrate_cd(50)
+    b.integrate_cd_graph(50)
+    a.integrate_verlet(150)
+    b.integrate_verlet_graph(150)
+    for fid in (capi.FIELD_F, capi.FIELD_U, capi.FIELD_V, capi.FIELD_VMAG):
+        np.testing.assert_array_equal(a.field(fid), b.field(fid))
+    assert a.time() == b.time()
+
+
+def test_verlet_mixed_precision():
+    spec = configs.packing_3d(n_spheres=6, nodes_per_sphere=300, L=5.0e-3, seed=2, gap_factor=1.0,
+                              init_speed=2.0)
+    print(f"[verlet] mixed nodes={spec.n_nodes}", flush=True)
+    g = capi.GpuEngine(spec, capi.PREC_MIXED)
+    r = oracle.OracleEngine(spec, 8)
+    g.integrate_begin()
+    r.integrate_begin()
+    g.integrate_verlet_graph(40)
+    r.integrate_verlet(40)
+    f, fr = g.field(capi.FIELD_F), r.field(capi.FIELD_F)
+    d = np.abs(f - fr)
+    emax = float(d.max() / np.abs(fr).max())
+    erms = float(np.sqrt((d ** 2).sum(1).mean()) / np.abs(fr).max())
+    print(f"[verlet] mixed force err max {emax:.3e} rms {erms:.3e}", flush=True)
+    assert emax < 1e-4 and erms < 1e-5
+
+
+def test_verlet_deck(tmp_path):
+    cfg = yaml.safe_load(open(SINGLE))
+    d = os.path.dirname(SINGLE)
+    mk = cfg["Mesh"]["Zone_1"]
+    mk["File"] = os.path.join(d, os.path.basename(mk["File"].strip()))
+    cfg["Model"].setdefault("Discretization_Type", {})["Time"] = "velocity_verlet"
+    path = tmp_path / "verlet.yaml"
+    path.write_text(yaml.safe_dump(cfg))
+    spec = deck.load_deck(str(path))
+    assert spec.time_scheme == "velocity_verlet"
+    print(f"[verlet] deck nodes={spec.n_nodes}", flush=True)
+    g = capi.GpuEngine(spec, capi.PREC_FP64)
+    r = oracle.OracleEngine(spec, 8)
+    g.integrate_begin()
+    r.integrate_begin()
+    g.integrate_graph(100)
+    r.integrate(100)
+    out = compare_state(g, r, tag="verlet deck")
+    print(f"[verlet] deck: {out}", flush=True)
+    assert np.abs(r.field(capi.FIELD_U)).max() > 0

@@ -102,6 +102,60 @@ def test_free_fall_recurrence():
     assert np.all(v[:, 0] == 0.0)
 
 
+def test_free_fall_verlet_is_exact():
+    """Velocity Verlet integrates a constant acceleration exactly:
+    u_n = v0 t + g t^2 / 2, v_n = v0 + g t with t = n dt (central difference
+    gives g dt^2 n (n + 1) / 2 instead)."""
+    g = (0.0, -9.81, 0.0)
+    spec = single(2, False, gravity=g)
+    o = oracle.OracleEngine(spec, 4)
+    v0 = np.zeros((spec.n_nodes, 3))
+    v0[:, 0] = 0.3
+    o.set_field(capi.FIELD_V, v0)
+    o.integrate_begin()
+    n = 50
+    o.integrate_verlet(n)
+    t = n * spec.dt
+    v = o.field(capi.FIELD_V)
+    u = o.field(capi.FIELD_U)
+    np.testing.assert_allclose(v[:, 1], g[1] * t, rtol=1e-12)
+    np.testing.assert_allclose(u[:, 1], 0.5 * g[1] * t * t, rtol=1e-12)
+    np.testing.assert_allclose(u[:, 0], 0.3 * t, rtol=1e-12)
+    np.testing.assert_allclose(o.field(capi.FIELD_VMAG), np.hypot(0.3, g[1] * t), rtol=1e-12)
+    assert o.time() == (n, pytest.approx(t, rel=1e-14))
+
+
+def _reversal_error(step, n=300):
+    """Integrate an elastic particle from a stretched state n steps, negate
+    v, integrate n steps again: distance from the initial state over the
+    displacement range of the run."""
+    x, vol = disk_mesh(1e-3, 1e-4)
+    b = ModelBuilder(2, 1e-8, [ZoneDeck(configs.pdelastic_material())], {}, multi_particle=False)
+    b.add_particle(0, x, vol, center_node(x, (0, 0, 0), 1e-3), 1e-3)
+    spec = b.build()
+    o = oracle.OracleEngine(spec, 4)
+    u0 = np.zeros((spec.n_nodes, 3))
+    u0[:, 0] = 1e-3 * spec.x_ref[:, 0]
+    set_displacement(o, spec, u0)
+    o.integrate_begin()
+    getattr(o, step)(n)
+    u1 = o.field(capi.FIELD_U)
+    o.set_field(capi.FIELD_V, -o.field(capi.FIELD_V))
+    getattr(o, step)(n)
+    u2 = o.field(capi.FIELD_U)
+    return np.abs(u2 - u0).max() / np.abs(u1 - u0).max()
+
+
+def test_verlet_is_time_reversible():
+    """Velocity Verlet is time-symmetric: run forward, flip v, run forward
+    again and the start comes back to rounding; the one-sided central
+    difference step does not come back."""
+    ev = _reversal_error("integrate_verlet")
+    ec = _reversal_error("integrate_cd")
+    assert ev < 1e-9
+    assert ec > 1e-3
+
+
 def _two_particles(friction=True, damping=False, gap=-5e-5, speed=0.5):
     mat = configs.rnp_material()
     cp = ContactPairDeck(contact_radius_factor=0.95, Kn=1.0e22, friction_coeff=0.5,
