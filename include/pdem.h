@@ -232,6 +232,26 @@ int pdem_pp_compressive_test(pdem_ctx *ctx, int32_t wall_id, int32_t f_dir,
 /* number of broken bonds and of all bonds (damage curve) */
 int pdem_broken_bonds(pdem_ctx *ctx, int64_t *broken, int64_t *total);
 
+/* Strain_Stress post-processing (DEMModel::output, demModel.cpp:2026-2072:
+ * fe::getCurrentQuadPoints and fe::getStrainStress, meshUtil.cpp:176-420) at
+ * the quadrature points of every particle's mesh.
+ * pdem_fe_setup: the element-node connectivity of all particle meshes,
+ * particle after particle in mesh order, as global node ids: linear
+ * triangles (dim 2, 3 ids per element) or tetrahedra (dim 3, 4 ids);
+ * elem_particle [n_elem] the owning particle; elastic [n_particles][4] =
+ * {nu, lambda, mu, is_plane_strain} of computeMaterialProperties(dim);
+ * quad_order 1-5 (triangles) or 1-3 (tetrahedra), Quad_Approximation_Order.
+ * Point k of element e is quadrature point e * points_per_element + k.
+ * pdem_strain_stress: current point positions xq [nq][3] and strain, stress
+ * [nq][6] in the order xx yy zz yz xz xy, single precision like the
+ * reference's SymMatrix3, at the current displacement (any output may be
+ * NULL).  The reference's normal strain sums u_i + dN_i/dx_i
+ * (meshUtil.cpp:364-372); the output keeps that. */
+int pdem_fe_setup(pdem_ctx *ctx, int32_t quad_order, int64_t n_elem, const int64_t *elem_nodes,
+                  const int32_t *elem_particle, const double *elastic);
+int pdem_fe_quad_count(pdem_ctx *ctx, int64_t *nq);
+int pdem_strain_stress(pdem_ctx *ctx, double *xq, float *strain, float *stress);
+
 /* Timing of the last pdem_integrate_cd / _graph call's kernels: for the
  * dominant (PD bond) kernel, average duration in ms measured with HIP events
  * on the stream it was launched on. */

@@ -10,7 +10,11 @@
  *   src/material/mparticle/influenceFn.cpp influence functions
  *   src/util/function.cpp                  isGreater / isLess
  *   src/nsearch/nsearch.h + nanoflann      kd-tree radius searches
+ *   src/fe/triElem.cpp, tetElem.cpp,       linear element quadrature data
+ *   baseElem.cpp, src/util/matrix.cpp
  */
+#include "fe/tetElem.h"
+#include "fe/triElem.h"
 #include "material/mparticle/material.h"
 #include "nsearch/nsearch.h"
 
@@ -225,6 +229,51 @@ void ref_neighbors(const double *x, int64_t n, const int64_t *tags, const double
     rowptr[i + 1] = rowptr[i] + (int64_t)keep.size();
     if (cols) std::memcpy(cols + rowptr[i], keep.data(), keep.size() * sizeof(int64_t));
   }
+}
+
+/* computeMaterialProperties(dim) of a material: out = nu, lambda, mu.
+ * law 0 RNPBond, 1 PDState, 2 PMBBond, 3 PDElasticBond. */
+void ref_material_properties(int law, double E, double K, double G, double nu, double Gc, double rho,
+                             double horizon, int dim, int inf_type, const double *inf_params, int n_inf,
+                             double *out) {
+  static const char *names[4] = {"RNPBond", "PDState", "PMBBond", "PDElasticBond"};
+  auto deck = makeDeck(names[law], E, K, G, nu, Gc, rho, inf_type, inf_params, n_inf);
+  inp::MatData md;
+  if (law == 0) md = material::RnpMaterial(deck, (size_t)dim, horizon).computeMaterialProperties((size_t)dim);
+  if (law == 1) md = material::PdState(deck, (size_t)dim, horizon).computeMaterialProperties((size_t)dim);
+  if (law == 2) md = material::PmbMaterial(deck, (size_t)dim, horizon).computeMaterialProperties((size_t)dim);
+  if (law == 3) md = material::PdElastic(deck, (size_t)dim, horizon).computeMaterialProperties((size_t)dim);
+  out[0] = md.d_nu;
+  out[1] = md.d_lambda;
+  out[2] = md.d_mu;
+}
+
+/* TriElem (dim 2) / TetElem (dim 3) getQuadDatas on one element with
+ * vertices nodes [dim + 1][3]: returns the number of points; per point the
+ * mapped position pts [3], shape values [dim + 1] and physical shape
+ * gradients [dim + 1][dim]. */
+int ref_fe_quad_datas(int dim, int order, const double *nodes, double *pts, double *shapes, double *ders) {
+  std::vector<util::Point> nds;
+  for (int k = 0; k <= dim; k++) nds.emplace_back(nodes[3 * k], nodes[3 * k + 1], nodes[3 * k + 2]);
+  std::vector<fe::QuadData> qds;
+  if (dim == 2) {
+    fe::TriElem e((size_t)order);
+    qds = e.getQuadDatas(nds);
+  } else {
+    fe::TetElem e((size_t)order);
+    qds = e.getQuadDatas(nds);
+  }
+  const int nv = dim + 1;
+  for (size_t q = 0; q < qds.size(); q++) {
+    pts[3 * q] = qds[q].d_p.d_x;
+    pts[3 * q + 1] = qds[q].d_p.d_y;
+    pts[3 * q + 2] = qds[q].d_p.d_z;
+    for (int i = 0; i < nv; i++) {
+      shapes[q * nv + i] = qds[q].d_shapes[i];
+      for (int d = 0; d < dim; d++) ders[(q * nv + i) * dim + d] = qds[q].d_derShapes[i][d];
+    }
+  }
+  return (int)qds.size();
 }
 
 }  // extern "C"

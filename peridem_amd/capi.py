@@ -333,6 +333,12 @@ def engine_library():
         lib.pdem_integrate_cd_graph.argtypes = [C.c_void_p, C.c_int64]
         lib.pdem_integrate_verlet_graph.restype = C.c_int
         lib.pdem_integrate_verlet_graph.argtypes = [C.c_void_p, C.c_int64]
+        lib.pdem_fe_setup.restype = C.c_int
+        lib.pdem_fe_setup.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.pdem_fe_quad_count.restype = C.c_int
+        lib.pdem_fe_quad_count.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+        lib.pdem_strain_stress.restype = C.c_int
+        lib.pdem_strain_stress.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.pdem_synchronize.restype = C.c_int
         lib.pdem_synchronize.argtypes = [C.c_void_p]
         for nm in ("update_contact_neighborlist", "compute_contact_forces",
@@ -386,6 +392,28 @@ class GpuEngine(Engine):
 
     def integrate_cd_graph(self, nsteps):
         self._check(self.lib.pdem_integrate_cd_graph(self.ctx, nsteps), "integrate_cd_graph")
+
+    def fe_setup(self, quad_order, elem_nodes, elem_particle, elastic):
+        """Element data for strain_stress (pdem_fe_setup)."""
+        en = np.ascontiguousarray(elem_nodes, np.int64)
+        ep = np.ascontiguousarray(elem_particle, np.int32)
+        el = np.ascontiguousarray(elastic, np.float64)
+        self._check(self.lib.pdem_fe_setup(self.ctx, int(quad_order), len(en), en.ctypes.data_as(C.c_void_p),
+                                           ep.ctypes.data_as(C.c_void_p), el.ctypes.data_as(C.c_void_p)),
+                    "fe_setup")
+
+    def strain_stress(self):
+        """Current quadrature points [nq][3], strain and stress [nq][6]
+        (xx yy zz yz xz xy, float32) at the current displacement."""
+        n = C.c_int64()
+        self._check(self.lib.pdem_fe_quad_count(self.ctx, C.byref(n)), "fe_quad_count")
+        xq = np.zeros((n.value, 3))
+        eps = np.zeros((n.value, 6), np.float32)
+        sig = np.zeros((n.value, 6), np.float32)
+        self._check(self.lib.pdem_strain_stress(self.ctx, xq.ctypes.data_as(C.c_void_p),
+                                                eps.ctypes.data_as(C.c_void_p), sig.ctypes.data_as(C.c_void_p)),
+                    "strain_stress")
+        return xq, eps, sig
 
     def integrate_verlet_graph(self, nsteps):
         self._check(self.lib.pdem_integrate_verlet_graph(self.ctx, nsteps), "integrate_verlet_graph")

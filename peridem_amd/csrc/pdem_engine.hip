@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -21,6 +22,7 @@
 #include "pdem_kernels.hpp"
 #include "pdem_mixed.hpp"
 #include "pdem_closure.hpp"
+#include "pdem_fe.hpp"
 
 using namespace pdem;
 
@@ -60,6 +62,9 @@ struct pdem_ctx {
   const pdem_mlp* cl_net = nullptr;
   bool graph_ready = false;
   bool graph_verlet = false;  // the captured step is integrateVerlet
+  // strain / stress post-processing (pdem_fe_setup)
+  FeArgs fe{};
+  std::vector<void*> fe_bufs;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   bool profiling = false;
@@ -1138,6 +1143,119 @@ int pdem_broken_bonds(pdem_ctx* c, int64_t* broken, int64_t* total) {
     Ctrl h = read_ctrl(c);
     *broken = (int64_t)h.broken;
     *total = c->nnz;
+  });
+}
+
+// Reference-element quadrature points of TriElem::init (triElem.cpp:192-417)
+// and TetElem::init (tetElem.cpp:300-409); the shape values at a point are
+// N = {1 - xi - eta [- zeta], xi, eta[, zeta]} (triElem.cpp:119-123,
+// tetElem.cpp:180-184).
+static std::vector<std::array<double, 3>> fe_points(int dim, int order) {
+  using P = std::array<double, 3>;
+  if (dim == 2) {
+    switch (order) {
+      case 1: return {P{1. / 3., 1. / 3., 0.}};
+      case 2: return {P{1. / 6., 1. / 6., 0.}, P{2. / 3., 1. / 6., 0.}, P{1. / 6., 2. / 3., 0.}};
+      case 3:
+        return {P{1. / 3., 1. / 3., 0.}, P{1. / 5., 3. / 5., 0.}, P{1. / 5., 1. / 5., 0.}, P{3. / 5., 1. / 5., 0.}};
+      case 4: {
+        const double a = 0.44594849091597, b = 0.10810301816807, c = 0.09157621350977, d = 0.81684757298046;
+        return {P{a, a, 0.}, P{a, b, 0.}, P{b, a, 0.}, P{c, c, 0.}, P{c, d, 0.}, P{d, c, 0.}};
+      }
+      case 5: {
+        const double a = 0.33333333333333, b = 0.47014206410511, c = 0.05971587178977, d = 0.10128650732346,
+                     f = 0.79742698535309;
+        return {P{a, a, 0.}, P{b, b, 0.}, P{b, c, 0.}, P{c, b, 0.}, P{d, d, 0.}, P{d, f, 0.}, P{f, d, 0.}};
+      }
+    }
+  } else {
+    switch (order) {
+      case 1: return {P{1. / 4., 1. / 4., 1. / 4.}};
+      case 2: {
+        const double a = 0.585410196624969, b = 0.138196601125011;
+        return {P{a, b, b}, P{b, a, b}, P{b, b, a}, P{b, b, b}};
+      }
+      case 3: {
+        const double a = 0.25, b = 0.5, c = 1. / 6.;
+        return {P{a, a, a}, P{b, c, c}, P{c, b, c}, P{c, c, b}, P{c, c, c}};
+      }
+    }
+  }
+  throw HipError{"quadrature order must be 1-5 (triangles) or 1-3 (tetrahedra)"};
+}
+
+int pdem_fe_setup(pdem_ctx* c, int32_t quad_order, int64_t n_elem, const int64_t* elem_nodes,
+                  const int32_t* elem_particle, const double* elastic) {
+  return guarded(c, [&] {
+    if (n_elem <= 0 || !elem_nodes || !elem_particle || !elastic) throw HipError{"empty element list"};
+    const int dim = c->dim, nv = dim + 1;
+    const auto pts = fe_points(dim, quad_order);
+    const int nq = (int)pts.size();
+    if (n_elem > (int64_t)1 << 40) throw HipError{"too many elements"};
+    for (int64_t e = 0; e < n_elem; e++) {
+      const int32_t p = elem_particle[e];
+      if (p < 0 || p >= c->np) throw HipError{"element particle id out of range"};
+      for (int k = 0; k < nv; k++) {
+        const int64_t g = elem_nodes[e * nv + k];
+        if (g < c->hp[p].glob_start || g >= c->hp[p].glob_end)
+          throw HipError{"element node outside its particle's node range"};
+      }
+    }
+    std::vector<double> qpt(3 * nq), qsh((size_t)nv * nq);
+    for (int q = 0; q < nq; q++) {
+      const auto& x = pts[q];
+      for (int d = 0; d < 3; d++) qpt[3 * q + d] = x[d];
+      if (dim == 2) {
+        qsh[3 * q + 0] = 1. - x[0] - x[1];
+        qsh[3 * q + 1] = x[0];
+        qsh[3 * q + 2] = x[1];
+      } else {
+        qsh[4 * q + 0] = 1. - x[0] - x[1] - x[2];
+        qsh[4 * q + 1] = x[0];
+        qsh[4 * q + 2] = x[1];
+        qsh[4 * q + 3] = x[2];
+      }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (void* b : c->fe_bufs) c->dfree(b);
+    c->fe_bufs.clear();
+    auto keep = [&](auto* ptr) {
+      c->fe_bufs.push_back((void*)ptr);
+      return ptr;
+    };
+    FeArgs a{};
+    a.n_elem = n_elem;
+    a.nq = nq;
+    a.nodes = keep(c->upload(elem_nodes, (size_t)n_elem * nv));
+    a.part = keep(c->upload((const int*)elem_particle, (size_t)n_elem));
+    a.elastic = keep(c->upload(elastic, (size_t)4 * c->np));
+    a.qpt = keep(c->upload(qpt.data(), qpt.size()));
+    a.qshape = keep(c->upload(qsh.data(), qsh.size()));
+    a.xq = keep(c->dalloc<double>((size_t)n_elem * nq * 3));
+    a.strain = keep(c->dalloc<float>((size_t)n_elem * nq * 6));
+    a.stress = keep(c->dalloc<float>((size_t)n_elem * nq * 6));
+    c->fe = a;
+  });
+}
+
+int pdem_fe_quad_count(pdem_ctx* c, int64_t* nq) {
+  return guarded(c, [&] { *nq = c->fe.n_elem * c->fe.nq; });
+}
+
+int pdem_strain_stress(pdem_ctx* c, double* xq, float* strain, float* stress) {
+  return guarded(c, [&] {
+    const FeArgs& a = c->fe;
+    if (a.n_elem == 0) throw HipError{"call pdem_fe_setup first"};
+    const unsigned nb = blocks(a.n_elem);
+    dispatch(
+        c, [&] { hipLaunchKernelGGL(k_strain_stress<2>, dim3(nb), dim3(TPB), 0, c->stream, c->ds, a); },
+        [&] { hipLaunchKernelGGL(k_strain_stress<3>, dim3(nb), dim3(TPB), 0, c->stream, c->ds, a); });
+    check_launch(c);
+    const size_t nq = (size_t)a.n_elem * a.nq;
+    if (xq) HIPCHK(hipMemcpyAsync(xq, a.xq, nq * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (strain) HIPCHK(hipMemcpyAsync(strain, a.strain, nq * 6 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (stress) HIPCHK(hipMemcpyAsync(stress, a.stress, nq * 6 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
   });
 }
 

@@ -58,9 +58,10 @@ def _is_greater(a, b):
 
 
 # --------------------------------------------------------------- meshes
-def read_msh(path: str, dim: int) -> Tuple[np.ndarray, np.ndarray]:
+def read_msh(path: str, dim: int, with_cells: bool = False):
     """gmsh 2.x ASCII mesh (rw/mshReader.cpp:22-230): nodes and finite-element
-    nodal volumes of the triangle / quadrangle (2D) or tetrahedron (3D) cells."""
+    nodal volumes of the triangle / quadrangle (2D) or tetrahedron (3D) cells;
+    with `with_cells`, also the cells (0-based node ids, file order)."""
     with open(path) as fh:
         tok = fh.read().split("\n")
     nodes = None
@@ -99,7 +100,9 @@ def read_msh(path: str, dim: int) -> Tuple[np.ndarray, np.ndarray]:
             i += 1
     if nodes is None or not cells:
         raise ValueError(f"{path}: no nodes or no supported cells")
-    return nodes, fe_nodal_volumes(nodes, np.array(cells, dtype=np.int64), dim)
+    enc = np.array(cells, dtype=np.int64)
+    vol = fe_nodal_volumes(nodes, enc, dim)
+    return (nodes, vol, enc) if with_cells else (nodes, vol)
 
 
 _GP = 1.0 / math.sqrt(3.0)
@@ -150,11 +153,12 @@ def read_csv_mesh(path: str, dim: int) -> Tuple[np.ndarray, np.ndarray]:
 
 
 def read_mesh(path: str, dim: int):
+    """Nodes, nodal volumes and cells (None for node files) of a mesh file."""
     ext = os.path.splitext(path)[1].lower()
     if ext == ".msh":
-        return read_msh(path, dim)
+        return read_msh(path, dim, with_cells=True)
     if ext == ".csv":
-        return read_csv_mesh(path, dim)
+        return read_csv_mesh(path, dim) + (None,)
     raise ValueError(f"{path}: only .msh and .csv meshes are read (.vtu needs VTK)")
 
 
@@ -221,6 +225,8 @@ def _material(e: dict) -> MaterialDeck:
             setattr(m, attr, float(e[key]))
     if "Compute_From_Classical" in e:
         m.compute_from_classical = bool(e["Compute_From_Classical"])
+    if "Is_Plane_Strain" in e:
+        m.plane_strain = bool(e["Is_Plane_Strain"])
     inf = e.get("Influence_Function") or {}
     if "Type" in inf:
         m.influence_fn_type = int(inf["Type"])
@@ -299,8 +305,8 @@ def load_deck(path: str, name: Optional[str] = None) -> ModelSpec:
         mk = meshblock.get(zk) or meshblock
         if "File" not in mk:
             raise ValueError(f"{zk}: mesh file missing (mesh generation is not part of the reader)")
-        x, vol = read_mesh(_resolve(str(mk["File"]).strip(), deck_dir), dim)
-        zone_mesh.append((x, vol, float(mk["Mesh_Size"]) if "Mesh_Size" in mk else None))
+        x, vol, cells = read_mesh(_resolve(str(mk["File"]).strip(), deck_dir), dim)
+        zone_mesh.append((x, vol, float(mk["Mesh_Size"]) if "Mesh_Size" in mk else None, cells))
 
     contact: Dict[Tuple[int, int], ContactPairDeck] = {}
     cblock = cfg.get("Contact") or {}
@@ -329,7 +335,7 @@ def load_deck(path: str, name: Optional[str] = None) -> ModelSpec:
     for z in range(nz):
         gtype, prm = zone_geom[z]
         center, R = _geom_center_radius(gtype, prm)
-        x_ref, vol_ref, hdeck = zone_mesh[z]
+        x_ref, vol_ref, hdeck, cells = zone_mesh[z]
         cl = center_node(x_ref, center, R)
         from_file = "From_File" in gen and not zones[z].is_wall and multi
         pz = pblock.get(f"Zone_{z + 1}") or {}
@@ -344,10 +350,10 @@ def load_deck(path: str, name: Optional[str] = None) -> ModelSpec:
                 c, sn = math.cos(th), math.sin(th)
                 rot = np.array([[c, -sn, 0.0], [sn, c, 0.0], [0.0, 0.0, 1.0]])
                 x = np.asarray(site)[None, :] + s * (x_ref @ rot.T)
-                b.add_particle(z, x, vol_ref * s ** dim, cl, rad)
+                b.add_particle(z, x, vol_ref * s ** dim, cl, rad, cells=cells)
                 h_over.append(hdeck * s if hdeck else None)
         else:
-            b.add_particle(z, x_ref.copy(), vol_ref.copy(), cl, R)
+            b.add_particle(z, x_ref.copy(), vol_ref.copy(), cl, R, cells=cells)
             h_over.append(hdeck)
     if pblock.get("Test_Name", "") == "two_particle" and b.parts:
         b.force_off.append(0)
@@ -393,6 +399,8 @@ def load_deck(path: str, name: Optional[str] = None) -> ModelSpec:
     if scheme not in ("central_difference", "velocity_verlet"):
         raise ValueError(f"time discretization {scheme}: central_difference or velocity_verlet")
     spec.time_scheme = scheme
+    # strain / stress post-processing: quadrature order (modelDeck.h:107, input.cpp:222-223)
+    spec.info["quad_order"] = int(model.get("Quad_Approximation_Order", 1))
     return spec
 
 

@@ -21,7 +21,7 @@ from .capi import INF_CONST, INF_GAUSSIAN, INF_LINEAR, MAT_PDELASTIC, MAT_PDSTAT
 class MaterialDeck:
     """Fields of inp::MaterialDeck (src/inp/decks/materialDeck.h:230-301) that the
     time step depends on.  Negative elastic constants mean "not given"."""
-    type: str = "RNPBond"            # RNPBond | PDState
+    type: str = "RNPBond"            # RNPBond | PMBBond | PDElasticBond | PDState
     E: float = -1.0
     K: float = -1.0
     G: float = -1.0
@@ -37,6 +37,7 @@ class MaterialDeck:
     irreversible: bool = True
     compute_from_classical: bool = True
     bond_potential_params: List[float] = field(default_factory=list)
+    plane_strain: bool = False       # Is_Plane_Strain (strain / stress output only)
 
 
 class InfluenceFn:
@@ -267,6 +268,24 @@ def material_params(deck: MaterialDeck, dim: int, horizon: float, inf: Influence
     if deck.type == "PDElasticBond":
         return pdelastic_params(deck, dim, horizon)
     raise ValueError(f"material type {deck.type} is not supported (RNPBond, PMBBond, PDElasticBond, PDState)")
+
+
+def elastic_constants(mp: MaterialParams, dim: int, horizon: float, inf: InfluenceFn):
+    """(nu, lambda, mu) of computeMaterialProperties(dim): RNP
+    (material.h:461-489), PMB (:846-870), PdElastic (:1189-1213), PdState
+    (:1504-1530).  Used by the strain / stress post-processing."""
+    if mp.material_type == MAT_RNP:
+        Cc, beta = mp.mat[0], mp.mat[1]
+        M = inf.moment(dim)
+        lam = Cc * M * beta / 4.0 if dim == 2 else Cc * M * beta / 5.0
+        return 0.25, lam, lam
+    if mp.material_type in (MAT_PMB, MAT_PDELASTIC):
+        lam = mp.mat[0] * (math.pi * math.pow(horizon, 3.0)) / 24.0
+        return 0.25, lam, lam
+    K, G = mp.mat[0], mp.mat[1]
+    nu = (3.0 * K - 2.0 * G) / (2.0 * (3.0 * K + G))
+    E = _toE(K, nu)
+    return nu, _toLambdaE(E, nu), G
 
 
 def particle_contact_constants(K: float, horizon: float, h: float):

@@ -20,7 +20,7 @@ import numpy as np
 
 from . import capi
 from .geometry import mesh_size
-from .material import InfluenceFn, MaterialDeck, material_params, particle_contact_constants
+from .material import InfluenceFn, MaterialDeck, elastic_constants, material_params, particle_contact_constants
 
 
 @dataclass
@@ -160,11 +160,15 @@ class ModelBuilder:
         self.fbc: List[BcDeck] = []
         self.ic = []             # (particle ids, velocity)
         self.force_off: List[int] = []
+        self.cells: List[Optional[np.ndarray]] = []
 
     def add_particle(self, zone: int, x: np.ndarray, vol: np.ndarray, center_local: int,
-                     bounding_radius: float) -> int:
+                     bounding_radius: float, cells: Optional[np.ndarray] = None) -> int:
+        """cells: the particle mesh's element-node connectivity (local node
+        ids), kept for the strain / stress post-processing."""
         self.parts.append((zone, np.asarray(x, float), np.asarray(vol, float), int(center_local),
                            float(bounding_radius)))
+        self.cells.append(None if cells is None else np.asarray(cells, np.int64))
         return len(self.parts) - 1
 
     def build(self, name="", n_steps=0, wall_id=-1, f_dir=1, h_override=None) -> ModelSpec:
@@ -197,7 +201,9 @@ class ModelBuilder:
                 all_dofs_constrained=int(zd.all_dofs_constrained), compute_force=int(compute_force),
                 material_type=mp.material_type, irreversible=int(mp.irreversible),
                 density=float(md.density), horizon=float(horizon), mesh_size=float(h),
-                bounding_radius=br, Kn=float(Kn), Rc=float(Rc), mat=list(mp.mat)))
+                bounding_radius=br, Kn=float(Kn), Rc=float(Rc), mat=list(mp.mat),
+                elastic=list(elastic_constants(mp, dim, horizon, inf)), plane_strain=int(md.plane_strain),
+                fe_cells=self.cells[pid] if pid < len(self.cells) else None))
         # DEMModel::setupContact (demModel.cpp:1531-1600)
         nz = len(self.zones)
         h_min = h_max = 0.0

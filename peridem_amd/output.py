@@ -7,6 +7,8 @@
   Fixity, Particle_ID, Zone_ID, Nodal_Volume, Damage_Z and Theta one.  The
   file is the XML UnstructuredGrid format (ASCII, 17 significant digits), so
   ParaView and VTK read it as they read the reference's output.
+* Strain and stress at the mesh quadrature points (the Strain_Stress tag,
+  VtkParticleWriter::appendStrainStress, rw/vtkParticleWriter.cpp:444-495).
 * The compressive-test curve as CSV (t, wall_penetration, tot_reaction_force):
   the values DEMModel::ppCompressiveTest computes (demModel.cpp:2202-2250),
   written as numbers (the reference's format string writes literal text).
@@ -94,6 +96,29 @@ def read_vtu(path):
     pts = arr(piece.find("Points/DataArray"))
     data = {el.get("Name"): arr(el) for el in piece.find("PointData")}
     return pts, data
+
+
+def write_strain_vtu(path, xq, strain, stress):
+    """The Strain_Stress output (VtkParticleWriter::appendStrainStress,
+    rw/vtkParticleWriter.cpp:444-495): quadrature points as points, Strain
+    and Stress with 6 components (xx yy zz yz xz xy)."""
+    xq = np.asarray(xq)
+    n = len(xq)
+    with open(path, "w") as fh:
+        fh.write('<?xml version="1.0"?>\n<VTKFile type="UnstructuredGrid" version="0.1" '
+                 'byte_order="LittleEndian">\n<UnstructuredGrid>\n'
+                 f'<Piece NumberOfPoints="{n}" NumberOfCells="0">\n<PointData>\n')
+        names = ' '.join(f'ComponentName{i}="{c}"' for i, c in enumerate(("xx", "yy", "zz", "yz", "xz", "xy")))
+        for tag, a in (("Strain", strain), ("Stress", stress)):
+            body = " ".join(f"{float(v):.9g}" for v in np.asarray(a).reshape(-1))
+            fh.write(f'<DataArray type="Float64" Name="{tag}" NumberOfComponents="6" {names} format="ascii">'
+                     + body + "</DataArray>\n")
+        fh.write("</PointData>\n<Points>\n")
+        fh.write(_array("Points", xq, 3))
+        fh.write('</Points>\n<Cells>\n<DataArray type="Int64" Name="connectivity" format="ascii"></DataArray>\n'
+                 '<DataArray type="Int64" Name="offsets" format="ascii"></DataArray>\n'
+                 '<DataArray type="UInt8" Name="types" format="ascii"></DataArray>\n</Cells>\n'
+                 "</Piece>\n</UnstructuredGrid>\n</VTKFile>\n")
 
 
 def write_compressive_csv(path, rows):

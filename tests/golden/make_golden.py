@@ -3,8 +3,9 @@
 Run in the container that holds the reference tree, after
 `make -C oracle ref` has compiled oracle/_ref/libref.so from the reference's
 material laws (src/material/mparticle/material.h, influenceFn.cpp), compare
-functions (src/util/function.cpp) and nanoflann radius searches
-(src/nsearch/nsearch.h).  The file holds inputs and the reference's outputs
+functions (src/util/function.cpp), nanoflann radius searches
+(src/nsearch/nsearch.h) and linear finite elements (src/fe/triElem.cpp,
+tetElem.cpp).  The file holds inputs and the reference's outputs
 only; the tests compare the oracle against it.
 """
 import ctypes as C
@@ -46,6 +47,9 @@ def lib():
     L.ref_bond_params.argtypes = [C.c_int, D, D, D, D, C.c_int, C.c_int, PD, C.c_int, PD]
     L.ref_bond_ef.argtypes = [C.c_int, D, D, D, D, C.c_int, C.c_int, PD, C.c_int, D, D, C.c_int, PD, PD, PD]
     L.ref_bond_ef.restype = C.c_int
+    L.ref_material_properties.argtypes = [C.c_int, D, D, D, D, D, D, D, C.c_int, C.c_int, PD, C.c_int, PD]
+    L.ref_fe_quad_datas.argtypes = [C.c_int, C.c_int, PD, PD, PD, PD]
+    L.ref_fe_quad_datas.restype = C.c_int
     return L
 
 
@@ -174,6 +178,46 @@ def main():
         L.ref_neighbors(p(pts), len(pts), tags.ctypes.data_as(PI64), p(rad), None, 0,
                         rp2.ctypes.data_as(PI64), cols2.ctypes.data_as(PI64))
         out[f"nbr{dim}_exc_rp"], out[f"nbr{dim}_exc_cols"] = rp2, cols2[: rp2[-1]]
+    # computeMaterialProperties(dim): law, E, K, G, nu, Gc, horizon, dim, inf -> nu, lambda, mu
+    rows = []
+    inf_p = np.zeros(2)
+    for law in range(4):
+        for dim in (2, 3):
+            for inf in (0, 1, 2):
+                for _ in range(3):
+                    E = 10 ** rng.uniform(6, 11)
+                    if law == 1:
+                        K, G, Ein = E / 1.2, E / 2.6, -1.0
+                    else:
+                        K, G, Ein = -1.0, -1.0, E
+                    Gc = 10 ** rng.uniform(0, 3)
+                    hz = 10 ** rng.uniform(-4, -2)
+                    o = np.zeros(3)
+                    L.ref_material_properties(law, Ein, K, G, -1.0, Gc, 1000.0, hz, dim, inf, p(inf_p), 0, p(o))
+                    rows.append([law, Ein, K, G, Gc, hz, dim, inf, *o])
+    out["mat_props"] = np.array(rows)
+    # linear element quadrature data: vertices -> mapped points, shapes, gradients
+    for dim, orders in ((2, range(1, 6)), (3, range(1, 4))):
+        nv = dim + 1
+        verts = []
+        for _ in range(6):
+            base = rng.uniform(-1e-3, 1e-3, 3)
+            V = base + rng.uniform(-2e-4, 2e-4, (nv, 3))
+            if dim == 2:
+                V[:, 2] = 0.0
+            verts.append(V)
+        out[f"fe{dim}_nodes"] = np.array(verts)
+        for order in orders:
+            P, S, G = [], [], []
+            for V in verts:
+                pts, sh, de = np.zeros(3 * 8), np.zeros(8 * nv), np.zeros(8 * nv * dim)
+                nq = L.ref_fe_quad_datas(dim, order, p(np.ascontiguousarray(V)), p(pts), p(sh), p(de))
+                P.append(pts[:3 * nq].reshape(nq, 3))
+                S.append(sh[:nq * nv].reshape(nq, nv))
+                G.append(de[:nq * nv * dim].reshape(nq, nv, dim))
+            out[f"fe{dim}_o{order}_pts"] = np.array(P)
+            out[f"fe{dim}_o{order}_shapes"] = np.array(S)
+            out[f"fe{dim}_o{order}_ders"] = np.array(G)
     np.savez_compressed(os.path.join(HERE, "ref_vectors.npz"), **out)
     print({k: v.shape for k, v in out.items()})
 

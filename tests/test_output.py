@@ -33,3 +33,18 @@ def test_compressive_csv(tmp_path):
     output.write_compressive_csv(str(path), rows)
     back = np.loadtxt(str(path), delimiter=",", skiprows=1)
     np.testing.assert_array_equal(back, np.array(rows))
+
+
+def test_strain_vtu_round_trip(tmp_path):
+    """Strain_Stress output: points and the 6-component arrays read back
+    exactly (float32 values written with 9 significant digits)."""
+    rng = np.random.default_rng(0)
+    xq = rng.normal(size=(17, 3))
+    eps = rng.normal(size=(17, 6)).astype(np.float32)
+    sig = (rng.normal(size=(17, 6)) * 1e6).astype(np.float32)
+    path = tmp_path / "output_strain_0.vtu"
+    output.write_strain_vtu(str(path), xq, eps, sig)
+    pts, data = output.read_vtu(str(path))
+    np.testing.assert_array_equal(pts, xq)
+    np.testing.assert_array_equal(data["Strain"].astype(np.float32), eps)
+    np.testing.assert_array_equal(data["Stress"].astype(np.float32), sig)
