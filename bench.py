@@ -53,6 +53,11 @@ def pd_bytes_per_launch(spec, eng, precision):
         state-based force pass    24 d + 28 + 4.125 N_i  (adds m_x, theta; bonds read)
       this build:
         FP64 RNP kernel           24 d + 16 + 4.25 N_i   (adds the 4-byte degree)
+        FP64 RNP pair-tile kernel 24 d + 16 + 8.25 N_i   (adds the 2-byte pair slot of
+                                                           each bond and the 4-byte pair
+                                                           list entry of each pair; the
+                                                           per-pair scratch it writes and
+                                                           reads back in L2 is not counted)
         mixed state force kernel  24 d + 24 + 2.125 N_i  (2-byte ids; alpha, b terms
                                                            FP32 instead of m_x, theta)
     """
@@ -70,7 +75,8 @@ def pd_bytes_per_launch(spec, eng, precision):
     if precision == capi.PREC_MIXED:
         layout = np.where(st, 24 * d + 24 + 2.125 * nb, 24 * d + 16 + 4.25 * nb)
     else:
-        layout = np.where(st, 24 * d + 32 + 4.125 * nb, 24 * d + 16 + 4.25 * nb)
+        rnp = 24 * d + 16 + (8.25 if eng.launch_config()[0] == 512 else 4.25) * nb
+        layout = np.where(st, 24 * d + 32 + 4.125 * nb, rnp)
     return float(layout.sum()), float(canon.sum()), float(nb.mean()) if len(nb) else 0.0
 
 
@@ -246,7 +252,8 @@ def main():
     achieved_canon = bytes_canon / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     state = any(p["material_type"] == capi.MAT_PDSTATE for p in spec.particles)
     mixed = prec == capi.PREC_MIXED and state
-    kname = "k_mixed_force" if mixed else ("k_pd_state" if state else "k_pd_rnp")
+    kname = "k_mixed_force" if mixed else ("k_pd_state" if state else
+                                           ("k_pd_rnp_tile" if eng.launch_config()[0] == 512 else "k_pd_rnp"))
     ktag = kname[2:] + "_" + spec.name
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(ktag),

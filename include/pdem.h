@@ -261,11 +261,16 @@ int pdem_profile_get(pdem_ctx *ctx, double *pd_kernel_ms_avg, int64_t *pd_kernel
 /* Launch-shape autotuner for the peridynamic force pass. No counterpart in
  * the reference (its OpenMP loops have no launch shape). Times `reps` force
  * passes on the current state per candidate (FP64 kernels: workgroup size
- * 64/128/256; mixed tile kernels: workgroup 256/1024 x tile 256/512/1024
+ * 64/128/256 of the lane-per-node kernel, and for models whose
+ * force-computing particles are all RNP the pair-tile kernel, reported as
+ * pd_tpb 512; mixed tile kernels: workgroup 256/1024 x tile 256/512/1024
  * nodes), keeps the fastest and re-captures the step graph. Results do not
- * depend on the choice: forces are written per node, bond bits per lane. */
+ * depend on the choice: every kernel sums each node's bonds in list order.
+ * pdem_set_launch_config applies a shape directly (e.g. one the tuner chose
+ * in an earlier run). */
 int pdem_autotune(pdem_ctx *ctx, int32_t reps, double *best_ms);
 int pdem_get_launch_config(pdem_ctx *ctx, int32_t *pd_tpb, int32_t *mixed_tpb, int32_t *tile_nodes);
+int pdem_set_launch_config(pdem_ctx *ctx, int32_t pd_tpb, int32_t mixed_tpb, int32_t tile_nodes);
 
 /* ---------------------------------------------------------------------
  * Learned pair-contact closure network.  No counterpart in the reference:

@@ -375,6 +375,8 @@ def engine_library():
         lib.pdem_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
         lib.pdem_get_launch_config.restype = C.c_int
         lib.pdem_get_launch_config.argtypes = [C.c_void_p] + [C.POINTER(C.c_int32)] * 3
+        lib.pdem_set_launch_config.restype = C.c_int
+        lib.pdem_set_launch_config.argtypes = [C.c_void_p] + [C.c_int32] * 3
         _engine_lib = lib
     return _engine_lib
 
@@ -460,6 +462,10 @@ class GpuEngine(Engine):
         v = [C.c_int32() for _ in range(3)]
         self._check(self.lib.pdem_get_launch_config(self.ctx, *[C.byref(x) for x in v]), "launch_config")
         return tuple(x.value for x in v)
+
+    def set_launch_config(self, pd_tpb, mixed_tpb=1024, tile_nodes=1024):
+        """Apply a launch shape (pd_tpb 512: the RNP pair-tile kernel)."""
+        self._check(self.lib.pdem_set_launch_config(self.ctx, pd_tpb, mixed_tpb, tile_nodes), "set_launch_config")
 
     def closure_setup(self, codes=None):
         """Closure data; codes: one float per particle (shape/material class)."""

@@ -23,6 +23,7 @@
 #include "pdem_mixed.hpp"
 #include "pdem_closure.hpp"
 #include "pdem_fe.hpp"
+#include "pdem_rnp_tile.hpp"
 
 using namespace pdem;
 
@@ -62,6 +63,9 @@ struct pdem_ctx {
   const pdem_mlp* cl_net = nullptr;
   bool graph_ready = false;
   bool graph_verlet = false;  // the captured step is integrateVerlet
+  // bond-based RNP pair-tile kernel (FP64, all force-computing particles RNP)
+  RnpTileArgs rt{};
+  bool rnp_tile = false;  // use it (set up by default, autotuned)
   // strain / stress post-processing (pdem_fe_setup)
   FeArgs fe{};
   std::vector<void*> fe_bufs;
@@ -181,7 +185,11 @@ struct Launch {
       bin = 0;
     }
     auto ev = ev_begin(c);
-    if (c->rnp_any || !(c->state_any || c->pmb_any || c->elastic_any)) {
+    if (c->rnp_tile) {
+      hipLaunchKernelGGL((k_pd_rnp_tile<DIM, INF>), dim3(c->rt.ntiles), dim3(RT_THREADS), c->rt.lds_bytes, st,
+                         c->ds, c->rt, bin);
+      bin = 0;
+    } else if (c->rnp_any || !(c->state_any || c->pmb_any || c->elastic_any)) {
       hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(pnb), dim3(pb), 0, st, c->ds, bin);
       bin = 0;
     }
@@ -336,6 +344,126 @@ void check_overflow(pdem_ctx* c) {
   if (h.overflow == 5)
     throw HipError{"closure pair list capacity exceeded (" + std::to_string(c->ds.cpair_cap) + " pairs)"};
   if (h.overflow == 6) throw HipError{"closure: damping partner lists are not symmetric"};
+}
+
+// Pair tiles of the RNP kernel: each force-computing particle is cut into
+// chunks of at most RT_THREADS consecutive nodes whose in-tile pairs fit
+// RT_MAX_PAIRS; other particles get force-free chunks (zero and bin only).
+// Per tile: its pairs (i < j, both inside) in slot order, node i's upper
+// neighbors being a contiguous range of its ascending list; per bond: the
+// pair slot of in-tile bonds (the partner's slot for lower neighbors).
+void build_rnp_tiles(pdem_ctx* c) {
+  const int64_t N = c->N;
+  if (N > INT_MAX / 4) return;
+  std::vector<int> col((size_t)c->nnz_padded);
+  HIPCHK(hipMemcpyAsync(col.data(), c->ds.col, col.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  auto at = [&](int64_t i, int k) { return c->h_sptr[i / WAVE] + (int64_t)k * WAVE + i % WAVE; };
+  std::vector<uint16_t> ps((size_t)c->nnz_padded, RT_OUT);
+  std::vector<uint32_t> pl;
+  std::vector<RtTile> tiles;
+  // upper in-tile range [kb, ke) of node i in a tile ending at t1
+  auto range = [&](int64_t i, int64_t t1, int& kb, int& ke) {
+    const int n = c->h_deg[i];
+    kb = 0;
+    while (kb < n && col[at(i, kb)] <= i) kb++;
+    ke = kb;
+    while (ke < n && col[at(i, ke)] < t1) ke++;
+  };
+  auto pairs = [&](int64_t t0, int64_t t1) {
+    int64_t np = 0;
+    for (int64_t i = t0; i < t1; i++) {
+      int kb, ke;
+      range(i, t1, kb, ke);
+      np += ke - kb;
+    }
+    return np;
+  };
+  std::vector<std::pair<int64_t, int64_t>> work;
+  for (const auto& q : c->hp) {
+    const bool w = q.compute_force && q.material_type == PDEM_MAT_RNP;
+    const int64_t n = q.glob_end - q.glob_start;
+    const int64_t nt = (n + RT_THREADS - 1) / RT_THREADS;
+    const int64_t chunk = (n + nt - 1) / nt;
+    for (int64_t t0 = q.glob_start; t0 < q.glob_end; t0 += chunk) {
+      const int64_t t1 = std::min<int64_t>(t0 + chunk, q.glob_end);
+      if (!w) {
+        tiles.push_back(RtTile{(int)t0, (int)t1, 0, 0, 0});
+        continue;
+      }
+      // halve until the pairs fit
+      std::vector<std::pair<int64_t, int64_t>> st{{t0, t1}};
+      while (!st.empty()) {
+        auto r = st.back();
+        st.pop_back();
+        if (r.second - r.first > 1 && pairs(r.first, r.second) > RT_MAX_PAIRS) {
+          const int64_t m = (r.first + r.second) / 2;
+          st.push_back({m, r.second});
+          st.push_back({r.first, m});
+          continue;
+        }
+        work.push_back(r);
+      }
+    }
+  }
+  size_t lds = 0;
+  std::vector<int> kb_(RT_THREADS), ke_(RT_THREADS), base_(RT_THREADS);
+  for (const auto& r : work) {
+    const int64_t t0 = r.first, t1 = r.second;
+    const int64_t pbase = (int64_t)pl.size();
+    int base = 0;
+    for (int64_t i = t0; i < t1; i++) {
+      int kb, ke;
+      range(i, t1, kb, ke);
+      kb_[i - t0] = kb;
+      ke_[i - t0] = ke;
+      base_[i - t0] = base;
+      for (int k = kb; k < ke; k++) {
+        ps[at(i, k)] = (uint16_t)(base + k - kb);
+        pl.push_back((uint32_t)(i - t0) | ((uint32_t)(col[at(i, k)] - t0) << 16));
+      }
+      base += ke - kb;
+    }
+    if (base > RT_MAX_PAIRS) return;  // a single node with too many neighbors: keep the lane kernel
+    // lower in-tile neighbors take the partner's slot
+    for (int64_t i = t0; i < t1; i++) {
+      const int n = c->h_deg[i];
+      for (int k = 0; k < n; k++) {
+        const int64_t j = col[at(i, k)];
+        if (j < t0 || j >= i) continue;
+        const int jb = kb_[j - t0], je = ke_[j - t0];
+        int lo = jb, hi = je;  // ascending: binary search for i in j's upper range
+        while (lo < hi) {
+          const int m = (lo + hi) / 2;
+          if (col[at(j, m)] < i) lo = m + 1; else hi = m;
+        }
+        if (lo >= je || col[at(j, lo)] != i) throw HipError{"PD neighbor lists are not symmetric"};
+        ps[at(i, k)] = (uint16_t)(base_[j - t0] + lo - jb);
+      }
+    }
+    tiles.push_back(RtTile{(int)t0, (int)t1, base, 1, pbase});
+    lds = std::max(lds, (size_t)(base + (2 * c->dim + 1) * (t1 - t0)) * sizeof(double));
+  }
+  RnpTileArgs& a = c->rt;
+  a.tiles = c->upload(tiles.data(), tiles.size());
+  a.pl = c->upload(pl.data(), pl.size());
+  a.ps = c->upload(ps.data(), ps.size());
+  a.rz = c->dalloc<double>(pl.size());
+  a.ntiles = (int)tiles.size();
+  a.lds_bytes = lds;
+  dispatch(
+      c,
+      [&] {
+        for (auto f : {(const void*)k_pd_rnp_tile<2, 0>, (const void*)k_pd_rnp_tile<2, 1>,
+                       (const void*)k_pd_rnp_tile<2, 2>})
+          HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      },
+      [&] {
+        for (auto f : {(const void*)k_pd_rnp_tile<3, 0>, (const void*)k_pd_rnp_tile<3, 1>,
+                       (const void*)k_pd_rnp_tile<3, 2>})
+          HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      });
+  c->rnp_tile = true;
 }
 
 // Tiles of the mixed kernels: for every state-based particle, the largest
@@ -543,6 +671,7 @@ void build(pdem_ctx* c, const pdem_desc* d) {
       c, [&] { hipLaunchKernelGGL(k_pd_fill<2>, dim3(blocks(N)), dim3(TPB), 0, c->stream, s); },
       [&] { hipLaunchKernelGGL(k_pd_fill<3>, dim3(blocks(N)), dim3(TPB), 0, c->stream, s); });
   check_launch(c);
+  if (c->rnp_any && !c->state_any && !c->pmb_any && !c->elastic_any) build_rnp_tiles(c);
 
   // ---- mixed precision: 16-bit neighbor ids and particle tiles
   if (c->precision == PDEM_PREC_MIXED && c->state_any) {
@@ -1598,6 +1727,9 @@ int pdem_autotune(pdem_ctx* c, int32_t reps, double* best_ms) {
       build_tiles(c, bn);
     } else {
       int bp = c->pd_tpb;
+      const bool have_tile = c->rt.ntiles > 0;
+      bool bt = false;
+      c->rnp_tile = false;
       for (int tpb : {64, 128, 256}) {
         c->pd_tpb = tpb;
         const double ms = time_pass();
@@ -1606,7 +1738,16 @@ int pdem_autotune(pdem_ctx* c, int32_t reps, double* best_ms) {
           bp = tpb;
         }
       }
+      if (have_tile) {
+        c->rnp_tile = true;
+        const double ms = time_pass();
+        if (ms < best) {
+          best = ms;
+          bt = true;
+        }
+      }
       c->pd_tpb = bp;
+      c->rnp_tile = bt;
     }
     c->profiling = prof;
     for (int d = 0; d < c->dim; d++)
@@ -1622,9 +1763,25 @@ int pdem_autotune(pdem_ctx* c, int32_t reps, double* best_ms) {
   });
 }
 
+int pdem_set_launch_config(pdem_ctx* c, int32_t pd_tpb, int32_t mixed_tpb, int32_t tile_nodes) {
+  return guarded(c, [&] {
+    if (pd_tpb != 64 && pd_tpb != 128 && pd_tpb != 256 && pd_tpb != RT_THREADS)
+      throw HipError{"pd_tpb must be 64, 128, 256 or 512 (the RNP pair-tile kernel)"};
+    if (pd_tpb == RT_THREADS && c->rt.ntiles == 0) throw HipError{"the RNP pair-tile kernel is not set up for this model"};
+    if (mixed_tpb != 256 && mixed_tpb != 1024) throw HipError{"mixed_tpb must be 256 or 1024"};
+    if (tile_nodes != 256 && tile_nodes != 512 && tile_nodes != 1024) throw HipError{"tile_nodes must be 256, 512 or 1024"};
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->rnp_tile = pd_tpb == RT_THREADS;
+    if (!c->rnp_tile) c->pd_tpb = pd_tpb;
+    c->mixed_tpb = mixed_tpb;
+    if (c->precision == PDEM_PREC_MIXED && c->ntiles > 0) build_tiles(c, tile_nodes);
+    drop_graph(c);
+  });
+}
+
 int pdem_get_launch_config(pdem_ctx* c, int32_t* pd_tpb, int32_t* mixed_tpb, int32_t* tile_nodes) {
   return guarded(c, [&] {
-    *pd_tpb = c->pd_tpb;
+    *pd_tpb = c->rnp_tile ? RT_THREADS : c->pd_tpb;
     *mixed_tpb = c->mixed_tpb;
     *tile_nodes = c->tile_nodes;
   });

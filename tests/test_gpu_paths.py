@@ -168,3 +168,38 @@ def test_capacity_overflow_is_an_error():
         g = capi.GpuEngine(spec, capi.PREC_FP64, contact_capacity=32)
         g.integrate_begin()
         g.integrate_cd(50)
+
+
+@pytest.mark.parametrize("case", ["compressive", "rnp2d_pull_break", "rnp3d"])
+def test_rnp_kernels_agree(case):
+    """The lane-per-node RNP kernel (workgroups of 64 and 256) and the
+    pair-tile kernel (each bond scalar once per pair, pd_tpb 512) give the
+    same step bit for bit: forces, displacements, velocities, bond bits and
+    damage, with bonds breaking."""
+    R = 1e-3
+    if case == "compressive":
+        spec = configs.compressive_2d(n_particles=12, nodes_per_particle=150, seed=3, init_speed=1.0,
+                                      pitch_factor=1.01, R_lo=1.2e-3, R_hi=1.2e-3)
+    elif case == "rnp2d_pull_break":
+        m = configs.rnp_material(Gc=5.0)
+        spec = _single(2, m, ubc=[BcDeck([], [1], capi.UFN_LINEAR, [-20.0],
+                                         region=_box(2, (-R, -R), (-0.7 * R, R)))])
+    else:
+        spec = CASES["rnp3d_sin_linear_step"]
+    runs = {}
+    for tpb in (64, 256, 512):
+        g = capi.GpuEngine(spec, capi.PREC_FP64)
+        g.set_launch_config(tpb)
+        assert g.launch_config()[0] == tpb
+        g.integrate_begin()
+        g.integrate_cd_graph(300)
+        runs[tpb] = {f: g.field(f) for f in (capi.FIELD_F, capi.FIELD_U, capi.FIELD_V, capi.FIELD_Z,
+                                              capi.FIELD_BONDS)}
+        g.close()
+    nb = int(runs[512][capi.FIELD_BONDS].sum())
+    print(f"[paths] rnp kernels {case}: nodes={spec.n_nodes} broken bonds {nb}", flush=True)
+    for tpb in (64, 256):
+        for f, a in runs[tpb].items():
+            np.testing.assert_array_equal(a, runs[512][f])
+    if case == "rnp2d_pull_break":
+        assert nb > 0
