@@ -216,7 +216,7 @@ struct Launch {
   static void contact_search(pdem_ctx* c) {
     hipStream_t st = c->stream;
     const unsigned nb = blocks(c->N);
-    hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, st, c->ds);
+    hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, st, c->ds, 0);
     hipLaunchKernelGGL(k_bin<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
     rebuild_lists(c);
     hipLaunchKernelGGL(k_cells_init, dim3((unsigned)std::min<int64_t>(blocks(c->ds.ncells), 4096)), dim3(TPB), 0,
@@ -231,10 +231,15 @@ struct Launch {
     hipLaunchKernelGGL(k_contact_force<DIM>, dim3((unsigned)((waves + CWAVES - 1) / CWAVES)),
                        dim3(CWAVES * WAVE), 0, c->stream, c->ds);
   }
-  static void damping(pdem_ctx* c) {
-    if (c->ds.n_nonwall > 0)
-      hipLaunchKernelGGL(k_damping<DIM>, dim3(c->ds.n_nonwall), dim3(64), 0, c->stream, c->ds);
+  static void damping(pdem_ctx* c, bool fin = false) {
+    if (c->ds.n_nonwall > 0) {
+      // with fin: blocks after the particles add gravity to the wall nodes
+      const int64_t wb = fin && c->ds.has_gravity ? std::min<int64_t>((c->ds.n_wall_nodes + 255) / 256, 256) : 0;
+      hipLaunchKernelGGL(k_damping<DIM>, dim3((unsigned)(c->ds.n_nonwall + wb)), dim3(fin ? 256 : 64), 0, c->stream,
+                         c->ds, fin ? 1 : 0);
+    }
   }
+  static bool fuse_finalize(pdem_ctx* c) { return c->multi && c->ds.n_nonwall > 0; }
   // rebuild steps: candidate query and damping partner lists in one launch
   // (grid-stride, capped grids keep the no-op launch of other steps short)
   static void rebuild_lists(pdem_ctx* c) {
@@ -260,9 +265,9 @@ struct Launch {
     damping(c);
     if (apply_damping) hipLaunchKernelGGL(k_apply_damping<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
   }
-  static void external_forces(pdem_ctx* c, bool with_damping) {
+  static void external_forces(pdem_ctx* c, bool with_damping, bool finalized = false) {
     hipStream_t st = c->stream;
-    if (c->ds.has_gravity || with_damping)
+    if (!finalized && (c->ds.has_gravity || with_damping))
       hipLaunchKernelGGL(k_finalize<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, st, c->ds,
                          with_damping ? 1 : 0);
     for (auto& b : c->fbc)
@@ -282,25 +287,36 @@ struct Launch {
   // (query on a rebuild, then forces), damping, external forces
   static void forces_after_ctrl(pdem_ctx* c) {
     pd_forces(c, c->multi ? 1 : 0);
+    const bool fin = fuse_finalize(c);
     if (c->multi) {
       rebuild_lists(c);
       closure(c);
       contact_force_kernel(c);
-      damping(c);
+      damping(c, fin);
     }
-    external_forces(c, c->multi != 0);
+    external_forces(c, c->multi != 0, fin);
   }
   static void compute_forces(pdem_ctx* c) {
-    if (c->multi) hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, c->stream, c->ds);
+    if (c->multi) hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, c->stream, c->ds, 0);
     forces_after_ctrl(c);
   }
   // integrateCD, or integrateVerlet when `verlet` (half kick and drift,
   // forces, second half kick)
   static void integrate_step(pdem_ctx* c, bool verlet = false) {
+    // the node update carries the displacement BC, the centre data and the
+    // controller unless a zero-displacement set still waits for its one
+    // application (it must come in set order)
+    bool fused = c->multi != 0;
+    for (size_t k = 0; k < c->ubc.size(); k++)
+      if (c->ubc_zero[k] && !c->zero_done[k]) fused = false;
     hipLaunchKernelGGL(k_integrate<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds, c->multi,
-                       verlet ? 1 : 0);
-    displacement_bc(c);
-    if (c->multi) hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, c->stream, c->ds);
+                       verlet ? 1 : 0, fused ? 1 : 0);
+    if (fused) {
+      hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, c->stream, c->ds, 1);
+    } else {
+      displacement_bc(c);
+      if (c->multi) hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, c->stream, c->ds, 0);
+    }
     forces_after_ctrl(c);
     if (verlet) hipLaunchKernelGGL(k_kick<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds);
   }
@@ -616,6 +632,32 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     c->ubc_zero.push_back(b.is_zero ? 1 : 0);
   }
   c->zero_done.assign(c->ubc.size(), 0);
+  {
+    // per-node entries of the sets applied every step (not the zero sets),
+    // in set order, for the fused node update
+    std::vector<int> cnt(N + 1, 0);
+    std::vector<BcSetParams> sets(d->n_ubc + 1);
+    for (int k = 0; k < d->n_ubc; k++) {
+      const auto& b = d->ubc[k];
+      sets[k].time_fn = b.time_fn;
+      sets[k].dir_mask = b.dir_mask;
+      for (int q = 0; q < 4; q++) sets[k].tp[q] = b.time_params[q];
+      if (b.is_zero) continue;
+      for (int64_t e = 0; e < b.n_entries; e++) cnt[b.nodes[e] + 1]++;
+    }
+    for (int64_t i = 0; i < N; i++) cnt[i + 1] += cnt[i];
+    if (cnt[N] > INT_MAX / 2) throw HipError{"too many displacement-BC entries"};
+    std::vector<BcEntry> ent(cnt[N] + 1);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int k = 0; k < d->n_ubc; k++) {
+      const auto& b = d->ubc[k];
+      if (b.is_zero) continue;
+      for (int64_t e = 0; e < b.n_entries; e++) ent[fill[b.nodes[e]]++] = BcEntry{b.amp[e], k, 0};
+    }
+    s.bc_ptr = c->upload(cnt.data(), cnt.size());
+    s.bc_ent = c->upload(ent.data(), ent.size());
+    s.bc_sets = c->upload(sets.data(), sets.size());
+  }
   for (int k = 0; k < d->n_fbc; k++) {
     const auto& b = d->fbc[k];
     DevBc o{};
@@ -835,6 +877,12 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     s.light_max = dim == 3 ? LIGHT_MAX_C : 0;
     s.nblockmax = (int)blocks(N);
     s.blockmax = c->dalloc<unsigned long long>(s.nblockmax);
+    std::vector<int64_t> wn;
+    for (int p = 0; p < c->np; p++)
+      if (c->hp[p].type_index != 0)
+        for (int64_t i = c->hp[p].glob_start; i < c->hp[p].glob_end; i++) wn.push_back(i);
+    s.n_wall_nodes = (int64_t)wn.size();
+    s.wall_nodes = c->upload(wn.data(), wn.size());
   }
   // state carried between steps: node fields, bond bits, damage, controller,
   // candidate and partner lists of the last rebuild (everything else is

@@ -68,6 +68,18 @@ struct Ctrl {
   unsigned long long broken;
 };
 
+// displacement-BC set parameters and the per-node entries of the sets that
+// are applied every step (zero sets excluded), for the fused node update
+struct BcSetParams {
+  int time_fn, dir_mask;
+  double tp[4];
+};
+struct BcEntry {
+  double amp;
+  int set;
+  int pad;
+};
+
 struct DS {
   int64_t N;
   int np, nz;
@@ -149,6 +161,13 @@ struct DS {
   // node-update block maxima of |v|
   unsigned long long* blockmax;
   int nblockmax;
+  // nodes of wall particles, ascending (finalisation inside the damping pass)
+  const int64_t* wall_nodes;
+  int64_t n_wall_nodes;
+  // per-node displacement-BC entries (CSR over nodes, set order)
+  const int* bc_ptr;
+  const BcEntry* bc_ent;
+  const BcSetParams* bc_sets;
   double gravity[3];
   int has_gravity;
   Ctrl* ctrl;
@@ -360,11 +379,21 @@ __device__ void ctrl_decide(Ctrl* cg) {
 // after each rebuild's query (k_contact_force), so no step clears it here.
 constexpr int CTRL_BLOCKS = 64;
 
-__global__ void __launch_bounds__(1024) k_ctrl(DS s) {
+__global__ void __launch_bounds__(1024) k_ctrl(DS s, int advance) {
   __shared__ unsigned long long red[1024 / WAVE];
   Ctrl* c = s.ctrl;
   const int t = threadIdx.x;
   if (blockIdx.x == 0) {
+    if (advance) {
+      // the fused node update left the time advance to this kernel
+      if (t == 0) {
+        c->current_dt = c->dt;
+        c->n += 1;
+        c->time += c->dt;
+        c->vdirty = 1;
+      }
+      __syncthreads();
+    }
     if (c->vdirty) {
       unsigned long long m = 0;
       for (int k = t; k < s.nblockmax; k += blockDim.x) m = s.blockmax[k] > m ? s.blockmax[k] : m;
@@ -382,6 +411,7 @@ __global__ void __launch_bounds__(1024) k_ctrl(DS s) {
     }
     if (t == 0) ctrl_decide(c);
   }
+  if (advance) return;  // centre data packed by the fused node update
   const int nn = s.n_nonwall;
   for (int q = blockIdx.x * blockDim.x + t; q < nn; q += gridDim.x * blockDim.x) {
     const int64_t cn = s.P[s.nonwall_pid[q]].center;
@@ -402,22 +432,73 @@ __device__ __forceinline__ void clear_cells(const DS& s, int64_t first, int64_t 
 }
 __global__ void k_cells_init(DS s) { clear_cells(s, gid(), (int64_t)gridDim.x * blockDim.x); }
 
+// ParticleULoading::apply (particleULoading.cpp:115-230) for one node of a set
+template <int DIM>
+__device__ __forceinline__ void apply_ubc(const DS& s, int time_fn, int dir_mask, const double* tp, double umax,
+                                          int64_t i, double time) {
+  double du = 0., dv = 0.;
+  if (time_fn == 0)
+    du = umax;
+  else if (time_fn == 1) {
+    du = umax * time;
+    dv = umax;
+  } else if (time_fn == 2) {
+    du = umax * time + tp[1] * time * time;
+    dv = umax + tp[1] * time;
+  } else if (time_fn == 3) {
+    double a = M_PI * tp[1];
+    du = umax * sin(a * time);
+    dv = umax * a * cos(a * time);
+  }
+  double ui[3] = {0., 0., 0.}, vi[3] = {0., 0., 0.};
+  for (int d = 0; d < 3; d++)
+    if (dir_mask >> d & 1) {
+      ui[d] = du;
+      vi[d] = dv;
+    }
+  if (time_fn == 4) {
+    double dx0 = s.xr[0][i] - tp[1], dx1 = s.xr[1][i] - tp[2];
+    double dx2 = (DIM == 3 ? s.xr[2][i] : 0.) - tp[3];
+    double th = tp[0] * time;
+    double rx0 = dx0 * cos(th) - dx1 * sin(th), rx1 = dx0 * sin(th) + dx1 * cos(th);
+    double d0 = -dx0 * sin(th) - dx1 * cos(th), d1 = dx0 * cos(th) - dx1 * sin(th);
+    ui[0] += rx0 - dx0;
+    ui[1] += rx1 - dx1;
+    ui[2] += 0.0 - dx2;
+    vi[0] += tp[0] * d0;
+    vi[1] += tp[0] * d1;
+    vi[2] += tp[0] * 0.0;
+  }
+  for (int d = 0; d < DIM; d++)
+    if (dir_mask >> d & 1) {
+      s.u[d][i] = ui[d];
+      s.v[d][i] = vi[d];
+      s.x[d][i] = ui[d] + s.xr[d][i];
+    }
+}
+
 // ------------------------------------------------------------ node update
 // integrateCD (demModel.cpp:349-390): semi-implicit Euler on free dofs;
 // with `half`, the first half kick and drift of integrateVerlet
 // (demModel.cpp:392-425): v += 0.5 (dt / rho) f.
+// With `fused` (multi-particle models), the same launch also applies the
+// displacement-BC sets of the step (computeExternalDisplacementBC, per node
+// in set order, at the advanced time t + dt) and packs the non-wall
+// particles' centre data for the damping pass; the time advance is then left
+// to the controller kernel that follows (k_ctrl with `advance`).
 template <int DIM>
-__global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
+__global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half, int fused) {
   int64_t i = gid();
   Ctrl* ct = s.ctrl;
   const double dt = ct->dt;
   double vm = 0.;
   if (i < s.N) {
-    const DevParticle& p = s.P[s.ptid[i]];
+    const int pid = s.ptid[i];
+    const DevParticle& p = s.P[pid];
+    const uint8_t fx = s.fix[i];
     if (p.compute_force) {
       const double rho = p.rho;
       const double a = half ? 0.5 * (dt / rho) : dt / rho;
-      const uint8_t fx = s.fix[i];
       double vv[3] = {0., 0., 0.};
       for (int d = 0; d < DIM; d++) {
         double vd = s.v[d][i];
@@ -431,6 +512,23 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
       }
       vm = len3<DIM>(vv);
       s.vmag[i] = vm;
+    }
+    if (fused) {
+      const double t_new = ct->time + dt;
+      const int e1 = s.bc_ptr[i + 1];
+      for (int e = s.bc_ptr[i]; e < e1; e++) {
+        const BcEntry be = s.bc_ent[e];
+        const BcSetParams& bs = s.bc_sets[be.set];
+        apply_ubc<DIM>(s, bs.time_fn, bs.dir_mask, bs.tp, be.amp, i, t_new);
+      }
+      if (p.type_index == 0 && p.center == i) {
+        const int nn = s.n_nonwall, q = p.nonwall_index;
+        for (int d = 0; d < 3; d++) {
+          s.pcx[d * nn + q] = d < DIM ? s.x[d][i] : 0.;
+          s.pcv[d * nn + q] = d < DIM ? s.v[d][i] : 0.;
+        }
+        if (s.cl_on) s.pflags[q] = 0;
+      }
     }
   }
   if (!multi) {
@@ -456,7 +554,7 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
     unsigned long long m = 0;
     for (int k = 0; k < (int)(blockDim.x / WAVE); k++) m = wmax[k] > m ? wmax[k] : m;
     s.blockmax[blockIdx.x] = m;
-    if (blockIdx.x == 0) {
+    if (!fused && blockIdx.x == 0) {
       ct->current_dt = dt;
       ct->n += 1;
       ct->time += dt;
@@ -496,52 +594,11 @@ struct DevBc {
   const double* amp;
 };
 
-// ParticleULoading::apply (particleULoading.cpp:115-230)
 template <int DIM>
 __global__ void k_ubc(DS s, DevBc bc) {
   int64_t k = gid();
   if (k >= bc.n) return;
-  const double time = s.ctrl->time;
-  const int64_t i = bc.nodes[k];
-  double umax = bc.amp[k], du = 0., dv = 0.;
-  if (bc.time_fn == 0)
-    du = umax;
-  else if (bc.time_fn == 1) {
-    du = umax * time;
-    dv = umax;
-  } else if (bc.time_fn == 2) {
-    du = umax * time + bc.tp[1] * time * time;
-    dv = umax + bc.tp[1] * time;
-  } else if (bc.time_fn == 3) {
-    double a = M_PI * bc.tp[1];
-    du = umax * sin(a * time);
-    dv = umax * a * cos(a * time);
-  }
-  double ui[3] = {0., 0., 0.}, vi[3] = {0., 0., 0.};
-  for (int d = 0; d < 3; d++)
-    if (bc.dir_mask >> d & 1) {
-      ui[d] = du;
-      vi[d] = dv;
-    }
-  if (bc.time_fn == 4) {
-    double dx0 = s.xr[0][i] - bc.tp[1], dx1 = s.xr[1][i] - bc.tp[2];
-    double dx2 = (DIM == 3 ? s.xr[2][i] : 0.) - bc.tp[3];
-    double th = bc.tp[0] * time;
-    double rx0 = dx0 * cos(th) - dx1 * sin(th), rx1 = dx0 * sin(th) + dx1 * cos(th);
-    double d0 = -dx0 * sin(th) - dx1 * cos(th), d1 = dx0 * cos(th) - dx1 * sin(th);
-    ui[0] += rx0 - dx0;
-    ui[1] += rx1 - dx1;
-    ui[2] += 0.0 - dx2;
-    vi[0] += bc.tp[0] * d0;
-    vi[1] += bc.tp[0] * d1;
-    vi[2] += bc.tp[0] * 0.0;
-  }
-  for (int d = 0; d < DIM; d++)
-    if (bc.dir_mask >> d & 1) {
-      s.u[d][i] = ui[d];
-      s.v[d][i] = vi[d];
-      s.x[d][i] = ui[d] + s.xr[d][i];
-    }
+  apply_ubc<DIM>(s, bc.time_fn, bc.dir_mask, bc.tp, bc.amp[k], bc.nodes[k], s.ctrl->time);
 }
 
 // ParticleFLoading::apply (particleFLoading.cpp:85-173)
@@ -1301,11 +1358,17 @@ __global__ void __launch_bounds__(256) k_rebuild_lists(DS s, int qblocks) {
 // Damping (demModel.cpp:937-1052): one wavefront per non-wall particle.
 // Particle-particle terms are evaluated by all lanes and accumulated in
 // ascending particle order; condensed wall nodes in ascending node order.
+// With `fin`, the wavefront then finalises its particle's nodes as
+// k_finalize does (damping force, closure forces, gravity), and the blocks
+// after the last particle add gravity to the wall nodes, so no separate pass
+// over the nodes is needed.
+__device__ __forceinline__ void closure_node_force(const DS& s, int a, int64_t i, double* fi);
+
+// the damping sums of non-wall particle q, one wavefront (lanes 0..63);
+// lane 0 stores the result in fdamp and in out[0..DIM)
 template <int DIM>
-__global__ void __launch_bounds__(64) k_damping(DS s) {
-  const int q = blockIdx.x;
+__device__ __forceinline__ void damping_wave(const DS& s, int q, double* out) {
   const int nn = s.n_nonwall;
-  if (q >= nn) return;
   const int lane = threadIdx.x;
   const int pid = s.nonwall_pid[q];
   const double Ri = s.prad[q];
@@ -1381,6 +1444,41 @@ __global__ void __launch_bounds__(64) k_damping(DS s) {
     s.wlo[q] = 0x7fffffff;
     s.whi[q] = -1;
     for (int d = 0; d < 3; d++) s.fdamp[3 * pid + d] = d < DIM ? force[d] : 0.;
+    for (int d = 0; d < DIM; d++) out[d] = force[d];
+  }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_damping(DS s, int fin) {
+  const int q = blockIdx.x;
+  const int nn = s.n_nonwall;
+  if (q >= nn) {
+    // finalisation of wall nodes: gravity only (k_finalize)
+    if (fin && s.has_gravity)
+      for (int64_t k = (int64_t)(q - nn) * blockDim.x + threadIdx.x; k < s.n_wall_nodes;
+           k += (int64_t)(gridDim.x - nn) * blockDim.x) {
+        const int64_t i = s.wall_nodes[k];
+        const double rho = s.P[s.ptid[i]].rho;
+        for (int d = 0; d < DIM; d++) s.f[d][i] += rho * s.gravity[d];
+      }
+    return;
+  }
+  __shared__ double fsh[3];
+  if (threadIdx.x < WAVE) damping_wave<DIM>(s, q, fsh);
+  if (!fin) return;
+  __syncthreads();
+  const DevParticle& p = s.P[s.nonwall_pid[q]];
+  const bool cl = DIM == 2 && s.cl_on && (s.pflags[q] & 2);
+  for (int64_t i = p.glob_start + threadIdx.x; i < p.glob_end; i += blockDim.x) {
+    for (int d = 0; d < DIM; d++) s.f[d][i] += fsh[d];
+    if (cl && s.surf[i]) {
+      double fi[2] = {0., 0.};
+      closure_node_force(s, q, i, fi);
+      s.f[0][i] += fi[0];
+      s.f[1][i] += fi[1];
+    }
+    if (s.has_gravity)
+      for (int d = 0; d < DIM; d++) s.f[d][i] += p.rho * s.gravity[d];
   }
 }
 
