@@ -271,6 +271,12 @@ int pdem_profile_get(pdem_ctx *ctx, double *pd_kernel_ms_avg, int64_t *pd_kernel
 int pdem_autotune(pdem_ctx *ctx, int32_t reps, double *best_ms);
 int pdem_get_launch_config(pdem_ctx *ctx, int32_t *pd_tpb, int32_t *mixed_tpb, int32_t *tile_nodes);
 int pdem_set_launch_config(pdem_ctx *ctx, int32_t pd_tpb, int32_t mixed_tpb, int32_t tile_nodes);
+/* Step fusion for multi-particle models of up to 2^20 nodes (on by default):
+ * the node update also applies the displacement BC and packs the particle
+ * centres, the controller runs as one workgroup and the damping pass
+ * finalises the nodes, which saves launches where the step is launch-bound.
+ * Results are the same bit for bit either way. No counterpart in the reference. */
+int pdem_set_step_fusion(pdem_ctx *ctx, int32_t on);
 
 /* ---------------------------------------------------------------------
  * Learned pair-contact closure network.  No counterpart in the reference:

@@ -377,6 +377,8 @@ def engine_library():
         lib.pdem_get_launch_config.argtypes = [C.c_void_p] + [C.POINTER(C.c_int32)] * 3
         lib.pdem_set_launch_config.restype = C.c_int
         lib.pdem_set_launch_config.argtypes = [C.c_void_p] + [C.c_int32] * 3
+        lib.pdem_set_step_fusion.restype = C.c_int
+        lib.pdem_set_step_fusion.argtypes = [C.c_void_p, C.c_int32]
         _engine_lib = lib
     return _engine_lib
 
@@ -466,6 +468,10 @@ class GpuEngine(Engine):
     def set_launch_config(self, pd_tpb, mixed_tpb=1024, tile_nodes=1024):
         """Apply a launch shape (pd_tpb 512: the RNP pair-tile kernel)."""
         self._check(self.lib.pdem_set_launch_config(self.ctx, pd_tpb, mixed_tpb, tile_nodes), "set_launch_config")
+
+    def set_step_fusion(self, on=True):
+        """Fused step kernels for small multi-particle models (default on)."""
+        self._check(self.lib.pdem_set_step_fusion(self.ctx, int(on)), "set_step_fusion")
 
     def closure_setup(self, codes=None):
         """Closure data; codes: one float per particle (shape/material class)."""

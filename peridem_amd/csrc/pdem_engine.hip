@@ -31,6 +31,9 @@ namespace {
 thread_local std::string g_create_err;
 }  // namespace
 
+// multi-particle models up to this many nodes use the fused step kernels
+constexpr int64_t FUSE_MAX_NODES = (int64_t)1 << 20;
+
 struct pdem_ctx {
   int dim = 2, precision = 0, multi = 0;
   int64_t N = 0;
@@ -66,6 +69,7 @@ struct pdem_ctx {
   // bond-based RNP pair-tile kernel (FP64, all force-computing particles RNP)
   RnpTileArgs rt{};
   bool rnp_tile = false;  // use it (set up by default, autotuned)
+  bool step_fusion = true;  // fused step kernels for small multi-particle models
   // strain / stress post-processing (pdem_fe_setup)
   FeArgs fe{};
   std::vector<void*> fe_bufs;
@@ -239,7 +243,17 @@ struct Launch {
                          c->ds, fin ? 1 : 0);
     }
   }
-  static bool fuse_finalize(pdem_ctx* c) { return c->multi && c->ds.n_nonwall > 0; }
+  // Fused step (node update with BC and centre data, one-workgroup
+  // controller, damping pass finalising the nodes): fewer launches for
+  // models small enough that the step is launch-bound; the unfused kernels
+  // stream large models at full occupancy.
+  static bool fuse_step(pdem_ctx* c) {
+    if (!c->multi || !c->step_fusion || c->N > FUSE_MAX_NODES) return false;
+    for (size_t k = 0; k < c->ubc.size(); k++)
+      if (c->ubc_zero[k] && !c->zero_done[k]) return false;  // must apply in set order first
+    return true;
+  }
+  static bool fuse_finalize(pdem_ctx* c) { return fuse_step(c) && c->ds.n_nonwall > 0; }
   // rebuild steps: candidate query and damping partner lists in one launch
   // (grid-stride, capped grids keep the no-op launch of other steps short)
   static void rebuild_lists(pdem_ctx* c) {
@@ -303,14 +317,13 @@ struct Launch {
   // integrateCD, or integrateVerlet when `verlet` (half kick and drift,
   // forces, second half kick)
   static void integrate_step(pdem_ctx* c, bool verlet = false) {
-    // the node update carries the displacement BC, the centre data and the
-    // controller unless a zero-displacement set still waits for its one
-    // application (it must come in set order)
-    bool fused = c->multi != 0;
-    for (size_t k = 0; k < c->ubc.size(); k++)
-      if (c->ubc_zero[k] && !c->zero_done[k]) fused = false;
-    hipLaunchKernelGGL(k_integrate<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds, c->multi,
-                       verlet ? 1 : 0, fused ? 1 : 0);
+    const bool fused = fuse_step(c);
+    if (fused)
+      hipLaunchKernelGGL((k_integrate<DIM, true>), dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds, c->multi,
+                         verlet ? 1 : 0);
+    else
+      hipLaunchKernelGGL((k_integrate<DIM, false>), dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds, c->multi,
+                         verlet ? 1 : 0);
     if (fused) {
       hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(1024), 0, c->stream, c->ds, 1);
     } else {
@@ -1823,6 +1836,14 @@ int pdem_set_launch_config(pdem_ctx* c, int32_t pd_tpb, int32_t mixed_tpb, int32
     if (!c->rnp_tile) c->pd_tpb = pd_tpb;
     c->mixed_tpb = mixed_tpb;
     if (c->precision == PDEM_PREC_MIXED && c->ntiles > 0) build_tiles(c, tile_nodes);
+    drop_graph(c);
+  });
+}
+
+int pdem_set_step_fusion(pdem_ctx* c, int32_t on) {
+  return guarded(c, [&] {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->step_fusion = on != 0;
     drop_graph(c);
   });
 }

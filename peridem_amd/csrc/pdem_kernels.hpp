@@ -486,8 +486,8 @@ __device__ __forceinline__ void apply_ubc(const DS& s, int time_fn, int dir_mask
 // in set order, at the advanced time t + dt) and packs the non-wall
 // particles' centre data for the damping pass; the time advance is then left
 // to the controller kernel that follows (k_ctrl with `advance`).
-template <int DIM>
-__global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half, int fused) {
+template <int DIM, bool fused>
+__global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
   int64_t i = gid();
   Ctrl* ct = s.ctrl;
   const double dt = ct->dt;
