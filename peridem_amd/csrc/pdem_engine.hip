@@ -471,13 +471,12 @@ void build_rnp_tiles(pdem_ctx* c) {
       }
     }
     tiles.push_back(RtTile{(int)t0, (int)t1, base, 1, pbase});
-    lds = std::max(lds, (size_t)(base + (2 * c->dim + 1) * (t1 - t0)) * sizeof(double));
+    lds = std::max(lds, (size_t)(base + (2 * c->dim + 1) * (t1 - t0)) * sizeof(double) + (size_t)base * sizeof(float));
   }
   RnpTileArgs& a = c->rt;
   a.tiles = c->upload(tiles.data(), tiles.size());
   a.pl = c->upload(pl.data(), pl.size());
   a.ps = c->upload(ps.data(), ps.size());
-  a.rz = c->dalloc<double>(pl.size());
   a.ntiles = (int)tiles.size();
   a.lds_bytes = lds;
   dispatch(

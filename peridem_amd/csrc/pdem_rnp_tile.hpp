@@ -8,14 +8,18 @@
 // |S / Sc|.  A workgroup owns a tile of consecutive nodes of one particle:
 //   1. every pair (i, j) with i < j inside the tile is evaluated once, one
 //      lane per pair (the exp, four of the eight divides and a square root);
-//      its force scalar ef goes to LDS and |S / Sc| (negative when the bond
-//      breaks now) to a per-pair array that stays in L2;
+//      its force scalar ef (FP64) and |S / Sc| rounded to FP32 (negative when
+//      the bond breaks now) go to LDS;
 //   2. every node then runs its neighbor list in order, as
 //      computePeridynamicForces does (demModel.cpp:682-788): in-tile bonds
 //      read the pair record and add ef V_j (dx / r), out-of-tile bonds are
 //      evaluated in full.  Sums, the break bits and the damage maximum
 //      follow the list order, so the result equals the one-lane-per-node
-//      kernel bit for bit.
+//      kernel bit for bit.  The damage maximum keeps a value only as float,
+//      so the FP32 |S / Sc| is the value it stores; the relative compare that
+//      decides whether it is stored is taken on the FP32 value when that
+//      cannot change its outcome (margin 2^-22 of the value, eight times the
+//      rounding error) and on the FP64 value, recomputed, otherwise.
 // Tiles, the pair list of every tile and the pair slot of every in-tile bond
 // (uint16) are built once at set-up (PD lists never change).
 #pragma once
@@ -41,7 +45,6 @@ struct RnpTileArgs {
   const RtTile* tiles = nullptr;
   const uint32_t* pl = nullptr;    // [pairs] tile-local (i | j << 16), i < j, in slot order
   const uint16_t* ps = nullptr;    // sliced ELL like col: pair slot of in-tile bonds, RT_OUT otherwise
-  double* rz = nullptr;            // [pairs] +-|S / Sc| of this step
   int ntiles = 0;
   size_t lds_bytes = 0;
 };
@@ -64,6 +67,7 @@ __global__ void __launch_bounds__(RT_THREADS) k_pd_rnp_tile(DS s, RnpTileArgs a,
   double* sx = lds + t.npairs;            // [DIM][n_t] reference coordinates
   double* su = sx + DIM * n_t;            // [DIM][n_t] displacements
   double* sv = su + DIM * n_t;            // [n_t] volumes
+  float* pz = (float*)(sv + n_t);         // [npairs] +-|S / Sc| as float
   for (int q = threadIdx.x; q < n_t; q += blockDim.x) {
     const int64_t i = t.t0 + q;
     for (int d = 0; d < DIM; d++) {
@@ -78,7 +82,6 @@ __global__ void __launch_bounds__(RT_THREADS) k_pd_rnp_tile(DS s, RnpTileArgs a,
   const double Cc = p.mat[0], beta = p.mat[1], rbar = p.mat[2], invF = p.mat[3], fsc = p.mat[4];
   const bool irrev = p.irreversible != 0;
   const double Kn = p.Kn, Rc = p.Rc;
-  double* __restrict__ rz = a.rz + t.pbase;
   __syncthreads();
 
   // 1. pair records, one lane per pair
@@ -99,7 +102,7 @@ __global__ void __launch_bounds__(RT_THREADS) k_pd_rnp_tile(DS s, RnpTileArgs a,
     if (!brk) ef = J * 4. * Sji * Cc * beta * exp(-beta * rji * Sji * Sji) / invF;
     const double z = fabs(Sji / Sc);
     pe[q] = ef;
-    rz[q] = brk ? -z : z;
+    pz[q] = brk ? -(float)z : (float)z;
   }
   __syncthreads();
 
@@ -120,38 +123,18 @@ __global__ void __launch_bounds__(RT_THREADS) k_pd_rnp_tile(DS s, RnpTileArgs a,
     double force[3] = {0., 0., 0.};
     float Zi = 0.f;
     unsigned word = 0, wnew = 0;
-    // software pipeline: column and slot two bonds ahead, the pair's
-    // |S / Sc| one bond ahead (it depends on the slot)
-    int j1 = 0, s1 = RT_OUT, j2 = 0, s2 = RT_OUT;
-    double z1 = 0.;
-    if (n > 0) {
-      j1 = c[0];
-      s1 = ps[0];
-    }
-    if (n > 1) {
-      j2 = c[WAVE];
-      s2 = ps[WAVE];
-    }
-    if (s1 != RT_OUT) z1 = rz[s1];
     for (int k = 0; k < n; k++) {
       if ((k & 31) == 0) {
         if (k > 0 && wnew != word) bw[(int64_t)((k >> 5) - 1) * WAVE] = wnew;
         word = bw[(int64_t)(k >> 5) * WAVE];
         wnew = word;
       }
-      const int j = j1;
-      const int slot = s1;
-      const double zs_k = z1;
-      j1 = j2;
-      s1 = s2;
-      if (k + 2 < n) {
-        j2 = c[(int64_t)(k + 2) * WAVE];
-        s2 = ps[(int64_t)(k + 2) * WAVE];
-      }
-      if (k + 1 < n && s1 != RT_OUT) z1 = rz[s1];
+      const int j = c[(int64_t)k * WAVE];
+      const int slot = ps[(int64_t)k * WAVE];
       const bool fs0 = (word >> (k & 31)) & 1u;
       double dx[3], xj[3], uj[3];
       double volj, z;
+      bool zexact = true;  // z holds the FP64 value (else its FP32 rounding)
       if (slot != RT_OUT) {
         const int jq = j - t.t0;
         for (int d = 0; d < DIM; d++) {
@@ -160,13 +143,14 @@ __global__ void __launch_bounds__(RT_THREADS) k_pd_rnp_tile(DS s, RnpTileArgs a,
           dx[d] = xj[d] - xi[d];
         }
         volj = sv[jq];
-        const double zs = zs_k;
-        z = fabs(zs);
+        const float zs = pz[slot];
+        z = (double)fabsf(zs);
+        zexact = false;
         if (!fs0) {
           const double rji = len3<DIM>(dx);
           if (isGreater(rji, check_low)) volj *= (check_up - rji) / h;
           double ef = pe[slot];
-          if (zs < 0.) {  // breaks in this step: no force from it now
+          if (zs < 0.f) {  // breaks in this step: no force from it now
             wnew |= 1u << (k & 31);
             ef = 0.;
           }
@@ -207,7 +191,24 @@ __global__ void __launch_bounds__(RT_THREADS) k_pd_rnp_tile(DS s, RnpTileArgs a,
         if (scalar_f > 0.) scalar_f = 0.;
         for (int d = 0; d < DIM; d++) force[d] += scalar_f * y[d];
       }
-      if (isGreater(z, (double)Zi)) Zi = (float)z;
+      if (zexact) {
+        if (isGreater(z, (double)Zi)) Zi = (float)z;
+      } else {
+        // z is |S / Sc| rounded to float: decide on it unless the compare is
+        // within reach of that rounding, then on the recomputed FP64 value
+        const double Zd = (double)Zi;
+        const double g = (z - Zd) - ((z < Zd ? Zd : z) * COMPARE_EPS);
+        if (z != 0. && fabs(g) <= 0x1p-22 * z) {
+          double du[3];
+          for (int d = 0; d < DIM; d++) du[d] = uj[d] - ui[d];
+          const double rji = len3<DIM>(dx);
+          const double Sji = dot3<DIM>(dx, du) / dot3<DIM>(dx, dx);
+          const double zx = fabs(Sji / (rbar / sqrt(rji)));
+          if (isGreater(zx, Zd)) Zi = (float)zx;
+        } else if (g > 0.) {
+          Zi = (float)z;
+        }
+      }
     }
     if (n > 0 && wnew != word) bw[(int64_t)((n - 1) >> 5) * WAVE] = wnew;
     for (int d = 0; d < DIM; d++) s.f[d][i] = force[d];
