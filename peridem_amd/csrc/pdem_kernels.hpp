@@ -1076,6 +1076,19 @@ __device__ __forceinline__ bool pair_plain(const DS& s, int a, int b) {
 // ------------------------------------------------------- contact forces
 // Candidate query on a rebuild: one lane per searching node; nodes that end
 // with candidates (and compute forces) are appended to the active list.
+// append slot for the lanes of a wavefront whose `pred` holds: one atomic
+// per wavefront (the lists are consumed per node, so their order is free)
+__device__ __forceinline__ int wave_append(unsigned* counter, bool pred) {
+  const unsigned long long m = __ballot(pred);
+  if (!m) return -1;
+  const int leader = __ffsll((long long)m) - 1;
+  const int lane = (int)(threadIdx.x & (WAVE - 1));
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned)__popcll(m));
+  base = __shfl(base, leader);
+  return pred ? (int)(base + __popcll(m & ((1ull << lane) - 1ull))) : -1;
+}
+
 template <int DIM>
 __device__ __forceinline__ void query_one(const DS& s, int64_t i) {
   const int pid = s.ptid[i];
@@ -1089,12 +1102,13 @@ __device__ __forceinline__ void query_one(const DS& s, int64_t i) {
         s.wall_near[pi.nonwall_index] = (int)s.ctrl->n_rebuilds;
         break;
       }
-  if (cnt > 0 && pi.compute_force) {
-    if (cnt <= s.light_max)
-      s.light[atomicAdd(&s.ctrl->nlight, 1u)] = (int)i;
-    else
-      s.active[atomicAdd(&s.ctrl->nactive, 1u)] = (int)i;
-  }
+  const bool want = cnt > 0 && pi.compute_force;
+  const bool lt = want && cnt <= s.light_max;
+  const bool ac = want && cnt > s.light_max;
+  const int pl = wave_append(&s.ctrl->nlight, lt);
+  const int pa = wave_append(&s.ctrl->nactive, ac);
+  if (lt) s.light[pl] = (int)i;
+  if (ac) s.active[pa] = (int)i;
 }
 
 // computeContactForces node loop for nodes with few candidates: one lane per
