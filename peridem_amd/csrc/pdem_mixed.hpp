@@ -55,6 +55,9 @@ __device__ __forceinline__ bool isGreaterF(float a, float b) {
   return (a - b) > ((fa < fb ? fb : fa) * 1e-5f);
 }
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+// hardware square root (1 ulp, no denormal rescaling): bond lengths squared
+// are far from the FP32 denormal range
+__device__ __forceinline__ float sqrtf_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 template <int INF>
 __device__ __forceinline__ float infJf(const DS& s, float r_over_delta) {
@@ -170,8 +173,8 @@ __global__ void __launch_bounds__(TPB) k_mixed_dilation(DS s, MixedArgs m) {
             const float y = dx[d] + du[d];
             yy += y * y;
           }
-          const float r = sqrtf(xx);
-          const float cl = (2.f * xe + ee) * rcpf(sqrtf(yy) + r);
+          const float r = sqrtf_hw(xx);
+          const float cl = (2.f * xe + ee) * rcpf(sqrtf_hw(yy) + r);
           const float st = cl * rcpf(r);
           const unsigned bit = 1u << (k & 31);
           bool fs = (word & bit) != 0u;
@@ -256,8 +259,8 @@ __global__ void __launch_bounds__(TPB) k_mixed_force(DS s, MixedArgs m) {
             ee += du[d] * du[d];
             yy += y[d] * y[d];
           }
-          const float r = sqrtf(xx);
-          const float ly = sqrtf(yy);
+          const float r = sqrtf_hw(xx);
+          const float ly = sqrtf_hw(yy);
           const float inv_ly = rcpf(ly);
           const float cl = (2.f * xe + ee) * rcpf(ly + r);
           const float S = cl * rcpf(r);
