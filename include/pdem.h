@@ -229,6 +229,11 @@ int pdem_get_contact_state(pdem_ctx *ctx, pdem_contact_state *st);
  * sum_i f_i[f_dir] V_i over the wall's nodes; f_dir is 0-based. */
 int pdem_pp_compressive_test(pdem_ctx *ctx, int32_t wall_id, int32_t f_dir,
                              double *penetration, double *reaction);
+/* DEMModel::ppTwoParticleTest (demModel.cpp:2096-2149) quantities of the
+ * first two particles: out5 = {pen_dist, contact_area_radius, max_y,
+ * max_dist, max_y_loc} (the reference's key data of the same names; the
+ * reference's check_dt cadence stays with the caller). */
+int pdem_pp_two_particle_test(pdem_ctx *ctx, double *out5);
 /* number of broken bonds and of all bonds (damage curve) */
 int pdem_broken_bonds(pdem_ctx *ctx, int64_t *broken, int64_t *total);
 

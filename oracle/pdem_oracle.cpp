@@ -1051,6 +1051,41 @@ int oracle_pp_compressive_test(oracle_ctx *c, int32_t wall_id, int32_t f_dir, do
   return 0;
 }
 
+// DEMModel::ppTwoParticleTest, demModel.cpp:2096-2149 (without the
+// d_dtTestOut cadence): out5 = pen_dist, contact_area_radius, max_y,
+// max_dist, max_y_loc
+int oracle_pp_two_particle_test(oracle_ctx *c, double *out5) {
+  if (c->P.size() < 2 || !c->multi) return fail(c, "the two-particle test needs two particles");
+  const auto &p0 = c->P[0];
+  const auto &p1 = c->P[1];
+  const P3 xc0 = c->x[p0.center_node], xc1 = c->x[p1.center_node];
+  const double r = p0.bounding_radius;
+  const auto &ct = c->C[p0.zone * c->nz + p1.zone];
+  const double r_e = r + ct.contact_r;
+  const double dist = std::sqrt((xc0.x - xc1.x) * (xc0.x - xc1.x) + (xc0.y - xc1.y) * (xc0.y - xc1.y) +
+                                (xc0.z - xc1.z) * (xc0.z - xc1.z));
+  double pen = dist - r_e - r;
+  double area_r = 0.;
+  if (isLess(pen, 0.))
+    area_r = std::sqrt(std::pow(r_e, 2.) - std::pow(r_e + pen, 2.));
+  else if (isGreater(pen, 0.)) {
+    pen = 0.;
+    area_r = 0.;
+  }
+  const double max_dist = xc1.y + p1.bounding_radius;
+  double max_y_loc = c->x[p1.glob_start].y;
+  for (int64_t i = p1.glob_start; i < p1.glob_end; i++)
+    if (isLess(max_y_loc, c->x[i].y)) max_y_loc = c->x[i].y;
+  double max_y = 0.;
+  if (isLess(max_y, max_y_loc)) max_y = max_y_loc;
+  out5[0] = pen;
+  out5[1] = area_r;
+  out5[2] = max_y;
+  out5[3] = max_dist;
+  out5[4] = max_y_loc;
+  return 0;
+}
+
 int oracle_broken_bonds(oracle_ctx *c, int64_t *broken, int64_t *total) {
   int64_t b = 0;
   for (auto x : c->bits) b += x ? 1 : 0;

@@ -195,6 +195,7 @@ def declare(lib: C.CDLL, prefix: str, extra_create_int: bool = False):
     f("pp_compressive_test", C.c_int,
       [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)])
     f("broken_bonds", C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
+    f("pp_two_particle_test", C.c_int, [vp, C.POINTER(C.c_double)])
     return lib
 
 
@@ -300,6 +301,13 @@ class Engine:
         st = ContactState()
         self._check(self._f("get_contact_state")(self.ctx, C.byref(st)), "get_contact_state")
         return {k: getattr(st, k) for k, _ in ContactState._fields_}
+
+    def pp_two_particle_test(self):
+        """DEMModel::ppTwoParticleTest key data: pen_dist, contact_area_radius,
+        max_y, max_dist, max_y_loc."""
+        out = (C.c_double * 5)()
+        self._check(self._f("pp_two_particle_test")(self.ctx, out), "pp_two_particle_test")
+        return dict(zip(("pen_dist", "contact_area_radius", "max_y", "max_dist", "max_y_loc"), list(out)))
 
     def pp_compressive_test(self, wall_id, f_dir):
         pen = C.c_double()

@@ -70,6 +70,7 @@ struct pdem_ctx {
   RnpTileArgs rt{};
   bool rnp_tile = false;  // use it (set up by default, autotuned)
   bool step_fusion = true;  // fused step kernels for small multi-particle models
+  double* pp_out = nullptr;  // two-particle test results
   // strain / stress post-processing (pdem_fe_setup)
   FeArgs fe{};
   std::vector<void*> fe_bufs;
@@ -1320,6 +1321,18 @@ int pdem_pp_compressive_test(pdem_ctx* c, int32_t wall_id, int32_t f_dir, double
     Ctrl h = read_ctrl(c);
     *pen = h.pp_pen;
     *reac = h.pp_reac;
+  });
+}
+
+int pdem_pp_two_particle_test(pdem_ctx* c, double* out5) {
+  return guarded(c, [&] {
+    if (c->np < 2 || !c->multi) throw HipError{"the two-particle test needs a multi-particle model of two particles or more"};
+    if (!out5) throw HipError{"null output"};
+    if (!c->pp_out) c->pp_out = c->dalloc<double>(8);
+    hipLaunchKernelGGL(k_pp_two_particle, dim3(1), dim3(64), 0, c->stream, c->ds, c->pp_out);
+    check_launch(c);
+    HIPCHK(hipMemcpyAsync(out5, c->pp_out, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
   });
 }
 

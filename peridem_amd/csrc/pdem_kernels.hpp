@@ -1570,6 +1570,45 @@ __global__ void k_pp_compressive(DS s, int wall, int fdir) {
   s.ctrl->pp_reac = r;
 }
 
+// DEMModel::ppTwoParticleTest (demModel.cpp:2096-2149), one thread: centre
+// distance against the first particle's bounding radius (used for both, as
+// the reference does) plus the pair's contact radius, the contact-area radius,
+// and the largest y of the second particle's nodes (relative compare, node
+// order).  out: pen_dist, contact_area_radius, max_y, max_dist, max_y_loc.
+__global__ void k_pp_two_particle(DS s, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const DevParticle& p0 = s.P[0];
+  const DevParticle& p1 = s.P[1];
+  double xc0[3] = {0., 0., 0.}, xc1[3] = {0., 0., 0.};
+  for (int d = 0; d < 3; d++) {
+    xc0[d] = s.x[d] ? s.x[d][p0.center] : 0.;
+    xc1[d] = s.x[d] ? s.x[d][p1.center] : 0.;
+  }
+  const double r = p0.Rb;
+  const double r_e = r + s.CT[p0.zone * s.nz + p1.zone].contact_r;
+  const double dist = sqrt((xc1[0] - xc0[0]) * (xc1[0] - xc0[0]) + (xc1[1] - xc0[1]) * (xc1[1] - xc0[1]) +
+                           (xc1[2] - xc0[2]) * (xc1[2] - xc0[2]));
+  double pen = dist - r_e - r;
+  double area_r = 0.;
+  if (isLess(pen, 0.))
+    area_r = sqrt(pow(r_e, 2.) - pow(r_e + pen, 2.));
+  else if (isGreater(pen, 0.)) {
+    pen = 0.;
+    area_r = 0.;
+  }
+  const double max_dist = xc1[1] + p1.Rb;
+  double max_y_loc = s.x[1][p1.glob_start];
+  for (int64_t i = p1.glob_start; i < p1.glob_end; i++)
+    if (isLess(max_y_loc, s.x[1][i])) max_y_loc = s.x[1][i];
+  double max_y = 0.;
+  if (isLess(max_y, max_y_loc)) max_y = max_y_loc;
+  out[0] = pen;
+  out[1] = area_r;
+  out[2] = max_y;
+  out[3] = max_dist;
+  out[4] = max_y_loc;
+}
+
 __global__ void k_count_broken(DS s) {
   int64_t i = gid();
   unsigned long long c = 0;

@@ -138,3 +138,19 @@ def test_step_fusion_is_bit_exact(which):
             assert out[0][k] == out[1][k]
         else:
             np.testing.assert_array_equal(out[0][k], out[1][k])
+
+
+def test_two_particle_test_observables():
+    spec = configs.two_particle(gap=0.0, speed=0.5)
+    g = capi.GpuEngine(spec)
+    r = oracle.OracleEngine(spec, 8)
+    g.integrate_begin()
+    r.integrate_begin()
+    for k in range(4):
+        g.integrate_cd_graph(60)
+        r.integrate_cd(60)
+        a, b = g.pp_two_particle_test(), r.pp_two_particle_test()
+        print(f"[pp2] step {(k + 1) * 60}: {a}", flush=True)
+        for key in b:
+            assert abs(a[key] - b[key]) <= 1e-12 * max(abs(b[key]), 1e-30), (key, a[key], b[key])
+    assert a["pen_dist"] < 0 and a["contact_area_radius"] > 0

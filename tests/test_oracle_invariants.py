@@ -335,3 +335,25 @@ def test_ensemble_instances_independent():
         np.testing.assert_array_equal(e.field(capi.FIELD_F), F[n0:n1])
         touched += int(o.field(capi.FIELD_Z)[n0:n1].max() > 0.0)
     assert touched >= 1
+
+
+def test_two_particle_test_observables_closed_form():
+    """ppTwoParticleTest at rest: the centres are 2R + gap apart, so the
+    penetration is gap - Rc (the first particle's radius is used for both,
+    equal radii here), the contact-area radius is sqrt(r_e^2 - (r_e + pen)^2)
+    with r_e = R + Rc, max_dist is the second centre's y plus R and max_y
+    the largest y of the second particle's nodes."""
+    R, gap = 1.0e-3, 0.0
+    spec = configs.two_particle(R=R, gap=gap, speed=0.5)
+    o = oracle.OracleEngine(spec, 4)
+    pp = o.pp_two_particle_test()
+    Rc = spec.contact[0][0]["contact_r"]
+    pen = (2 * R + gap) - (R + Rc) - R
+    assert pp["pen_dist"] == pytest.approx(pen, rel=1e-12, abs=1e-18)
+    r_e = R + Rc
+    assert pp["contact_area_radius"] == pytest.approx(math.sqrt(r_e ** 2 - (r_e + pen) ** 2), rel=1e-9)
+    p1 = spec.particles[1]
+    x = o.field(capi.FIELD_X)
+    assert pp["max_dist"] == pytest.approx(x[p1["center_node"], 1] + R, rel=1e-15)
+    ymax = x[p1["glob_start"]:p1["glob_end"], 1].max()
+    assert pp["max_y_loc"] == pytest.approx(ymax, rel=1e-5) and pp["max_y"] == pp["max_y_loc"]
