@@ -234,6 +234,17 @@ int pdem_pp_compressive_test(pdem_ctx *ctx, int32_t wall_id, int32_t f_dir,
  * max_dist, max_y_loc} (the reference's key data of the same names; the
  * reference's check_dt cadence stays with the caller). */
 int pdem_pp_two_particle_test(pdem_ctx *ctx, double *out5);
+/* DEMModel::checkStop (demModel.cpp:2151-2200): the quantity of an output
+ * criterion and whether it exceeds param (relative compare).
+ * PDEM_STOP_MAX_PARTICLE_DIST ("max_particle_dist", two-particle test):
+ * distance of the first two particles' centre nodes; node = -1.
+ * PDEM_STOP_MAX_NODE_DIST ("max_node_dist"): the largest |x| over all nodes
+ * and the first node holding it. The reference exits the process when stop
+ * is set; here the caller decides. */
+#define PDEM_STOP_MAX_PARTICLE_DIST 1
+#define PDEM_STOP_MAX_NODE_DIST 2
+int pdem_check_stop(pdem_ctx *ctx, int32_t criterion, double param, int32_t *stop, double *value,
+                    int64_t *node);
 /* number of broken bonds and of all bonds (damage curve) */
 int pdem_broken_bonds(pdem_ctx *ctx, int64_t *broken, int64_t *total);
 

@@ -1086,6 +1086,35 @@ int oracle_pp_two_particle_test(oracle_ctx *c, double *out5) {
   return 0;
 }
 
+// DEMModel::checkStop, demModel.cpp:2151-2200 (the criterion's quantity;
+// util::methods::maxLengthAndMaxLengthIndex, methods.h:38-48, 208-215)
+int oracle_check_stop(oracle_ctx *c, int32_t criterion, double param, int32_t *stop, double *value,
+                      int64_t *node) {
+  *node = -1;
+  if (criterion == PDEM_STOP_MAX_PARTICLE_DIST) {
+    if (c->P.size() < 2) return fail(c, "max_particle_dist needs two particles");
+    const P3 a = c->x[c->P[0].center_node], b = c->x[c->P[1].center_node];
+    *value = std::sqrt((a.x - b.x) * (a.x - b.x) + (a.y - b.y) * (a.y - b.y) + (a.z - b.z) * (a.z - b.z));
+  } else if (criterion == PDEM_STOP_MAX_NODE_DIST) {
+    double m = -1.;
+    int64_t k = 0;
+    for (int64_t i = 0; i < c->n; i++) {
+      const P3 p = c->x[i];
+      const double l = std::sqrt(p.x * p.x + p.y * p.y + p.z * p.z);
+      if (l > m) {
+        m = l;
+        k = i;
+      }
+    }
+    *value = m;
+    *node = k;
+  } else {
+    return fail(c, "unknown stop criterion");
+  }
+  *stop = isGreater(*value, param) ? 1 : 0;
+  return 0;
+}
+
 int oracle_broken_bonds(oracle_ctx *c, int64_t *broken, int64_t *total) {
   int64_t b = 0;
   for (auto x : c->bits) b += x ? 1 : 0;

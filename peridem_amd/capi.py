@@ -27,6 +27,7 @@ FFN_CONSTANT, FFN_LINEAR, FFN_LINEAR_STEP, FFN_LINEAR_SLOW_FAST, FFN_SIN = range
 
 FIELD_X, FIELD_U, FIELD_V, FIELD_F, FIELD_XREF, FIELD_VOL, FIELD_Z, FIELD_THETA, \
     FIELD_MX, FIELD_VMAG, FIELD_BONDS = range(11)
+STOP_MAX_PARTICLE_DIST, STOP_MAX_NODE_DIST = 1, 2
 NEIGH_PD = 0
 NEIGH_CONTACT = 1
 
@@ -196,6 +197,8 @@ def declare(lib: C.CDLL, prefix: str, extra_create_int: bool = False):
       [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)])
     f("broken_bonds", C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
     f("pp_two_particle_test", C.c_int, [vp, C.POINTER(C.c_double)])
+    f("check_stop", C.c_int, [vp, C.c_int32, C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                              C.POINTER(C.c_int64)])
     return lib
 
 
@@ -308,6 +311,14 @@ class Engine:
         out = (C.c_double * 5)()
         self._check(self._f("pp_two_particle_test")(self.ctx, out), "pp_two_particle_test")
         return dict(zip(("pen_dist", "contact_area_radius", "max_y", "max_dist", "max_y_loc"), list(out)))
+
+    def check_stop(self, criterion, param):
+        """(stop, value, node) of a checkStop criterion (STOP_MAX_PARTICLE_DIST
+        or STOP_MAX_NODE_DIST)."""
+        st, v, nd = C.c_int32(), C.c_double(), C.c_int64()
+        self._check(self._f("check_stop")(self.ctx, criterion, param, C.byref(st), C.byref(v), C.byref(nd)),
+                    "check_stop")
+        return bool(st.value), v.value, nd.value
 
     def pp_compressive_test(self, wall_id, f_dir):
         pen = C.c_double()

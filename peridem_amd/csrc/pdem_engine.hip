@@ -1336,6 +1336,42 @@ int pdem_pp_two_particle_test(pdem_ctx* c, double* out5) {
   });
 }
 
+// util::isGreater (function.cpp:15-23) on the host
+static bool host_is_greater(double a, double b) {
+  const double fa = std::fabs(a), fb = std::fabs(b);
+  return (a - b) > ((fa < fb ? fb : fa) * COMPARE_EPS);
+}
+
+int pdem_check_stop(pdem_ctx* c, int32_t criterion, double param, int32_t* stop, double* value, int64_t* node) {
+  return guarded(c, [&] {
+    if (!stop || !value || !node) throw HipError{"null output"};
+    if (!c->pp_out) c->pp_out = c->dalloc<double>(8);
+    *node = -1;
+    if (criterion == PDEM_STOP_MAX_PARTICLE_DIST) {
+      if (c->np < 2) throw HipError{"max_particle_dist needs two particles"};
+      hipLaunchKernelGGL(k_stop_center_dist, dim3(1), dim3(64), 0, c->stream, c->ds, c->pp_out);
+      check_launch(c);
+      HIPCHK(hipMemcpyAsync(value, c->pp_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    } else if (criterion == PDEM_STOP_MAX_NODE_DIST) {
+      unsigned long long init[2] = {0ull, ~0ull};
+      auto* u = (unsigned long long*)c->pp_out;
+      HIPCHK(hipMemcpyAsync(u, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+      hipLaunchKernelGGL(k_stop_maxlen, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds, u);
+      hipLaunchKernelGGL(k_stop_argmax, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds, u, u + 1);
+      check_launch(c);
+      unsigned long long h[2];
+      HIPCHK(hipMemcpyAsync(h, u, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      std::memcpy(value, &h[0], sizeof(double));
+      *node = (int64_t)h[1];
+    } else {
+      throw HipError{"unknown stop criterion"};
+    }
+    *stop = host_is_greater(*value, param) ? 1 : 0;
+  });
+}
+
 int pdem_broken_bonds(pdem_ctx* c, int64_t* broken, int64_t* total) {
   return guarded(c, [&] {
     unsigned long long zero = 0;

@@ -1609,6 +1609,46 @@ __global__ void k_pp_two_particle(DS s, double* out) {
   out[4] = max_y_loc;
 }
 
+// DEMModel::checkStop (demModel.cpp:2151-2200) quantities.
+// max_node_dist: |x| of every node (util::Point::length), its maximum
+// (util::methods::maxLengthAndMaxLengthIndex: the first node holding it).
+__device__ __forceinline__ double node_length(const DS& s, int64_t i) {
+  const double x = s.x[0][i], y = s.x[1] ? s.x[1][i] : 0., z = s.x[2] ? s.x[2][i] : 0.;
+  return sqrt(x * x + y * y + z * z);
+}
+__global__ void k_stop_maxlen(DS s, unsigned long long* out) {
+  __shared__ unsigned long long red[256 / WAVE];
+  const int64_t i = gid();
+  unsigned long long b = i < s.N ? (unsigned long long)__double_as_longlong(node_length(s, i)) : 0ull;
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(b, off);
+    b = o > b ? o : b;
+  }
+  if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = 0;
+    for (int k = 0; k < (int)(blockDim.x / WAVE); k++) m = red[k] > m ? red[k] : m;
+    atomicMax(out, m);
+  }
+}
+__global__ void k_stop_argmax(DS s, const unsigned long long* maxbits, unsigned long long* idx) {
+  const int64_t i = gid();
+  if (i < s.N && (unsigned long long)__double_as_longlong(node_length(s, i)) == *maxbits)
+    atomicMin(idx, (unsigned long long)i);
+}
+// max_particle_dist: distance of the first two particles' centre nodes
+__global__ void k_stop_center_dist(DS s, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int64_t a = s.P[0].center, b = s.P[1].center;
+  double d2 = 0.;
+  for (int d = 0; d < 3; d++) {
+    const double t = s.x[d] ? s.x[d][a] - s.x[d][b] : 0.;
+    d2 += t * t;
+  }
+  *out = sqrt(d2);
+}
+
 __global__ void k_count_broken(DS s) {
   int64_t i = gid();
   unsigned long long c = 0;

@@ -154,3 +154,18 @@ def test_two_particle_test_observables():
         for key in b:
             assert abs(a[key] - b[key]) <= 1e-12 * max(abs(b[key]), 1e-30), (key, a[key], b[key])
     assert a["pen_dist"] < 0 and a["contact_area_radius"] > 0
+
+
+def test_check_stop_quantities():
+    spec = configs.compressive_2d(n_particles=12, nodes_per_particle=120, seed=5, interval=10, wall_speed=2.0,
+                                  init_speed=1.0)
+    g = capi.GpuEngine(spec)
+    r = oracle.OracleEngine(spec, 8)
+    g.integrate_begin()
+    r.integrate_begin()
+    g.integrate_cd(150)
+    r.integrate_cd(150)
+    for crit, param in ((capi.STOP_MAX_PARTICLE_DIST, 1e-3), (capi.STOP_MAX_NODE_DIST, 1e-2)):
+        a, b = g.check_stop(crit, param), r.check_stop(crit, param)
+        print(f"[stop] criterion {crit}: gpu {a} oracle {b}", flush=True)
+        assert a == b

@@ -357,3 +357,20 @@ def test_two_particle_test_observables_closed_form():
     assert pp["max_dist"] == pytest.approx(x[p1["center_node"], 1] + R, rel=1e-15)
     ymax = x[p1["glob_start"]:p1["glob_end"], 1].max()
     assert pp["max_y_loc"] == pytest.approx(ymax, rel=1e-5) and pp["max_y"] == pp["max_y_loc"]
+
+
+def test_check_stop_quantities():
+    """checkStop: the two-particle centre distance is 2R + gap at rest; the
+    largest node distance from the origin and the first node holding it
+    agree with numpy; stop is the relative compare against the limit."""
+    R, gap = 1.0e-3, 1.0e-4
+    spec = configs.two_particle(R=R, gap=gap)
+    o = oracle.OracleEngine(spec, 4)
+    st, v, nd = o.check_stop(capi.STOP_MAX_PARTICLE_DIST, 2 * R)
+    assert v == pytest.approx(2 * R + gap, rel=1e-12) and st and nd == -1
+    assert not o.check_stop(capi.STOP_MAX_PARTICLE_DIST, 2 * R + gap)[0]
+    x = o.field(capi.FIELD_X)
+    ln = np.sqrt((x ** 2).sum(1))
+    st, v, nd = o.check_stop(capi.STOP_MAX_NODE_DIST, 1.0)
+    assert v == ln.max() and nd == int(np.argmax(ln)) and not st
+    assert o.check_stop(capi.STOP_MAX_NODE_DIST, 0.5 * ln.max())[0]
