@@ -245,6 +245,14 @@ int pdem_pp_two_particle_test(pdem_ctx *ctx, double *out5);
 #define PDEM_STOP_MAX_NODE_DIST 2
 int pdem_check_stop(pdem_ctx *ctx, int32_t criterion, double param, int32_t *stop, double *value,
                     int64_t *node);
+/* DEMModel::restart (demModel.cpp:84-103): step counter = step, time =
+ * step * dt, and the contact-search controller's restart rule
+ * (demModel.cpp:1833-1837). Positions, displacements and velocities from the
+ * restart file are set with pdem_set_field (as VtkParticleReader::readNodes
+ * does, vtkParticleReader.cpp:41-115); call before pdem_integrate_begin.
+ * Bond states and damage are not part of the reference's restart (see
+ * pdem_checkpoint_* for an exact one). */
+int pdem_restart(pdem_ctx *ctx, int64_t step);
 /* number of broken bonds and of all bonds (damage curve) */
 int pdem_broken_bonds(pdem_ctx *ctx, int64_t *broken, int64_t *total);
 

@@ -198,6 +198,8 @@ struct oracle_ctx {
   double maxVelocity = 0;
   std::vector<double> maxVelParticle;
   int initCallCount = 0;
+  bool restartActive = false;
+  int64_t restartStep = 0;
   int64_t nRebuilds = 0;
 
   Pool *pool = nullptr;
@@ -383,6 +385,8 @@ bool searchParameters(oracle_ctx *c) {
       return (c->contCounter - 1) % c->contInterval == 0;
     }
   }
+  // restart step (demModel.cpp:1833-1837)
+  if (c->restartActive && c->step == c->restartStep) c->contCounter = c->step % c->contInterval;
   if (c->contInterval == 1) {
     c->contRadius = c->maxContactR;
     c->contCounter++;
@@ -1112,6 +1116,18 @@ int oracle_check_stop(oracle_ctx *c, int32_t criterion, double param, int32_t *s
     return fail(c, "unknown stop criterion");
   }
   *stop = isGreater(*value, param) ? 1 : 0;
+  return 0;
+}
+
+// DEMModel::restart (demModel.cpp:84-103): the step counter and time of the
+// restart step; positions, displacements and velocities are set by the
+// caller (set_field) from the restart file
+int oracle_restart(oracle_ctx *c, int64_t step) {
+  if (step < 0) return fail(c, "negative restart step");
+  c->step = step;
+  c->time = double(step) * c->dt;
+  c->restartActive = true;
+  c->restartStep = step;
   return 0;
 }
 

@@ -197,6 +197,7 @@ def declare(lib: C.CDLL, prefix: str, extra_create_int: bool = False):
       [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)])
     f("broken_bonds", C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)])
     f("pp_two_particle_test", C.c_int, [vp, C.POINTER(C.c_double)])
+    f("restart", C.c_int, [vp, C.c_int64])
     f("check_stop", C.c_int, [vp, C.c_int32, C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_double),
                               C.POINTER(C.c_int64)])
     return lib
@@ -311,6 +312,10 @@ class Engine:
         out = (C.c_double * 5)()
         self._check(self._f("pp_two_particle_test")(self.ctx, out), "pp_two_particle_test")
         return dict(zip(("pen_dist", "contact_area_radius", "max_y", "max_dist", "max_y_loc"), list(out)))
+
+    def restart(self, step):
+        """DEMModel::restart: step counter, time and the controller's restart rule."""
+        self._check(self._f("restart")(self.ctx, int(step)), "restart")
 
     def check_stop(self, criterion, param):
         """(stop, value, node) of a checkStop criterion (STOP_MAX_PARTICLE_DIST

@@ -1372,6 +1372,19 @@ int pdem_check_stop(pdem_ctx* c, int32_t criterion, double param, int32_t* stop,
   });
 }
 
+int pdem_restart(pdem_ctx* c, int64_t step) {
+  return guarded(c, [&] {
+    if (step < 0) throw HipError{"negative restart step"};
+    Ctrl h = read_ctrl(c);
+    h.n = step;
+    h.time = double(step) * h.dt;
+    h.restart_active = 1;
+    h.restart_step = step;
+    HIPCHK(hipMemcpyAsync(c->ds.ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
 int pdem_broken_bonds(pdem_ctx* c, int64_t* broken, int64_t* total) {
   return guarded(c, [&] {
     unsigned long long zero = 0;

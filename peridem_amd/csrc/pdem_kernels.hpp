@@ -66,6 +66,9 @@ struct Ctrl {
   // scratch results
   double pp_pen, pp_reac;
   unsigned long long broken;
+  int64_t restart_step;   // DEMModel::restart: the restart step (restart_active)
+  int32_t restart_active;
+  int32_t pad2_;
 };
 
 // displacement-BC set parameters and the per-node entries of the sets that
@@ -333,6 +336,8 @@ __device__ void ctrl_decide(Ctrl* cg) {
     }
   } else {
     if (c->n == 0) c->init_calls++;
+    // restart step (demModel.cpp:1833-1837)
+    if (c->restart_active && c->n == c->restart_step) c->counter = c->n % c->interval;
     if (c->interval == 1) {
       c->radius = c->max_contact_r;
       c->counter++;

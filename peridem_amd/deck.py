@@ -5,6 +5,7 @@ src/inp/input.cpp:189-1395) plus its mesh files (gmsh .msh 2.x ASCII or
 .csv `id,x,y[,z],volume`) becomes the ModelSpec the engine takes, so decks
 written for the reference run unmodified:
 
+  Restart          File, Step (restart file resolved; applied by output.restart_from_vtu)
   Model            Dimension, Final_Time / Time_Steps -> dt, Particle_Sim_Type,
                    Discretization_Type.Time (central_difference / velocity_verlet)
   Zone             Zones, Zone_k.Is_Wall
@@ -401,6 +402,17 @@ def load_deck(path: str, name: Optional[str] = None) -> ModelSpec:
     spec.time_scheme = scheme
     # strain / stress post-processing: quadrature order (modelDeck.h:107, input.cpp:222-223)
     spec.info["quad_order"] = int(model.get("Quad_Approximation_Order", 1))
+    # restart (input.cpp:835-860): file and step, applied by output.restart_from_vtu
+    rs = cfg.get("Restart")
+    if rs is not None:
+        if "File" not in rs or "Step" not in rs:
+            raise ValueError("Restart needs File and Step")
+        f = str(rs["File"]).strip()
+        try:
+            f = _resolve(f, deck_dir)
+        except FileNotFoundError:
+            f = os.path.join(deck_dir, f)  # written later by an earlier run
+        spec.info["restart"] = {"file": f, "step": int(rs["Step"])}
     return spec
 
 

@@ -121,6 +121,26 @@ def write_strain_vtu(path, xq, strain, stress):
                  "</Piece>\n</UnstructuredGrid>\n</VTKFile>\n")
 
 
+def restart_from_vtu(eng, spec, path, step):
+    """DEMModel::restart (demModel.cpp:84-103) from a VTU written by
+    write_vtu or by the reference: current positions from the points,
+    Displacement (or x - xRef when absent) and Velocity (required), as
+    VtkParticleReader::readNodes (vtkParticleReader.cpp:41-115) reads them;
+    then the step counter, time and the controller's restart rule.  Call
+    before integrate_begin."""
+    pts, data = read_vtu(path)
+    if len(pts) != spec.n_nodes:
+        raise ValueError(f"{path}: {len(pts)} points, the model has {spec.n_nodes} nodes")
+    if "Velocity" not in data:
+        raise ValueError(f"{path}: restart file does not have Velocity data")
+    x = np.asarray(pts, np.float64).reshape(-1, 3)
+    u = np.asarray(data["Displacement"], np.float64).reshape(-1, 3) if "Displacement" in data else x - spec.x_ref
+    eng.set_field(capi.FIELD_X, x)
+    eng.set_field(capi.FIELD_U, u)
+    eng.set_field(capi.FIELD_V, np.asarray(data["Velocity"], np.float64).reshape(-1, 3))
+    eng.restart(step)
+
+
 def write_compressive_csv(path, rows):
     """rows: iterable of (t, wall_penetration, tot_reaction_force)."""
     with open(path, "w") as fh:

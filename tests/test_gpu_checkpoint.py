@@ -41,3 +41,31 @@ def test_restore_continues_bit_for_bit(which):
     other = capi.GpuEngine(configs.two_particle(), capi.PREC_FP64)
     with pytest.raises(RuntimeError, match="checkpoint"):
         other.restore(blob)
+
+
+def test_restart_from_vtu_matches_oracle(tmp_path):
+    """DEMModel::restart from a VTU (positions, displacement, velocity; step
+    counter, time and the contact-search controller's restart rule): the GPU
+    and the oracle restarted from the same file at step 120 stay equal."""
+    from peridem_amd import output
+    import oracle
+    spec = configs.compressive_2d(n_particles=12, nodes_per_particle=120, seed=5, interval=10, wall_speed=2.0,
+                                  init_speed=1.0)
+    a = oracle.OracleEngine(spec, 8)
+    a.integrate_begin()
+    a.integrate_cd(120)
+    path = str(tmp_path / "output_3.vtu")
+    output.write_vtu(path, a, spec)
+    g = capi.GpuEngine(spec, capi.PREC_FP64)
+    r = oracle.OracleEngine(spec, 8)
+    for e in (g, r):
+        output.restart_from_vtu(e, spec, path, 120)
+        e.integrate_begin()
+        e.integrate_cd(200)
+    st_g, st_r = g.contact_state(), r.contact_state()
+    print(f"[restart] gpu {st_g} oracle {st_r}", flush=True)
+    for k in ("interval", "counter", "n_rebuilds"):
+        assert st_g[k] == st_r[k]
+    for fid in (capi.FIELD_F, capi.FIELD_U, capi.FIELD_V, capi.FIELD_X):
+        np.testing.assert_array_equal(g.field(fid), r.field(fid))
+    assert g.time() == r.time() and g.time()[0] == 320

@@ -2,6 +2,7 @@
 state: the arrays read back equal the state bit for bit (17 significant
 digits), with the reference's array names and component counts."""
 import numpy as np
+import pytest
 
 from peridem_amd import capi, configs, output
 import oracle
@@ -48,3 +49,45 @@ def test_strain_vtu_round_trip(tmp_path):
     np.testing.assert_array_equal(pts, xq)
     np.testing.assert_array_equal(data["Strain"].astype(np.float32), eps)
     np.testing.assert_array_equal(data["Stress"].astype(np.float32), sig)
+
+
+@pytest.mark.parametrize("with_bc", [False, True])
+def test_restart_from_vtu_continues(tmp_path, with_bc):
+    """DEMModel::restart semantics on the oracle: a single elastic particle
+    (no contact, no bond breaking) written to VTU at step 100 and restarted
+    from it continues as the uninterrupted run: bit for bit without a
+    time-dependent BC (the VTU numbers carry 17 significant digits); with a
+    linear displacement BC to rounding only, because the restart's time is
+    100 dt while the running time is a sum of 100 dt (as in the reference)."""
+    from peridem_amd.geometry import center_node, disk_mesh
+    from peridem_amd.model import BcDeck, ModelBuilder, ZoneDeck
+    from peridem_amd.deck import _Region
+    R = 1e-3
+    x, vol = disk_mesh(R, 1e-4)
+    b = ModelBuilder(2, 2e-8, [ZoneDeck(configs.pdelastic_material())], {}, multi_particle=False,
+                     gravity=(0.0, -9.81, 0.0))
+    b.add_particle(0, x, vol, center_node(x, (0, 0, 0), R), R)
+    b.ic.append(([0], (0.3, 0.0, 0.0)))
+    if with_bc:
+        b.ubc.append(BcDeck([], [1], capi.UFN_LINEAR, [-2.0],
+                            region=_Region("rectangle", [-R, -R, 0.0, -0.7 * R, R, 0.0], 2)))
+    spec = b.build()
+    a = oracle.OracleEngine(spec, 4)
+    a.integrate_begin()
+    a.integrate_cd(100)
+    path = str(tmp_path / "output_1.vtu")
+    output.write_vtu(path, a, spec)
+    a.integrate_cd(100)
+    r = oracle.OracleEngine(spec, 4)
+    output.restart_from_vtu(r, spec, path, 100)
+    n, t = r.time()
+    assert n == 100 and t == 100 * spec.dt
+    r.integrate_begin()
+    r.integrate_cd(100)
+    for fid in (capi.FIELD_X, capi.FIELD_U, capi.FIELD_V, capi.FIELD_F):
+        ga, gr = a.field(fid), r.field(fid)
+        if with_bc:
+            np.testing.assert_allclose(gr, ga, rtol=0, atol=1e-9 * np.abs(ga).max())
+        else:
+            np.testing.assert_array_equal(gr, ga)
+    assert np.abs(a.field(capi.FIELD_U)).max() > 0
