@@ -60,6 +60,10 @@ struct pdem_ctx {
   int mixed_tpb = 1024;    // mixed tile kernels' workgroup size (autotuned)
   int tile_nodes = TILE_NODES;
   std::vector<int> nbr_lo, nbr_hi;  // per-node neighbor id range (tile windows)
+  int* d_slot = nullptr;            // mixed thread slot -> node
+  int64_t* d_sp16 = nullptr;        // mixed 16-bit id slices, by slot
+  uint16_t* d_col16 = nullptr;
+  size_t col16_cap = 0;
   // learned pair-contact closure
   bool cl_ready = false;        // closure data allocated (pdem_closure_setup)
   ClosureArgs ca{};
@@ -88,7 +92,12 @@ struct pdem_ctx {
     if (n == 0) n = 1;
     HIPCHK(hipMalloc(&p, n * sizeof(T)));
     allocs.push_back(p);
-    if (zero) HIPCHK(hipMemset(p, 0, n * sizeof(T)));
+    // zeroed on the engine stream and waited for: the stream is non-blocking,
+    // so a null-stream memset would not be ordered with the stream's kernels
+    if (zero) {
+      HIPCHK(hipMemsetAsync(p, 0, n * sizeof(T), stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    }
     return (T*)p;
   }
   void dfree(void* p) {
@@ -530,6 +539,36 @@ void build_tiles(pdem_ctx* c, int64_t tile_max) {
       tiles.push_back(DevTile{(int)ts, (int)te, ws, we, p, (we - ws) <= LDS_NODES ? 1 : 0});
     }
   }
+  // thread slots: inside every tile, nodes by 8-bond block count (most
+  // first), then by id; the 16-bit id lists follow the slot order
+  const int64_t N = c->ds.N;
+  std::vector<int> slot(N);
+  for (int64_t q = 0; q < N; q++) slot[q] = (int)q;
+  for (const auto& t : tiles)
+    std::stable_sort(slot.begin() + t.ts, slot.begin() + t.te, [&](int a, int b) {
+      return (c->h_deg[a] + KB - 1) / KB > (c->h_deg[b] + KB - 1) / KB;
+    });
+  std::vector<int64_t> sp16(c->nslices + 1, 0);
+  for (int64_t sl = 0; sl < c->nslices; sl++) {
+    int w = 0;
+    for (int64_t q = sl * WAVE; q < std::min<int64_t>(N, (sl + 1) * WAVE); q++) w = std::max(w, c->h_deg[slot[q]]);
+    sp16[sl + 1] = sp16[sl] + (int64_t)((w + KB - 1) / KB) * KB * WAVE;
+  }
+  const size_t need = (size_t)sp16[c->nslices] + 8;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (need > c->col16_cap) {
+    if (c->d_col16) c->dfree(c->d_col16);
+    c->d_col16 = c->dalloc<uint16_t>(need);
+    c->col16_cap = need;
+  }
+  HIPCHK(hipMemcpy(c->d_slot, slot.data(), N * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_sp16, sp16.data(), sp16.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_col16, dim3(blocks(N)), dim3(TPB), 0, c->stream, c->ds, c->d_col16, c->d_sp16,
+                     (const int*)c->d_slot);
+  check_launch(c);
+  c->mixed.slot = c->d_slot;
+  c->mixed.col16 = c->d_col16;
+  c->mixed.sptr16 = c->d_sp16;
   std::stable_sort(tiles.begin(), tiles.end(), [](const DevTile& a, const DevTile& b) { return a.lds > b.lds; });
   if (tiles.size() > c->d_tiles_cap) throw HipError{"tile buffer too small"};
   c->ntiles = (int)tiles.size();
@@ -734,15 +773,8 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     for (auto& q : c->hp)
       if (q.glob_end - q.glob_start > 65536)
         throw HipError{"mixed precision needs particles of at most 65536 nodes"};
-    std::vector<int64_t> sp16(c->nslices + 1, 0);
-    for (int64_t sl = 0; sl < c->nslices; sl++) {
-      const int64_t w = (c->h_sptr[sl + 1] - c->h_sptr[sl]) / WAVE;
-      sp16[sl + 1] = sp16[sl] + ((w + KB - 1) / KB) * KB * WAVE;
-    }
-    int64_t* d_sp16 = c->upload(sp16.data(), sp16.size());
-    uint16_t* col16 = c->dalloc<uint16_t>(sp16[c->nslices] + 8);
-    hipLaunchKernelGGL(k_col16, dim3(blocks(N)), dim3(TPB), 0, c->stream, s, col16, d_sp16);
-    c->mixed.sptr16 = d_sp16;
+    c->d_slot = c->dalloc<int>(N, false);
+    c->d_sp16 = c->dalloc<int64_t>(c->nslices + 1, false);
     int* dlo = c->dalloc<int>(N);
     int* dhi = c->dalloc<int>(N);
     hipLaunchKernelGGL(k_nbr_range, dim3(blocks(N)), dim3(TPB), 0, c->stream, s, dlo, dhi);
@@ -759,7 +791,6 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     c->d_tiles = c->dalloc<DevTile>(maxt);
     c->d_tiles_cap = maxt;
     build_tiles(c, TILE_NODES);
-    c->mixed.col16 = col16;
     c->mixed.fa = c->dalloc<float>(N);
     c->mixed.fb = c->dalloc<float>(N);
   }
@@ -1577,6 +1608,7 @@ void closure_setup(pdem_ctx* c, const float* particle_code) {
   if (nn == 0) throw HipError{"the closure needs non-wall particles"};
   std::vector<float> code(nn, 0.f);
   for (int q = 0; q < nn; q++) code[q] = particle_code ? particle_code[nonwall[q]] : 0.f;
+  HIPCHK(hipStreamSynchronize(c->stream));  // steps in flight may read the codes
   if (c->cl_ready) {
     HIPCHK(hipMemcpy((void*)c->ca.code, code.data(), nn * sizeof(float), hipMemcpyHostToDevice));
     return;

@@ -270,15 +270,19 @@ __global__ void k_nbr_range(DS s, int* lo, int* hi) {
 }
 
 // neighbor ids relative to the particle's first node, 16 bit, in blocks of
-// 8 per lane (layout of the mixed-precision kernels)
-__global__ void k_col16(DS s, uint16_t* col16, const int64_t* sptr16) {
-  int64_t i = gid();
-  if (i >= s.N) return;
+// 8 per lane (layout of the mixed-precision kernels), in thread-slot order:
+// slot q holds node slot[q], and slot slices of 64 are padded to their
+// largest degree
+__global__ void k_col16(DS s, uint16_t* col16, const int64_t* sptr16, const int* slot) {
+  int64_t q = gid();
+  if (q >= s.N) return;
+  const int64_t i = slot[q];
   const int64_t sl = i / WAVE, lane = i % WAVE;
   const int64_t gs = s.P[s.ptid[i]].glob_start;
+  const int64_t base = sptr16[q / WAVE] + (q % WAVE) * 8;
   for (int k = 0; k < s.deg[i]; k++) {
     const int64_t o = s.sptr[sl] + (int64_t)k * WAVE + lane;
-    col16[sptr16[sl] + (int64_t)(k / 8) * WAVE * 8 + lane * 8 + (k % 8)] = (uint16_t)(s.col[o] - gs);
+    col16[base + (int64_t)(k / 8) * WAVE * 8 + (k % 8)] = (uint16_t)(s.col[o] - gs);
   }
 }
 

@@ -17,6 +17,13 @@
 // blocks of KB = 8 per lane: one 16-byte load yields a lane's next 8 ids and
 // the block loop is unrolled so each id is a compile-time lane of the block.
 //
+// Threads take nodes in slot order: inside every tile the nodes are ordered
+// by their number of 8-bond blocks (most first, node id second), so the 64
+// lanes of a wavefront run lists of about the same length (a wavefront runs
+// as long as its longest list; surface nodes have far fewer bonds than
+// interior ones).  The 16-bit id lists are laid out by slot; node data,
+// bond bits and results stay indexed by node.
+//
 // FMA contraction is allowed inside these kernels only (the FP64 parity
 // kernels are built without it).
 //
@@ -29,9 +36,12 @@
 namespace pdem {
 
 constexpr int TILE_NODES = 1024;
-constexpr int LDS_NODES = 1280;   // 1280 x 48 B = 60 KB of LDS per workgroup
+constexpr int LDS_NODES = 1280;   // window of up to 1280 nodes, ~61 KB of LDS per workgroup
 constexpr int MIXED_TPB = 1024;   // default workgroup size (the autotuner may pick 256)
 constexpr int KB = 8;             // neighbor ids per 16-byte block
+// waves per SIMD the bond kernels are compiled for (at most 64 VGPRs): two
+// 1024-thread workgroups per CU, so one stages its window while the other computes
+constexpr int MIXED_WAVES_PER_SIMD = 8;
 
 struct DevTile {
   int ts, te;     // nodes [ts, te)
@@ -43,35 +53,54 @@ struct DevTile {
 struct MixedArgs {
   const DevTile* tiles;   // LDS tiles first, then HBM tiles
   int ntiles_lds, ntiles_hbm;
-  // col16[sptr16[slice] + (k / KB) * 64 * KB + lane * KB + k % KB]
+  // thread slot q runs node slot[q]; its k-th neighbor id is
+  // col16[sptr16[q / 64] + (k / KB) * 64 * KB + (q % 64) * KB + k % KB]
+  const int* slot;
   const uint16_t* col16;
   const int64_t* sptr16;
   float* fa;              // per node alpha = 15 G / m           (force pass)
   float* fb;              // per node theta ((3K/m) - alpha / 3)  (force pass)
 };
 
-__device__ __forceinline__ bool isGreaterF(float a, float b) {
-  float fa = fabsf(a), fb = fabsf(b);
-  return (a - b) > ((fa < fb ? fb : fa) * 1e-5f);
-}
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
-// hardware square root (1 ulp, no denormal rescaling): bond lengths squared
-// are far from the FP32 denormal range
+// hardware square root and reciprocal square root (1 ulp, no denormal
+// rescaling): bond lengths squared are far from the FP32 denormal range.
+// The reciprocal square root yields both |v| = |v|^2 rsq(|v|^2) and 1 / |v|
+// from one transcendental (a quarter-rate instruction).
 __device__ __forceinline__ float sqrtf_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float rsqf_hw(float x) { return __builtin_amdgcn_rsqf(x); }
+// isGreater(a, b) with a, b >= 0 (util::isGreater, function.h, slack 1e-5 of
+// the larger operand) reduces to a (1 - 1e-5) > b: one compare against a
+// threshold scaled once per particle.
+constexpr float REL_KEEP = 1.f - 1e-5f;
 
+// Influence function J(r) in FP32 (util::ConstInfluenceFn, LinearInfluenceFn,
+// GaussianInfluenceFn of influenceFn.h, argument r / delta) with its
+// constants folded once per particle: J = a, a + b r (b = p1 / delta) or
+// a exp(-b |xi|^2) (b = 1 / (p1 delta^2)).
 template <int INF>
-__device__ __forceinline__ float infJf(const DS& s, float r_over_delta) {
-  if (INF == 0) return (float)s.inf_p0;
-  if (INF == 1) return (float)s.inf_p0 + (float)s.inf_p1 * r_over_delta;
-  return (float)s.inf_p0 * __expf(-r_over_delta * r_over_delta / (float)s.inf_p1);
-}
+struct InfF {
+  float a, b;
+  __device__ InfF(const DS& s, float inv_delta)
+      : a((float)s.inf_p0),
+        b(INF == 1 ? (float)s.inf_p1 * inv_delta
+                   : INF == 2 ? inv_delta * inv_delta / (float)s.inf_p1 : 0.f) {}
+  __device__ __forceinline__ float operator()(float r, float xx) const {
+    if (INF == 0) return a;
+    if (INF == 1) return fmaf(b, r, a);
+    return a * __expf(-b * xx);
+  }
+};
 
 __device__ __forceinline__ int block_id(const uint4& b, int t) {
   const unsigned w = t < 2 ? b.x : t < 4 ? b.y : t < 6 ? b.z : b.w;
   return (int)((w >> ((t & 1) * 16)) & 0xffffu);
 }
 
-// node data access: LDS window or HBM
+// node data access by particle-relative id jp (node gs + jp): LDS window or HBM.
+// LDS arrays of the three components are LDS_ST apart (LDS_NODES plus 16
+// entries), which shifts a component's banks against the next one's.
+constexpr int LDS_ST = LDS_NODES + 16;
 template <int DIM, bool LDS>
 struct Win {
   const float* lx;
@@ -81,17 +110,18 @@ struct Win {
   const float* lb;
   const DS* s;
   const MixedArgs* m;
-  int ws;
+  int gs;   // particle's first node
+  int off;  // gs - window start: LDS index of jp
   double org[3];
-  __device__ __forceinline__ float x(int d, int j) const {
-    return LDS ? lx[d * LDS_NODES + (j - ws)] : (float)(s->xr[d][j] - org[d]);
+  __device__ __forceinline__ float x(int d, int jp) const {
+    return LDS ? lx[d * LDS_ST + jp + off] : (float)(s->xr[d][gs + jp] - org[d]);
   }
-  __device__ __forceinline__ double u(int d, int j) const {
-    return LDS ? lu[d * LDS_NODES + (j - ws)] : s->u[d][j];
+  __device__ __forceinline__ double u(int d, int jp) const {
+    return LDS ? lu[d * LDS_ST + jp + off] : s->u[d][gs + jp];
   }
-  __device__ __forceinline__ float v(int j) const { return LDS ? lv[j - ws] : (float)s->vol[j]; }
-  __device__ __forceinline__ float a(int j) const { return LDS ? la[j - ws] : m->fa[j]; }
-  __device__ __forceinline__ float b(int j) const { return LDS ? lb[j - ws] : m->fb[j]; }
+  __device__ __forceinline__ float v(int jp) const { return LDS ? lv[jp + off] : (float)s->vol[gs + jp]; }
+  __device__ __forceinline__ float a(int jp) const { return LDS ? la[jp + off] : m->fa[gs + jp]; }
+  __device__ __forceinline__ float b(int jp) const { return LDS ? lb[jp + off] : m->fb[gs + jp]; }
 };
 
 template <int DIM, bool FORCE>
@@ -102,8 +132,8 @@ __device__ __forceinline__ void stage_window(const DS& s, const MixedArgs& m, co
   for (int q = threadIdx.x; q < nw; q += blockDim.x) {
     const int j = t.ws + q;
     for (int d = 0; d < DIM; d++) {
-      lx[d * LDS_NODES + q] = (float)(s.xr[d][j] - org[d]);
-      lu[d * LDS_NODES + q] = s.u[d][j];
+      lx[d * LDS_ST + q] = (float)(s.xr[d][j] - org[d]);
+      lu[d * LDS_ST + q] = s.u[d][j];
     }
     lv[q] = (float)s.vol[j];
     if (FORCE) {
@@ -116,32 +146,35 @@ __device__ __forceinline__ void stage_window(const DS& s, const MixedArgs& m, co
 
 // Dilation pass (demModel.cpp:610-679) in mixed precision.
 template <int DIM, int INF, bool LDS, int TPB>
-__global__ void __launch_bounds__(TPB) k_mixed_dilation(DS s, MixedArgs m) {
+__global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_dilation(DS s, MixedArgs m) {
 #pragma clang fp contract(fast)
-  __shared__ float lx[LDS ? 3 * LDS_NODES : 1];
-  __shared__ double lu[LDS ? 3 * LDS_NODES : 1];
+  __shared__ float lx[LDS ? 3 * LDS_ST : 1];
+  __shared__ double lu[LDS ? 3 * LDS_ST : 1];
   __shared__ float lv[LDS ? LDS_NODES : 1];
   const DevTile t = m.tiles[LDS ? blockIdx.x : m.ntiles_lds + blockIdx.x];
   const DevParticle& p = s.P[t.pid];
-  Win<DIM, LDS> w{lx, lu, lv, nullptr, nullptr, &s, &m, t.ws, {0., 0., 0.}};
-  for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][p.glob_start];
+  const int gs = (int)p.glob_start;
+  Win<DIM, LDS> w{lx, lu, lv, nullptr, nullptr, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
+  for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][gs];
   if (LDS) stage_window<DIM, false>(s, m, t, lx, lu, lv, nullptr, nullptr, w.org);
   const float horizon = (float)p.horizon, h = (float)p.h;
   const float check_up = horizon + 0.5f * h, check_low = horizon - 0.5f * h;
-  const float thr = (float)(p.mat[2] + 1.0e-10);
-  const float inv_h = rcpf(h), inv_delta = rcpf(horizon);
+  const float low_rel = check_low / REL_KEEP;             // r > low_rel: isGreater(r, check_low)
+  const float thr_rel = (float)(p.mat[2] + 1.0e-10) / REL_KEEP;  // |S| > thr_rel: breaks
+  const float inv_h = rcpf(h), inv_delta = rcpf(horizon), up_h = check_up * inv_h;
+  const InfF<INF> J(s, inv_delta);
   const double K = p.mat[0], G = p.mat[1];
-  const int gs = (int)p.glob_start;
-  for (int i = t.ts + threadIdx.x; i < t.te; i += blockDim.x) {
-    const int64_t sl = i / WAVE, lane = i % WAVE;
-    const uint4* cb = (const uint4*)(m.col16 + m.sptr16[sl] + lane * KB);
-    unsigned* bw = s.bits + s.bptr[sl] + lane;
+  for (int q = t.ts + threadIdx.x; q < t.te; q += blockDim.x) {
+    const int i = m.slot[q];
+    const int ip = i - gs;
+    const uint4* cb = (const uint4*)(m.col16 + m.sptr16[q / WAVE] + (q % WAVE) * KB);
+    unsigned* bw = s.bits + s.bptr[i / WAVE] + i % WAVE;
     const int n = s.deg[i];
     float xi[3];
     double ui[3];
     for (int d = 0; d < DIM; d++) {
-      xi[d] = w.x(d, i);
-      ui[d] = w.u(d, i);
+      xi[d] = w.x(d, ip);
+      ui[d] = w.u(d, ip);
     }
     double theta = 0.;
     unsigned word = 0, wnew = 0;
@@ -154,40 +187,40 @@ __global__ void __launch_bounds__(TPB) k_mixed_dilation(DS s, MixedArgs m) {
       }
       const uint4 cur = blk;
       if (kb + KB < n) blk = cb[(int64_t)(kb / KB + 1) * WAVE];
-#pragma unroll
-      for (int tt = 0; tt < KB; tt++) {
+      auto bond = [&](const int tt) {
         const int k = kb + tt;
         if (k < n) {
-          const int j = gs + block_id(cur, tt);
+          const int jp = block_id(cur, tt);
           float dx[3], du[3];
           for (int d = 0; d < DIM; d++) {
-            dx[d] = w.x(d, j) - xi[d];
-            du[d] = (float)(w.u(d, j) - ui[d]);
+            dx[d] = w.x(d, jp) - xi[d];
+            du[d] = (float)(w.u(d, jp) - ui[d]);
           }
-          float volj = w.v(j);
-          float xx = 0.f, xe = 0.f, ee = 0.f, yy = 0.f;
-          for (int d = 0; d < DIM; d++) {
+          float volj = w.v(jp);
+          float xx = dx[0] * dx[0], xe = dx[0] * du[0], ee = du[0] * du[0];
+          for (int d = 1; d < DIM; d++) {
             xx += dx[d] * dx[d];
             xe += dx[d] * du[d];
             ee += du[d] * du[d];
-            const float y = dx[d] + du[d];
-            yy += y * y;
           }
-          const float r = sqrtf_hw(xx);
-          const float cl = (2.f * xe + ee) * rcpf(sqrtf_hw(yy) + r);
-          const float st = cl * rcpf(r);
+          const float t2 = 2.f * xe + ee;  // |xi + eta|^2 - |xi|^2
+          const float inv_r = rsqf_hw(xx);
+          const float r = xx * inv_r;
+          const float cl = t2 * rcpf(sqrtf_hw(xx + t2) + r);
           const unsigned bit = 1u << (k & 31);
           bool fs = (word & bit) != 0u;
-          if (!fs && isGreaterF(fabsf(st), thr)) {
+          if (!fs && fabsf(cl) * inv_r > thr_rel) {
             fs = true;
             wnew |= bit;
           }
           if (!fs) {
-            if (isGreaterF(r, check_low)) volj *= (check_up - r) * inv_h;
-            theta += (double)(r * cl * infJf<INF>(s, r * inv_delta) * volj);
+            if (r > low_rel) volj *= up_h - r * inv_h;
+            theta += (double)(r * cl * J(r, xx) * volj);
           }
         }
-      }
+      };
+#pragma unroll
+      for (int tt = 0; tt < KB; tt++) bond(tt);
     }
     if (n > 0 && wnew != word) bw[(int64_t)((n - 1) >> 5) * WAVE] = wnew;
     const double mi = s.mx[i];
@@ -201,38 +234,45 @@ __global__ void __launch_bounds__(TPB) k_mixed_dilation(DS s, MixedArgs m) {
 
 // Force pass (demModel.cpp:682-788, PdState::getBondEF material.h:1415-1429)
 // in mixed precision: ef_i + ef_j = J (r (b_i + b_j) + cl (a_i + a_j)).
+// The damage Z_i is the plain maximum of |S| / s0 over the list (the
+// reference's running compare with a 1e-5 relative slack keeps a value at
+// most 1e-5 below it; FP32 bond arithmetic is the larger difference).
 template <int DIM, int INF, bool LDS, int TPB>
-__global__ void __launch_bounds__(TPB) k_mixed_force(DS s, MixedArgs m) {
+__global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_force(DS s, MixedArgs m) {
 #pragma clang fp contract(fast)
-  __shared__ float lx[LDS ? 3 * LDS_NODES : 1];
-  __shared__ double lu[LDS ? 3 * LDS_NODES : 1];
+  __shared__ float lx[LDS ? 3 * LDS_ST : 1];
+  __shared__ double lu[LDS ? 3 * LDS_ST : 1];
   __shared__ float lv[LDS ? LDS_NODES : 1];
   __shared__ float la[LDS ? LDS_NODES : 1];
   __shared__ float lb[LDS ? LDS_NODES : 1];
   const DevTile t = m.tiles[LDS ? blockIdx.x : m.ntiles_lds + blockIdx.x];
   const DevParticle& p = s.P[t.pid];
-  Win<DIM, LDS> w{lx, lu, lv, la, lb, &s, &m, t.ws, {0., 0., 0.}};
-  for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][p.glob_start];
+  const int gs = (int)p.glob_start;
+  Win<DIM, LDS> w{lx, lu, lv, la, lb, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
+  for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][gs];
   if (LDS) stage_window<DIM, true>(s, m, t, lx, lu, lv, la, lb, w.org);
   const float horizon = (float)p.horizon, h = (float)p.h;
   const float check_up = horizon + 0.5f * h, check_low = horizon - 0.5f * h;
+  const float low_rel = check_low / REL_KEEP;  // r > low_rel: isGreater(r, check_low)
   const float inv_h = rcpf(h), inv_delta = rcpf(horizon), inv_s0 = (float)(1.0 / p.mat[2]);
+  const float up_h = check_up * inv_h;
+  const InfF<INF> J(s, inv_delta);
   const float Kn = (float)p.Kn, Rc = (float)p.Rc;
-  const int gs = (int)p.glob_start;
-  for (int i = t.ts + threadIdx.x; i < t.te; i += blockDim.x) {
-    const int64_t sl = i / WAVE, lane = i % WAVE;
-    const uint4* cb = (const uint4*)(m.col16 + m.sptr16[sl] + lane * KB);
-    const unsigned* bw = s.bits + s.bptr[sl] + lane;
+  for (int q = t.ts + threadIdx.x; q < t.te; q += blockDim.x) {
+    const int i = m.slot[q];
+    const int ip = i - gs;
+    const uint4* cb = (const uint4*)(m.col16 + m.sptr16[q / WAVE] + (q % WAVE) * KB);
+    const unsigned* bw = s.bits + s.bptr[i / WAVE] + i % WAVE;
     const int n = s.deg[i];
     float xi[3];
     double ui[3];
     for (int d = 0; d < DIM; d++) {
-      xi[d] = w.x(d, i);
-      ui[d] = w.u(d, i);
+      xi[d] = w.x(d, ip);
+      ui[d] = w.u(d, ip);
     }
-    const float ai = w.a(i), bi = w.b(i);
+    const float ai = w.a(ip), bi = w.b(ip);
     double force[3] = {0., 0., 0.};
-    float Zi = 0.f;
+    float Zs = 0.f;  // maximum of |S|; Z = Zs / s0
     unsigned word = 0;
     uint4 blk = n > 0 ? cb[0] : make_uint4(0, 0, 0, 0);
     for (int kb = 0; kb < n; kb += KB) {
@@ -240,49 +280,48 @@ __global__ void __launch_bounds__(TPB) k_mixed_force(DS s, MixedArgs m) {
       const uint4 cur = blk;
       if (kb + KB < n) blk = cb[(int64_t)(kb / KB + 1) * WAVE];
       float fblk[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-      for (int tt = 0; tt < KB; tt++) {
+      auto bond = [&](const int tt) {
         const int k = kb + tt;
         if (k < n) {
-          const int j = gs + block_id(cur, tt);
+          const int jp = block_id(cur, tt);
           float dx[3], du[3], y[3];
           for (int d = 0; d < DIM; d++) {
-            dx[d] = w.x(d, j) - xi[d];
-            du[d] = (float)(w.u(d, j) - ui[d]);
+            dx[d] = w.x(d, jp) - xi[d];
+            du[d] = (float)(w.u(d, jp) - ui[d]);
             y[d] = dx[d] + du[d];
           }
-          float volj = w.v(j);
-          float xx = 0.f, xe = 0.f, ee = 0.f, yy = 0.f;
-          for (int d = 0; d < DIM; d++) {
+          float volj = w.v(jp);
+          float xx = dx[0] * dx[0], xe = dx[0] * du[0], ee = du[0] * du[0];
+          for (int d = 1; d < DIM; d++) {
             xx += dx[d] * dx[d];
             xe += dx[d] * du[d];
             ee += du[d] * du[d];
-            yy += y[d] * y[d];
           }
-          const float r = sqrtf_hw(xx);
-          const float ly = sqrtf_hw(yy);
-          const float inv_ly = rcpf(ly);
-          const float cl = (2.f * xe + ee) * rcpf(ly + r);
-          const float S = cl * rcpf(r);
-          const bool fs = (word >> (k & 31)) & 1u;
+          const float t2 = 2.f * xe + ee;  // |y|^2 - |xi|^2
+          const float yy = xx + t2;
+          const float inv_r = rsqf_hw(xx), inv_ly = rsqf_hw(yy);
+          const float r = xx * inv_r, ly = yy * inv_ly;
+          const float cl = t2 * rcpf(ly + r);
+          const bool fs = (word & (1u << (k & 31))) != 0u;
           float sf;
           if (!fs) {
-            if (isGreaterF(r, check_low)) volj *= (check_up - r) * inv_h;
-            const float J = infJf<INF>(s, r * inv_delta);
-            sf = J * (r * (bi + w.b(j)) + cl * (ai + w.a(j))) * volj * inv_ly;
+            if (r > low_rel) volj *= up_h - r * inv_h;
+            const float Jr = J(r, xx);
+            sf = Jr * (r * (bi + w.b(jp)) + cl * (ai + w.a(jp))) * volj * inv_ly;
           } else {
             sf = Kn * volj * (ly - Rc) * inv_ly;
             if (sf > 0.f) sf = 0.f;
           }
           for (int d = 0; d < DIM; d++) fblk[d] += sf * y[d];
-          const float z = fabsf(S * inv_s0);
-          if (isGreaterF(z, Zi)) Zi = z;
+          Zs = fmaxf(Zs, fabsf(cl) * inv_r);
         }
-      }
+      };
+#pragma unroll
+      for (int tt = 0; tt < KB; tt++) bond(tt);
       for (int d = 0; d < DIM; d++) force[d] += (double)fblk[d];
     }
     for (int d = 0; d < DIM; d++) s.f[d][i] = force[d];
-    s.Z[i] = Zi;
+    s.Z[i] = Zs * inv_s0;
   }
 }
 

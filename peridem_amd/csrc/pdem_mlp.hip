@@ -213,11 +213,13 @@ int pdem_mlp_create(uint64_t seed, pdem_mlp** out) {
     HIPCHK(hipMalloc(&m->wb, mlp::NBF * sizeof(short)));
     HIPCHK(hipMalloc(&m->wtb, mlp::NBF * sizeof(short)));
     HIPCHK(hipMalloc(&m->norm, mlp::NNORM * sizeof(float)));
-    HIPCHK(hipMemset(m->m, 0, mlp::NPARAM * sizeof(float)));
-    HIPCHK(hipMemset(m->v, 0, mlp::NPARAM * sizeof(float)));
+    // zeroed on the network's (non-blocking) stream, so that its kernels see it
+    HIPCHK(hipMemsetAsync(m->m, 0, mlp::NPARAM * sizeof(float), m->stream));
+    HIPCHK(hipMemsetAsync(m->v, 0, mlp::NPARAM * sizeof(float), m->stream));
     // padded rows/columns of the last layer's BF16 copies stay zero
-    HIPCHK(hipMemset(m->wb, 0, mlp::NBF * sizeof(short)));
-    HIPCHK(hipMemset(m->wtb, 0, mlp::NBF * sizeof(short)));
+    HIPCHK(hipMemsetAsync(m->wb, 0, mlp::NBF * sizeof(short), m->stream));
+    HIPCHK(hipMemsetAsync(m->wtb, 0, mlp::NBF * sizeof(short), m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
     // He-uniform hidden layers, a small output layer, zero biases
     std::vector<float> p(mlp::NPARAM, 0.f);
     std::mt19937_64 rng(seed);
@@ -268,8 +270,8 @@ int pdem_mlp_set_params(pdem_mlp* m, const float* params) {
   return mguard(m, [&] {
     HIPCHK(hipStreamSynchronize(m->stream));
     HIPCHK(hipMemcpy(m->p, params, mlp::NPARAM * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(m->m, 0, mlp::NPARAM * sizeof(float)));
-    HIPCHK(hipMemset(m->v, 0, mlp::NPARAM * sizeof(float)));
+    HIPCHK(hipMemsetAsync(m->m, 0, mlp::NPARAM * sizeof(float), m->stream));
+    HIPCHK(hipMemsetAsync(m->v, 0, mlp::NPARAM * sizeof(float), m->stream));
     m->adam_step = 0;
     pack(m);
     HIPCHK(hipStreamSynchronize(m->stream));
