@@ -58,8 +58,9 @@ def pd_bytes_per_launch(spec, eng, precision):
                                                            list entry of each pair; the
                                                            per-pair scratch it writes and
                                                            reads back in L2 is not counted)
-        mixed state force kernel  24 d + 24 + 2.125 N_i  (2-byte ids; alpha, b terms
-                                                           FP32 instead of m_x, theta)
+        mixed state force kernel  24 d + 28 + 2.125 N_i  (2-byte ids; alpha, b terms
+                                                           FP32 instead of m_x, theta; the
+                                                           4-byte thread slot -> node entry)
     """
     import numpy as np
     from peridem_amd import capi
@@ -73,7 +74,7 @@ def pd_bytes_per_launch(spec, eng, precision):
     st = state[cf]
     canon = np.where(st, 24 * d + 28 + 4.125 * nb, 24 * d + 12 + 4.25 * nb)
     if precision == capi.PREC_MIXED:
-        layout = np.where(st, 24 * d + 24 + 2.125 * nb, 24 * d + 16 + 4.25 * nb)
+        layout = np.where(st, 24 * d + 28 + 2.125 * nb, 24 * d + 16 + 4.25 * nb)
     else:
         rnp = 24 * d + 16 + (8.25 if eng.launch_config()[0] == 512 else 4.25) * nb
         layout = np.where(st, 24 * d + 32 + 4.125 * nb, rnp)
@@ -222,11 +223,14 @@ def main():
     print(f"# workload {spec.name}: {spec.n_nodes} nodes, {spec.n_particles} particles, "
           f"setup {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
     eng.integrate_begin()
-    tuned = None
+    tms = None
     if not args.no_autotune:
-        tms = eng.autotune(reps=5)
-        tuned = dict(zip(("pd_tpb", "mixed_tpb", "tile_nodes"), eng.launch_config()), force_pass_ms=round(tms, 5))
-        print(f"# autotune: {tuned}", file=sys.stderr, flush=True)
+        tms = round(eng.autotune(reps=5), 5)
+    tuned = dict(zip(("pd_tpb", "mixed_tpb", "tile_nodes"), eng.launch_config()), force_pass_ms=tms,
+                 autotuned=not args.no_autotune)
+    if args.precision == "mixed":
+        tuned["tiles_lds"], tuned["tiles_hbm"] = eng.tile_counts()
+    print(f"# launch: {tuned}", file=sys.stderr, flush=True)
     eng.integrate_cd_graph(args.warmup)
     eng.synchronize()
     t0 = time.perf_counter()

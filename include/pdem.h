@@ -295,6 +295,12 @@ int pdem_profile_get(pdem_ctx *ctx, double *pd_kernel_ms_avg, int64_t *pd_kernel
 int pdem_autotune(pdem_ctx *ctx, int32_t reps, double *best_ms);
 int pdem_get_launch_config(pdem_ctx *ctx, int32_t *pd_tpb, int32_t *mixed_tpb, int32_t *tile_nodes);
 int pdem_set_launch_config(pdem_ctx *ctx, int32_t pd_tpb, int32_t mixed_tpb, int32_t tile_nodes);
+
+/* Tiles of the mixed-precision bond kernels under the current launch shape:
+ * tiles whose neighbor window is staged in LDS and tiles that gather from
+ * HBM (0 and 0 for FP64 models). No counterpart in the reference (launch
+ * shape introspection for tuning and tests). */
+int pdem_get_tile_counts(pdem_ctx *ctx, int32_t *lds_tiles, int32_t *hbm_tiles);
 /* Step fusion for multi-particle models of up to 2^20 nodes (on by default):
  * the node update also applies the displacement BC and packs the particle
  * centres, the controller runs as one workgroup and the damping pass

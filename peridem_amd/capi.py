@@ -401,6 +401,8 @@ def engine_library():
         lib.pdem_get_launch_config.argtypes = [C.c_void_p] + [C.POINTER(C.c_int32)] * 3
         lib.pdem_set_launch_config.restype = C.c_int
         lib.pdem_set_launch_config.argtypes = [C.c_void_p] + [C.c_int32] * 3
+        lib.pdem_get_tile_counts.restype = C.c_int
+        lib.pdem_get_tile_counts.argtypes = [C.c_void_p] + [C.POINTER(C.c_int32)] * 2
         lib.pdem_set_step_fusion.restype = C.c_int
         lib.pdem_set_step_fusion.argtypes = [C.c_void_p, C.c_int32]
         _engine_lib = lib
@@ -492,6 +494,12 @@ class GpuEngine(Engine):
     def set_launch_config(self, pd_tpb, mixed_tpb=1024, tile_nodes=1024):
         """Apply a launch shape (pd_tpb 512: the RNP pair-tile kernel)."""
         self._check(self.lib.pdem_set_launch_config(self.ctx, pd_tpb, mixed_tpb, tile_nodes), "set_launch_config")
+
+    def tile_counts(self):
+        """(tiles staged in LDS, tiles gathering from HBM) of the mixed bond kernels."""
+        v = [C.c_int32() for _ in range(2)]
+        self._check(self.lib.pdem_get_tile_counts(self.ctx, *[C.byref(x) for x in v]), "tile_counts")
+        return tuple(x.value for x in v)
 
     def set_step_fusion(self, on=True):
         """Fused step kernels for small multi-particle models (default on)."""

@@ -39,6 +39,10 @@ struct pdem_ctx {
   int64_t N = 0;
   int np = 0, nz = 0;
   hipStream_t stream = nullptr;
+  // side stream of the mixed bond passes: tiles that gather from HBM run on it
+  // beside the LDS tiles (forked from and joined back into `stream`)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   DS ds{};
   std::vector<void*> allocs;
   std::vector<pdem_particle> hp;
@@ -124,6 +128,10 @@ struct pdem_ctx {
     }
     if (stream) (void)hipStreamSynchronize(stream);
     for (void* p : allocs) (void)hipFree(p);
+    if (side) (void)hipStreamSynchronize(side);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -152,25 +160,38 @@ struct Launch {
     HIPCHK(hipEventRecord(ev.second, c->stream));
     c->ev_used.push_back(ev);
   }
+  // the side stream starts after everything enqueued so far on the engine stream
+  static void fork(pdem_ctx* c) {
+    HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  }
   template <int INF, int T>
   static void mixed_dilation(pdem_ctx* c) {
     const MixedArgs& ma = c->mixed;
+    if (ma.ntiles_hbm) {
+      fork(c);
+      hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, false, T>), dim3(ma.ntiles_hbm), dim3(T), 0, c->side,
+                         c->ds, ma);
+      HIPCHK(hipEventRecord(c->ev_join, c->side));
+    }
     if (ma.ntiles_lds)
       hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, true, T>), dim3(ma.ntiles_lds), dim3(T), 0, c->stream,
                          c->ds, ma);
-    if (ma.ntiles_hbm)
-      hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, false, T>), dim3(ma.ntiles_hbm), dim3(T), 0,
-                         c->stream, c->ds, ma);
+    if (ma.ntiles_hbm) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   }
   template <int INF, int T>
   static void mixed_force(pdem_ctx* c) {
     const MixedArgs& ma = c->mixed;
+    if (ma.ntiles_hbm) {
+      fork(c);
+      hipLaunchKernelGGL((k_mixed_force<DIM, INF, false, T>), dim3(ma.ntiles_hbm), dim3(T), 0, c->side,
+                         c->ds, ma);
+      HIPCHK(hipEventRecord(c->ev_join, c->side));
+    }
     if (ma.ntiles_lds)
       hipLaunchKernelGGL((k_mixed_force<DIM, INF, true, T>), dim3(ma.ntiles_lds), dim3(T), 0, c->stream,
                          c->ds, ma);
-    if (ma.ntiles_hbm)
-      hipLaunchKernelGGL((k_mixed_force<DIM, INF, false, T>), dim3(ma.ntiles_hbm), dim3(T), 0, c->stream,
-                         c->ds, ma);
+    if (ma.ntiles_hbm) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   }
   template <int INF>
   static void pd_inf(pdem_ctx* c, int bin) {
@@ -506,7 +527,7 @@ void build_rnp_tiles(pdem_ctx* c) {
 
 // Tiles of the mixed kernels: for every state-based particle, the largest
 // tile size (power of two, 64..tile_max) whose neighbor windows all fit in
-// LDS; LDS tiles first, then the tiles that gather from HBM.
+// LDS, else tiles of tile_max nodes that gather from HBM; LDS tiles first.
 void build_tiles(pdem_ctx* c, int64_t tile_max) {
   const auto& lo = c->nbr_lo;
   const auto& hi = c->nbr_hi;
@@ -523,8 +544,9 @@ void build_tiles(pdem_ctx* c, int64_t tile_max) {
       }
     };
     int64_t tile = tile_max;
-    for (; tile > 64; tile /= 2) {
-      bool fits = true;
+    bool fits = false;
+    for (; tile >= 64; tile /= 2) {
+      fits = true;
       for (int64_t ts = q.glob_start; ts < q.glob_end && fits; ts += tile) {
         int ws, we;
         window(ts, std::min<int64_t>(q.glob_end, ts + tile), ws, we);
@@ -532,6 +554,9 @@ void build_tiles(pdem_ctx* c, int64_t tile_max) {
       }
       if (fits) break;
     }
+    // no tile size down to 64 nodes fits (a long, thin numbering such as a
+    // meshed wall): full-size tiles that gather from HBM
+    if (!fits) tile = tile_max;
     for (int64_t ts = q.glob_start; ts < q.glob_end; ts += tile) {
       const int64_t te = std::min<int64_t>(q.glob_end, ts + tile);
       int ws, we;
@@ -773,6 +798,9 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     for (auto& q : c->hp)
       if (q.glob_end - q.glob_start > 65536)
         throw HipError{"mixed precision needs particles of at most 65536 nodes"};
+    HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     c->d_slot = c->dalloc<int>(N, false);
     c->d_sp16 = c->dalloc<int64_t>(c->nslices + 1, false);
     int* dlo = c->dalloc<int>(N);
@@ -1930,6 +1958,14 @@ int pdem_set_launch_config(pdem_ctx* c, int32_t pd_tpb, int32_t mixed_tpb, int32
     c->mixed_tpb = mixed_tpb;
     if (c->precision == PDEM_PREC_MIXED && c->ntiles > 0) build_tiles(c, tile_nodes);
     drop_graph(c);
+  });
+}
+
+int pdem_get_tile_counts(pdem_ctx* c, int32_t* lds_tiles, int32_t* hbm_tiles) {
+  return guarded(c, [&] {
+    if (!lds_tiles || !hbm_tiles) throw HipError{"null output pointer"};
+    *lds_tiles = c->mixed.ntiles_lds;
+    *hbm_tiles = c->mixed.ntiles_hbm;
   });
 }
 

@@ -11,7 +11,9 @@
 // position offset from the particle's first reference node (independent of
 // the tile shape), FP64 displacement,
 // FP32 volume, FP32 dilation terms) and every bond reads LDS; the remaining
-// tiles gather from HBM through the caches (a separate instantiation).
+// tiles (particles whose numbering spreads neighbors over more ids than the
+// window holds) gather from HBM through the caches: a separate
+// instantiation, launched on a second stream beside the LDS tiles.
 //
 // Neighbor ids are 16-bit offsets from the particle's first node, stored in
 // blocks of KB = 8 per lane: one 16-byte load yields a lane's next 8 ids and

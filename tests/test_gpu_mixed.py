@@ -51,3 +51,61 @@ def test_mixed_graph_matches_eager():
     a.integrate_cd(30)
     b.integrate_cd_graph(30)
     np.testing.assert_array_equal(a.field(capi.FIELD_F), b.field(capi.FIELD_F))
+
+
+def test_mixed_tiles_stage_in_lds():
+    """Spheres of ~300 nodes fit the LDS window whatever the tile size: every
+    tile of the mixed bond kernels stages its neighbor window in LDS, at the
+    default shape, at each shape set explicitly and after the autotuner (a
+    tile planned from stale neighbor ranges would fall back to HBM gathers).
+    Results do not depend on the shape."""
+    spec = configs.packing_3d(n_spheres=6, nodes_per_sphere=300, L=5.0e-3, seed=2,
+                              gap_factor=1.0, init_speed=2.0)
+    g = capi.GpuEngine(spec, capi.PREC_MIXED)
+    g.integrate_begin()
+    g.integrate_cd(5)
+    n_state = sum(1 for p in spec.particles if p["compute_force"])
+    lds, hbm = g.tile_counts()
+    print(f"[tiles] default {g.launch_config()}: lds {lds} hbm {hbm}", flush=True)
+    assert hbm == 0 and lds >= n_state
+    g.compute_forces()
+    ref = g.field(capi.FIELD_F)
+    for tpb, tn in ((256, 256), (1024, 512), (1024, 1024)):
+        g.set_launch_config(256, tpb, tn)
+        lds, hbm = g.tile_counts()
+        print(f"[tiles] mixed_tpb {tpb} tile {tn}: lds {lds} hbm {hbm}", flush=True)
+        assert hbm == 0 and lds >= n_state
+        g.compute_forces()
+        np.testing.assert_array_equal(g.field(capi.FIELD_F), ref)
+    g.autotune(reps=2)
+    lds, hbm = g.tile_counts()
+    print(f"[tiles] autotuned {g.launch_config()}: lds {lds} hbm {hbm}", flush=True)
+    assert hbm == 0 and lds >= n_state
+
+
+
+def test_mixed_hbm_tiles_against_oracle():
+    """A model whose wall is numbered so that its neighbor windows do not fit
+    LDS: the wall's tiles gather from HBM on the side stream, beside the LDS
+    tiles of the spheres.  Forces within the FP32 bond arithmetic bar of the
+    oracle after eager steps, and the captured step equal to the eager one bit
+    for bit (the side stream is forked and joined inside the graph)."""
+    spec = configs.packing_3d(n_spheres=2, nodes_per_sphere=1000, L=20.0e-3, seed=2,
+                              gap_factor=1.0, init_speed=2.0)
+    steps = 20
+    a = capi.GpuEngine(spec, capi.PREC_MIXED)
+    b = capi.GpuEngine(spec, capi.PREC_MIXED)
+    lds, hbm = a.tile_counts()
+    print(f"[hbm-tiles] nodes={spec.n_nodes} lds {lds} hbm {hbm}", flush=True)
+    assert hbm > 0 and lds > 0
+    r = oracle.OracleEngine(spec, 8)
+    for e in (a, b, r):
+        e.integrate_begin()
+    a.integrate_cd(steps)
+    b.integrate_cd_graph(steps)
+    r.integrate_cd(steps)
+    emax, erms = _errors(a.field(capi.FIELD_F), r.field(capi.FIELD_F))
+    print(f"[hbm-tiles] force max err {emax:.3e} rms {erms:.3e}", flush=True)
+    assert emax < 1e-4 and erms < 1e-5
+    np.testing.assert_array_equal(a.field(capi.FIELD_F), b.field(capi.FIELD_F))
+    np.testing.assert_array_equal(a.field(capi.FIELD_U), b.field(capi.FIELD_U))
