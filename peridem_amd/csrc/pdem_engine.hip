@@ -883,6 +883,7 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     hipLaunchKernelGGL(k_cells_init, dim3((unsigned)std::min<int64_t>(blocks(ncells), 4096)), dim3(TPB), 0,
                        c->stream, s);
     s.cell_nodes = c->dalloc<int>((size_t)ncells * s.cell_cap, false);
+    s.cell_xf = c->dalloc<float4>((size_t)ncells * s.cell_cap, false);
     // candidate capacity: nodes of a ball of the largest search radius at the
     // finest mesh spacing
     double hmin = 1e300;

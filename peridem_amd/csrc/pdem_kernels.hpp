@@ -121,6 +121,7 @@ struct DS {
   int cell_cap;     // fixed capacity of a cell
   int* cell_count;  // [ncells]
   int* cell_nodes;  // [ncells][cell_cap], unordered inside a cell
+  float4* cell_xf;  // [ncells][cell_cap] beside cell_nodes: position - glo in FP32, particle id (bits)
   int* cell_owner;  // [ncells] particle id when one particle fills the cell, -1 mixed, -2 empty
   // damping
   int n_nonwall;
@@ -962,10 +963,14 @@ __device__ void bin_node(const DS& s, int64_t i) {
   const int old = atomicCAS(&s.cell_owner[c], -2, pid);
   if (old != -2 && old != pid) s.cell_owner[c] = -1;
   int slot = atomicAdd(&s.cell_count[c], 1);
-  if (slot < s.cell_cap)
-    s.cell_nodes[(int64_t)c * s.cell_cap + slot] = (int)i;
-  else
+  if (slot < s.cell_cap) {
+    const int64_t e = (int64_t)c * s.cell_cap + slot;
+    s.cell_nodes[e] = (int)i;
+    s.cell_xf[e] = make_float4((float)(s.x[0][i] - s.glo[0]), (float)(s.x[1][i] - s.glo[1]),
+                               DIM == 3 ? (float)(s.x[2][i] - s.glo[2]) : 0.f, __int_as_float(pid));
+  } else {
     s.ctrl->overflow = 2;
+  }
 }
 
 template <int DIM>
@@ -977,11 +982,28 @@ __global__ void k_bin(DS s) {
 
 // radiusSearchExcludeTag for one searching node (demModel.cpp:1717-1756):
 // other particles, j != i, 0 + dx^2 + dy^2 + dz^2 < r^2.
+// Cell entries are first tested on their FP32 copy (one 16-byte load: the
+// position relative to the grid origin and the particle id): an entry is
+// dropped when its FP32 distance is at least Rf = r + 2^-19 (max|a| + r),
+// a = x_i - glo.  Rounding a and b = x_j - glo to FP32 moves each component
+// by at most 2^-24 |a| and 2^-24 (|a| + r) for |x_j - x_i| < r, and the FP32
+// difference and sum add a relative 2^-22 at most, so an entry within r of
+// node i always passes, with a factor of four to spare.  Entries that pass
+// take the FP64 test below, which alone decides the candidate set.
 template <int DIM>
 __device__ void query_node(const DS& s, int64_t i, int pi) {
   const double r = s.ctrl->radius, r2 = r * r;
   double xi[3] = {0., 0., 0.};
   for (int d = 0; d < DIM; d++) xi[d] = s.x[d][i];
+  float af[3] = {0.f, 0.f, 0.f};
+  float amax = 0.f;
+  for (int d = 0; d < DIM; d++) {
+    af[d] = (float)(xi[d] - s.glo[d]);
+    amax = fmaxf(amax, fabsf(af[d]));
+  }
+  const float rf = (float)r;
+  const float Rf = rf + 0x1p-19f * (amax + rf);
+  const float R2f = Rf * Rf;
   int cc0[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   for (int d = 0; d < DIM; d++) {
     cc0[d] = cellc(xi[d], s.glo[d], s.cell, s.gn[d]);
@@ -1010,9 +1032,14 @@ __device__ void query_node(const DS& s, int64_t i, int pi) {
         int m = s.cell_count[cc];
         m = m < s.cell_cap ? m : s.cell_cap;
         const int* cn = s.cell_nodes + (int64_t)cc * s.cell_cap;
+        const float4* cx = s.cell_xf + (int64_t)cc * s.cell_cap;
         for (int q = 0; q < m; q++) {
+          const float4 e = cx[q];
+          if (__float_as_int(e.w) == pi) continue;  // the node's own particle (and i itself)
+          const float t0 = af[0] - e.x, t1 = af[1] - e.y, t2 = DIM == 3 ? af[2] - e.z : 0.f;
+          const float d2f = t0 * t0 + t1 * t1 + t2 * t2;
+          if (!(d2f < R2f)) continue;
           const int j = cn[q];
-          if (j == i || s.ptid[j] == pi) continue;
           double d2 = 0.;
           for (int d = 0; d < DIM; d++) {
             double t = xi[d] - s.x[d][j];
