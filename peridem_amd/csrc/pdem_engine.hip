@@ -174,7 +174,10 @@ struct Launch {
                          c->ds, ma);
       HIPCHK(hipEventRecord(c->ev_join, c->side));
     }
-    if (ma.ntiles_lds)
+    if (ma.ntiles_lds && ma.small)
+      hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, true, T, LDS_SMALL>), dim3(ma.ntiles_lds), dim3(T), 0, c->stream,
+                         c->ds, ma);
+    else if (ma.ntiles_lds)
       hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, true, T>), dim3(ma.ntiles_lds), dim3(T), 0, c->stream,
                          c->ds, ma);
     if (ma.ntiles_hbm) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
@@ -188,7 +191,10 @@ struct Launch {
                          c->ds, ma);
       HIPCHK(hipEventRecord(c->ev_join, c->side));
     }
-    if (ma.ntiles_lds)
+    if (ma.ntiles_lds && ma.small)
+      hipLaunchKernelGGL((k_mixed_force<DIM, INF, true, T, LDS_SMALL>), dim3(ma.ntiles_lds), dim3(T), 0, c->stream,
+                         c->ds, ma);
+    else if (ma.ntiles_lds)
       hipLaunchKernelGGL((k_mixed_force<DIM, INF, true, T>), dim3(ma.ntiles_lds), dim3(T), 0, c->stream,
                          c->ds, ma);
     if (ma.ntiles_hbm) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
@@ -600,6 +606,10 @@ void build_tiles(pdem_ctx* c, int64_t tile_max) {
   c->ntiles_global = 0;
   for (auto& t : tiles) c->ntiles_global += t.lds ? 0 : 1;
   c->mixed.ntiles_lds = c->ntiles - c->ntiles_global;
+  int maxw = 0;
+  for (auto& t : tiles)
+    if (t.lds) maxw = std::max(maxw, t.we - t.ws);
+  c->mixed.small = maxw <= LDS_SMALL ? 1 : 0;
   c->mixed.ntiles_hbm = c->ntiles_global;
   c->tile_nodes = (int)tile_max;
   if (!tiles.empty())

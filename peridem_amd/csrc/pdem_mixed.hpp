@@ -39,6 +39,10 @@ namespace pdem {
 
 constexpr int TILE_NODES = 1024;
 constexpr int LDS_NODES = 1280;   // window of up to 1280 nodes, ~61 KB of LDS per workgroup
+// Window classes: a launch whose LDS windows all hold at most LDS_SMALL nodes
+// uses the small instantiation (~19 KB of LDS per workgroup, so small particles
+// do not hold LDS sized for the largest window and more workgroups fit a CU).
+constexpr int LDS_SMALL = 384;
 constexpr int MIXED_TPB = 1024;   // default workgroup size (the autotuner may pick 256)
 constexpr int KB = 8;             // neighbor ids per 16-byte block
 // waves per SIMD the bond kernels are compiled for (at most 64 VGPRs): two
@@ -55,6 +59,7 @@ struct DevTile {
 struct MixedArgs {
   const DevTile* tiles;   // LDS tiles first, then HBM tiles
   int ntiles_lds, ntiles_hbm;
+  int small;              // every LDS window holds at most LDS_SMALL nodes
   // thread slot q runs node slot[q]; its k-th neighbor id is
   // col16[sptr16[q / 64] + (k / KB) * 64 * KB + (q % 64) * KB + k % KB]
   const int* slot;
@@ -100,10 +105,11 @@ __device__ __forceinline__ int block_id(const uint4& b, int t) {
 }
 
 // node data access by particle-relative id jp (node gs + jp): LDS window or HBM.
-// LDS arrays of the three components are LDS_ST apart (LDS_NODES plus 16
+// LDS arrays of the three components are W apart (the window class plus 16
 // entries), which shifts a component's banks against the next one's.
-constexpr int LDS_ST = LDS_NODES + 16;
-template <int DIM, bool LDS>
+template <int NODES>
+constexpr int lds_stride() { return NODES + 16; }
+template <int DIM, bool LDS, int W>
 struct Win {
   const float* lx;
   const double* lu;
@@ -116,17 +122,17 @@ struct Win {
   int off;  // gs - window start: LDS index of jp
   double org[3];
   __device__ __forceinline__ float x(int d, int jp) const {
-    return LDS ? lx[d * LDS_ST + jp + off] : (float)(s->xr[d][gs + jp] - org[d]);
+    return LDS ? lx[d * W + jp + off] : (float)(s->xr[d][gs + jp] - org[d]);
   }
   __device__ __forceinline__ double u(int d, int jp) const {
-    return LDS ? lu[d * LDS_ST + jp + off] : s->u[d][gs + jp];
+    return LDS ? lu[d * W + jp + off] : s->u[d][gs + jp];
   }
   __device__ __forceinline__ float v(int jp) const { return LDS ? lv[jp + off] : (float)s->vol[gs + jp]; }
   __device__ __forceinline__ float a(int jp) const { return LDS ? la[jp + off] : m->fa[gs + jp]; }
   __device__ __forceinline__ float b(int jp) const { return LDS ? lb[jp + off] : m->fb[gs + jp]; }
 };
 
-template <int DIM, bool FORCE>
+template <int DIM, bool FORCE, int W>
 __device__ __forceinline__ void stage_window(const DS& s, const MixedArgs& m, const DevTile& t,
                                              float* lx, double* lu, float* lv, float* la,
                                              float* lb, const double* org) {
@@ -134,8 +140,8 @@ __device__ __forceinline__ void stage_window(const DS& s, const MixedArgs& m, co
   for (int q = threadIdx.x; q < nw; q += blockDim.x) {
     const int j = t.ws + q;
     for (int d = 0; d < DIM; d++) {
-      lx[d * LDS_ST + q] = (float)(s.xr[d][j] - org[d]);
-      lu[d * LDS_ST + q] = s.u[d][j];
+      lx[d * W + q] = (float)(s.xr[d][j] - org[d]);
+      lu[d * W + q] = s.u[d][j];
     }
     lv[q] = (float)s.vol[j];
     if (FORCE) {
@@ -147,18 +153,19 @@ __device__ __forceinline__ void stage_window(const DS& s, const MixedArgs& m, co
 }
 
 // Dilation pass (demModel.cpp:610-679) in mixed precision.
-template <int DIM, int INF, bool LDS, int TPB>
+template <int DIM, int INF, bool LDS, int TPB, int NODES = LDS_NODES>
 __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_dilation(DS s, MixedArgs m) {
 #pragma clang fp contract(fast)
-  __shared__ float lx[LDS ? 3 * LDS_ST : 1];
-  __shared__ double lu[LDS ? 3 * LDS_ST : 1];
-  __shared__ float lv[LDS ? LDS_NODES : 1];
+  constexpr int W = lds_stride<NODES>();
+  __shared__ float lx[LDS ? 3 * W : 1];
+  __shared__ double lu[LDS ? 3 * W : 1];
+  __shared__ float lv[LDS ? NODES : 1];
   const DevTile t = m.tiles[LDS ? blockIdx.x : m.ntiles_lds + blockIdx.x];
   const DevParticle& p = s.P[t.pid];
   const int gs = (int)p.glob_start;
-  Win<DIM, LDS> w{lx, lu, lv, nullptr, nullptr, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
+  Win<DIM, LDS, W> w{lx, lu, lv, nullptr, nullptr, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
   for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][gs];
-  if (LDS) stage_window<DIM, false>(s, m, t, lx, lu, lv, nullptr, nullptr, w.org);
+  if (LDS) stage_window<DIM, false, W>(s, m, t, lx, lu, lv, nullptr, nullptr, w.org);
   const float horizon = (float)p.horizon, h = (float)p.h;
   const float check_up = horizon + 0.5f * h, check_low = horizon - 0.5f * h;
   const float low_rel = check_low / REL_KEEP;             // r > low_rel: isGreater(r, check_low)
@@ -239,20 +246,21 @@ __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_dilation(DS
 // The damage Z_i is the plain maximum of |S| / s0 over the list (the
 // reference's running compare with a 1e-5 relative slack keeps a value at
 // most 1e-5 below it; FP32 bond arithmetic is the larger difference).
-template <int DIM, int INF, bool LDS, int TPB>
+template <int DIM, int INF, bool LDS, int TPB, int NODES = LDS_NODES>
 __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_force(DS s, MixedArgs m) {
 #pragma clang fp contract(fast)
-  __shared__ float lx[LDS ? 3 * LDS_ST : 1];
-  __shared__ double lu[LDS ? 3 * LDS_ST : 1];
-  __shared__ float lv[LDS ? LDS_NODES : 1];
-  __shared__ float la[LDS ? LDS_NODES : 1];
-  __shared__ float lb[LDS ? LDS_NODES : 1];
+  constexpr int W = lds_stride<NODES>();
+  __shared__ float lx[LDS ? 3 * W : 1];
+  __shared__ double lu[LDS ? 3 * W : 1];
+  __shared__ float lv[LDS ? NODES : 1];
+  __shared__ float la[LDS ? NODES : 1];
+  __shared__ float lb[LDS ? NODES : 1];
   const DevTile t = m.tiles[LDS ? blockIdx.x : m.ntiles_lds + blockIdx.x];
   const DevParticle& p = s.P[t.pid];
   const int gs = (int)p.glob_start;
-  Win<DIM, LDS> w{lx, lu, lv, la, lb, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
+  Win<DIM, LDS, W> w{lx, lu, lv, la, lb, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
   for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][gs];
-  if (LDS) stage_window<DIM, true>(s, m, t, lx, lu, lv, la, lb, w.org);
+  if (LDS) stage_window<DIM, true, W>(s, m, t, lx, lu, lv, la, lb, w.org);
   const float horizon = (float)p.horizon, h = (float)p.h;
   const float check_up = horizon + 0.5f * h, check_low = horizon - 0.5f * h;
   const float low_rel = check_low / REL_KEEP;  // r > low_rel: isGreater(r, check_low)
