@@ -129,6 +129,7 @@ def _closure_run(spec, precision, net, steps, sample_every, reaction):
     if net is not None:
         e.closure_enable(net, pen_max_h=0.05, dz_max=0.5, check_every=100)
     e.integrate_begin()
+    e.autotune(reps=3)  # both forms at their tuned bond-kernel shape
     e.integrate_cd_graph(10)
     e.synchronize()
     curve, tt, done = [], 0.0, 0
