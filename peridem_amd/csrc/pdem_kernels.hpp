@@ -405,8 +405,19 @@ __global__ void __launch_bounds__(1024) k_ctrl(DS s, int advance) {
       __syncthreads();
     }
     if (c->vdirty) {
+      // eight loads in flight per thread (one workgroup reads every block maximum)
       unsigned long long m = 0;
-      for (int k = t; k < s.nblockmax; k += blockDim.x) m = s.blockmax[k] > m ? s.blockmax[k] : m;
+      const int nb = s.nblockmax, stride = (int)blockDim.x;
+      for (int k0 = t; k0 < nb; k0 += 8 * stride) {
+        unsigned long long v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int k = k0 + u * stride;
+          v[u] = k < nb ? s.blockmax[k] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) m = v[u] > m ? v[u] : m;
+      }
       for (int off = WAVE / 2; off > 0; off >>= 1) {
         unsigned long long o = __shfl_xor(m, off);
         m = o > m ? o : m;
@@ -1173,15 +1184,23 @@ __global__ void __launch_bounds__(256) k_contact_light(DS s) {
 #pragma unroll
     for (int q = 0; q < NACT; q++) act[q] = INT_MAX;
     int nact = 0;
-    int jn = s.cand[i];
+    // distance prefilter over all candidates before any particle data is
+    // loaded: independent loads, several candidates in flight
+    unsigned pass = 0;
+#pragma unroll 4
     for (int k = 0; k < n; k++) {
-      const int j = jn;
-      if (k + 1 < n) jn = s.cand[(int64_t)(k + 1) * s.N + i];
+      const int j = s.cand[(int64_t)k * s.N + i];
       double yji[3];
       for (int d = 0; d < DIM; d++) yji[d] = s.x[d][j] - yi[d];
-      // distance prefilter before any particle data is loaded
+      if (dot3<DIM>(yji, yji) < s.rc2_max) pass |= 1u << k;
+    }
+    while (pass) {
+      const int k = __builtin_ctz(pass);
+      pass &= pass - 1u;
+      const int j = s.cand[(int64_t)k * s.N + i];
+      double yji[3];
+      for (int d = 0; d < DIM; d++) yji[d] = s.x[d][j] - yi[d];
       const double d2 = dot3<DIM>(yji, yji);
-      if (d2 >= s.rc2_max) continue;
       const int pjid = s.ptid[j];
       const int mj = s.pmeta[pjid];
       const bool wj = (mj >> 16) != 0;
