@@ -1285,15 +1285,16 @@ __global__ void __launch_bounds__(CWAVES * WAVE) k_contact_force(DS s) {
       double term[6] = {0., 0., 0., 0., 0., 0.};
       if (k < n) {
         j = s.cand[(int64_t)k * s.N + i];
-        const int pjid = s.ptid[j];
+        // distance prefilter (largest contact radius) before any particle data is loaded
+        double yji[3];
+        for (int d = 0; d < DIM; d++) yji[d] = s.x[d][j] - yi[d];
+        const double d2 = dot3<DIM>(yji, yji);
+        const int pjid = d2 < s.rc2_max ? s.ptid[j] : pid;
         const int mj = s.pmeta[pjid];
         const bool wj = (mj >> 16) != 0;
         if (!(j == i || pjid == pid || (wi && wj))) {
-          double yji[3];
-          for (int d = 0; d < DIM; d++) yji[d] = s.x[d][j] - yi[d];
           const DevContact ct = s.CT[zi * s.nz + (mj & 0xffff)];
           // d^2 >= rc^2 implies R >= rc (1 - 2^-53): isLess(R, rc) is false
-          const double d2 = dot3<DIM>(yji, yji);
           if (d2 < ct.contact_r * ct.contact_r) {
             const double Rji = len3<DIM>(yji);
             if (isLess(Rji, ct.contact_r) &&
