@@ -87,25 +87,42 @@ def test_mixed_tiles_stage_in_lds():
 def test_mixed_hbm_tiles_against_oracle():
     """A model whose wall is numbered so that its neighbor windows do not fit
     LDS: the wall's tiles gather from HBM on the side stream, beside the LDS
-    tiles of the spheres.  Forces within the FP32 bond arithmetic bar of the
-    oracle after eager steps, and the captured step equal to the eager one bit
-    for bit (the side stream is forked and joined inside the graph)."""
+    tiles of the spheres.  A smooth non-rigid displacement field (strains up
+    to ~1e-3) is imposed on every body so that all bonds, the wall's
+    included, carry force; then forces within the FP32 bond arithmetic bar of
+    the oracle after eager steps, and the captured step equal to the eager one
+    bit for bit (the side stream is forked and joined inside the graph)."""
     spec = configs.packing_3d(n_spheres=2, nodes_per_sphere=1000, L=20.0e-3, seed=2,
                               gap_factor=1.0, init_speed=2.0)
-    steps = 20
+    steps = 5
     a = capi.GpuEngine(spec, capi.PREC_MIXED)
     b = capi.GpuEngine(spec, capi.PREC_MIXED)
     lds, hbm = a.tile_counts()
     print(f"[hbm-tiles] nodes={spec.n_nodes} lds {lds} hbm {hbm}", flush=True)
     assert hbm > 0 and lds > 0
     r = oracle.OracleEngine(spec, 8)
+    xr = spec.x_ref
+    lam = 2.0e-3
+    u = np.zeros_like(xr)
+    for d in range(3):
+        u[:, d] = 3.0e-7 * np.sin(2 * np.pi * xr[:, (d + 1) % 3] / lam + d)
     for e in (a, b, r):
         e.integrate_begin()
+        e.set_field(capi.FIELD_U, u)
+        e.set_field(capi.FIELD_X, xr + u)
+        e.compute_forces()
+    wall = slice(spec.particles[-1]["glob_start"], spec.particles[-1]["glob_end"])
+    f0 = r.field(capi.FIELD_F)
+    emax, erms = _errors(a.field(capi.FIELD_F), f0)
+    print(f"[hbm-tiles] deformed: max |f| wall {np.abs(f0[wall]).max():.3e}, "
+          f"force max err {emax:.3e} rms {erms:.3e}", flush=True)
+    assert np.abs(f0[wall]).max() > 1e-3 * np.abs(f0).max()
+    assert 0 < emax < 1e-4 and erms < 1e-5
     a.integrate_cd(steps)
     b.integrate_cd_graph(steps)
     r.integrate_cd(steps)
     emax, erms = _errors(a.field(capi.FIELD_F), r.field(capi.FIELD_F))
-    print(f"[hbm-tiles] force max err {emax:.3e} rms {erms:.3e}", flush=True)
+    print(f"[hbm-tiles] after {steps} steps: force max err {emax:.3e} rms {erms:.3e}", flush=True)
     assert emax < 1e-4 and erms < 1e-5
     np.testing.assert_array_equal(a.field(capi.FIELD_F), b.field(capi.FIELD_F))
     np.testing.assert_array_equal(a.field(capi.FIELD_U), b.field(capi.FIELD_U))
