@@ -81,6 +81,45 @@ def pd_bytes_per_launch(spec, eng, precision):
     return float(layout.sum()), float(canon.sum()), float(nb.mean()) if len(nb) else 0.0
 
 
+def force_error_vs_fp64(spec, eng):
+    """Mixed precision against the FP64 parity path (itself equal to the
+    oracle bit for bit in the tests) at full size: an FP64 engine takes the
+    mixed run's final state (x, u, v, bond states) plus a smooth displacement
+    perturbation (peak strain ~1e-4, so that every bond carries force even
+    where bodies still move rigidly), both recompute the peridynamic forces,
+    and the difference is reported as max and RMS over nodes, normalised by
+    the largest FP64 force."""
+    import numpy as np
+    from peridem_amd import capi
+
+    h = min(p["mesh_size"] for p in spec.particles)
+    lam = 10.0 * h
+    amp = 1e-4 * lam / (2 * np.pi)
+    xr = spec.x_ref
+    du = np.zeros_like(xr)
+    for d in range(spec.dim):
+        du[:, d] = amp * np.sin(2 * np.pi * xr[:, (d + 1) % spec.dim] / lam + d)
+    u = eng.field(capi.FIELD_U) + du
+    x = eng.field(capi.FIELD_X) + du
+    ref = capi.GpuEngine(spec, capi.PREC_FP64)
+    ref.integrate_begin()
+    for e in (eng, ref):
+        e.set_field(capi.FIELD_U, u)
+        e.set_field(capi.FIELD_X, x)
+    ref.set_field(capi.FIELD_V, eng.field(capi.FIELD_V))
+    ref.set_field(capi.FIELD_BONDS, eng.field(capi.FIELD_BONDS))
+    eng.compute_peridynamic_forces()
+    ref.compute_peridynamic_forces()
+    fm, fr = eng.field(capi.FIELD_F), ref.field(capi.FIELD_F)
+    ref.close()
+    den = float(np.abs(fr).max())
+    d = np.abs(fm - fr)
+    return {"max": float(d.max() / den), "rms": float(np.sqrt((d ** 2).sum(1).mean()) / den),
+            "state": "final state of the eager region plus a smooth displacement perturbation "
+                     "(wavelength 10 h, peak strain 1e-4); peridynamic forces of both recomputed on it",
+            "max_force": den}
+
+
 def read_traffic(kernel_tag: str):
     path = os.path.join(ROOT, "profiles", f"{kernel_tag}_traffic.txt")
     if not os.path.exists(path):
@@ -204,6 +243,7 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=300)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-closure", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--closure-steps", type=int, default=50000)
     ap.add_argument("--closure-train-instances", type=int, default=512)
     ap.add_argument("--closure-train-steps", type=int, default=5000)
@@ -276,6 +316,10 @@ def main():
     st = eng.contact_state()
     n_cand = eng.neighbor_count(capi.NEIGH_CONTACT)
     broken, total = eng.broken_bonds()
+    ferr = None
+    if mixed and not args.no_error:
+        ferr = force_error_vs_fp64(spec, eng)
+        print(f"# force error vs FP64: {ferr}", file=sys.stderr, flush=True)
     eng.close()
 
     closure_res = None
@@ -301,6 +345,8 @@ def main():
                    "broken_bond_fraction": broken / max(1, total), "launch": tuned},
         "roofline": roof, "cpu_baseline": cpu, "closure": closure_res,
     }
+    if ferr is not None:
+        out["force_error_vs_fp64"] = ferr
     print(json.dumps(out), flush=True)
 
 
