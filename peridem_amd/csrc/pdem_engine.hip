@@ -204,7 +204,7 @@ struct Launch {
     hipStream_t st = c->stream;
     const unsigned nb = blocks(c->N);
     if (c->precision == PDEM_PREC_MIXED && c->ntiles > 0) {
-      hipLaunchKernelGGL(k_bin_zero<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds, bin);
+      hipLaunchKernelGGL(k_bin_zero<DIM>, dim3(std::min<unsigned>(nb, BZ_BLOCKS)), dim3(TPB), 0, st, c->ds, bin);
       if (c->rnp_any) hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(nb), dim3(TPB), 0, st, c->ds, 0);
       if (c->pmb_any) hipLaunchKernelGGL((k_pd_bond<DIM, INF, 2>), dim3(nb), dim3(TPB), 0, st, c->ds, 0);
       if (c->elastic_any) hipLaunchKernelGGL((k_pd_bond<DIM, INF, 3>), dim3(nb), dim3(TPB), 0, st, c->ds, 0);
@@ -966,6 +966,14 @@ void build(pdem_ctx* c, const pdem_desc* d) {
         for (int64_t i = c->hp[p].glob_start; i < c->hp[p].glob_end; i++) wn.push_back(i);
     s.n_wall_nodes = (int64_t)wn.size();
     s.wall_nodes = c->upload(wn.data(), wn.size());
+  }
+  {
+    std::vector<int64_t> nf;
+    for (int p = 0; p < c->np; p++)
+      if (!c->hp[p].compute_force)
+        for (int64_t i = c->hp[p].glob_start; i < c->hp[p].glob_end; i++) nf.push_back(i);
+    s.n_noforce = (int64_t)nf.size();
+    s.noforce_nodes = c->upload(nf.data(), nf.size());
   }
   // state carried between steps: node fields, bond bits, damage, controller,
   // candidate and partner lists of the last rebuild (everything else is

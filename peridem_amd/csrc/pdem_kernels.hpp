@@ -168,6 +168,8 @@ struct DS {
   // nodes of wall particles, ascending (finalisation inside the damping pass)
   const int64_t* wall_nodes;
   int64_t n_wall_nodes;
+  const int64_t* noforce_nodes;  // nodes of particles that compute no peridynamic force
+  int64_t n_noforce;
   // per-node displacement-BC entries (CSR over nodes, set order)
   const int* bc_ptr;
   const BcEntry* bc_ent;
@@ -971,9 +973,28 @@ __device__ void bin_node(const DS& s, int64_t i) {
   int cz = DIM == 3 ? cellc(s.x[2][i], s.glo[2], s.cell, s.gn[2]) : 0;
   int c = (cz * s.gn[1] + cy) * s.gn[0] + cx;
   const int pid = s.ptid[i];
-  const int old = atomicCAS(&s.cell_owner[c], -2, pid);
-  if (old != -2 && old != pid) s.cell_owner[c] = -1;
-  int slot = atomicAdd(&s.cell_count[c], 1);
+  // the owner only moves -2 (empty) -> a particle id -> -1 (several): a cell
+  // already read as this particle's or as mixed needs no atomic
+  const int seen = s.cell_owner[c];
+  if (seen != pid && seen != -1) {
+    const int old = atomicCAS(&s.cell_owner[c], -2, pid);
+    if (old != -2 && old != pid) s.cell_owner[c] = -1;
+  }
+  // slots: one atomic per distinct cell among the wavefront's active lanes
+  // (the lanes of a wavefront bin neighboring nodes, mostly into a few cells)
+  const int lane = (int)(threadIdx.x & (WAVE - 1));
+  unsigned long long todo = __ballot(1);
+  int slot = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int cl = __shfl(c, leader);
+    const unsigned long long same = __ballot(c == cl);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&s.cell_count[cl], (int)__popcll(same));
+    base = __shfl(base, leader);
+    if (c == cl) slot = base + (int)__popcll(same & ((1ull << lane) - 1ull));
+    todo &= ~same;
+  }
   if (slot < s.cell_cap) {
     const int64_t e = (int64_t)c * s.cell_cap + slot;
     s.cell_nodes[e] = (int)i;

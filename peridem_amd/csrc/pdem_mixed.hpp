@@ -335,14 +335,19 @@ __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_force(DS s,
   }
 }
 
-// binning on a rebuild and the force reset of nodes that take no PD force
+// the force reset of nodes that take no PD force (a list built at set-up)
+// and, on a rebuild step, the binning of every node; grid-stride, so that a
+// step without a rebuild launches few workgroups
+constexpr int BZ_BLOCKS = 1024;
 template <int DIM>
 __global__ void k_bin_zero(DS s, int bin) {
-  int64_t i = gid();
-  if (i >= s.N) return;
-  if (bin && s.ctrl->rebuild) bin_node<DIM>(s, i);
-  if (!s.P[s.ptid[i]].compute_force)
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = gid(); k < s.n_noforce; k += stride) {
+    const int64_t i = s.noforce_nodes[k];
     for (int d = 0; d < DIM; d++) s.f[d][i] = 0.;
+  }
+  if (bin && s.ctrl->rebuild)
+    for (int64_t i = gid(); i < s.N; i += stride) bin_node<DIM>(s, i);
 }
 
 }  // namespace pdem
