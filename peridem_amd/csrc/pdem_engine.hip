@@ -510,7 +510,18 @@ void build_rnp_tiles(pdem_ctx* c) {
     tiles.push_back(RtTile{(int)t0, (int)t1, base, 1, pbase});
     lds = std::max(lds, (size_t)(base + (2 * c->dim + 1) * (t1 - t0)) * sizeof(double) + (size_t)base * sizeof(float));
   }
+  // pass-2 lane order: inside every tile, nodes by degree (most first), then id
+  std::vector<uint16_t> perm((size_t)N, 0);
+  for (const auto& t : tiles) {
+    const int nt = t.t1 - t.t0;
+    std::vector<int> loc(nt);
+    for (int q = 0; q < nt; q++) loc[q] = q;
+    std::stable_sort(loc.begin(), loc.end(),
+                     [&](int x, int y) { return c->h_deg[t.t0 + x] > c->h_deg[t.t0 + y]; });
+    for (int q = 0; q < nt; q++) perm[(size_t)t.t0 + q] = (uint16_t)loc[q];
+  }
   RnpTileArgs& a = c->rt;
+  a.perm = c->upload(perm.data(), perm.size());
   a.tiles = c->upload(tiles.data(), tiles.size());
   a.pl = c->upload(pl.data(), pl.size());
   a.ps = c->upload(ps.data(), ps.size());

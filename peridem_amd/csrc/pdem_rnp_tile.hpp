@@ -45,6 +45,7 @@ struct RnpTileArgs {
   const RtTile* tiles = nullptr;
   const uint32_t* pl = nullptr;    // [pairs] tile-local (i | j << 16), i < j, in slot order
   const uint16_t* ps = nullptr;    // sliced ELL like col: pair slot of in-tile bonds, RT_OUT otherwise
+  const uint16_t* perm = nullptr;  // [node t0 + q]: tile-local node of pass-2 lane q (degree order)
   int ntiles = 0;
   size_t lds_bytes = 0;
 };
@@ -106,8 +107,10 @@ __global__ void __launch_bounds__(RT_THREADS) k_pd_rnp_tile(DS s, RnpTileArgs a,
   }
   __syncthreads();
 
-  // 2. per-node sums in list order
-  for (int q = threadIdx.x; q < n_t; q += blockDim.x) {
+  // 2. per-node sums in list order; lanes take the tile's nodes in degree
+  // order, so a wavefront's lists have about the same length
+  for (int qq = threadIdx.x; qq < n_t; qq += blockDim.x) {
+    const int q = a.perm[t.t0 + qq];
     const int64_t i = t.t0 + q;
     if (bin && s.ctrl->rebuild) bin_node<DIM>(s, i);
     const int64_t sl = i / WAVE, lane = i % WAVE;
