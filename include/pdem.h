@@ -301,6 +301,15 @@ int pdem_set_launch_config(pdem_ctx *ctx, int32_t pd_tpb, int32_t mixed_tpb, int
  * HBM (0 and 0 for FP64 models). No counterpart in the reference (launch
  * shape introspection for tuning and tests). */
 int pdem_get_tile_counts(pdem_ctx *ctx, int32_t *lds_tiles, int32_t *hbm_tiles);
+
+/* Contact kernel selection: at each candidate rebuild, nodes with at most
+ * light_max candidates (0..16) go to the lane-per-node contact kernel, the
+ * others to the wavefront-per-node kernel. Default 16 for 3D models and for
+ * 2D models of 2^20 nodes or more (many nodes in contact), else 0 (few
+ * contacts: one launch less per step). Takes effect at the next rebuild;
+ * results do not depend on it (both kernels sum a node's active pairs in
+ * ascending node id). No counterpart in the reference (kernel selection). */
+int pdem_set_contact_light_max(pdem_ctx *ctx, int32_t light_max);
 /* Step fusion for multi-particle models of up to 2^20 nodes (on by default):
  * the node update also applies the displacement BC and packs the particle
  * centres, the controller runs as one workgroup and the damping pass

@@ -402,6 +402,8 @@ def engine_library():
         lib.pdem_set_launch_config.restype = C.c_int
         lib.pdem_set_launch_config.argtypes = [C.c_void_p] + [C.c_int32] * 3
         lib.pdem_get_tile_counts.restype = C.c_int
+        lib.pdem_set_contact_light_max.restype = C.c_int
+        lib.pdem_set_contact_light_max.argtypes = [C.c_void_p, C.c_int32]
         lib.pdem_get_tile_counts.argtypes = [C.c_void_p] + [C.POINTER(C.c_int32)] * 2
         lib.pdem_set_step_fusion.restype = C.c_int
         lib.pdem_set_step_fusion.argtypes = [C.c_void_p, C.c_int32]
@@ -494,6 +496,10 @@ class GpuEngine(Engine):
     def set_launch_config(self, pd_tpb, mixed_tpb=1024, tile_nodes=1024):
         """Apply a launch shape (pd_tpb 512: the RNP pair-tile kernel)."""
         self._check(self.lib.pdem_set_launch_config(self.ctx, pd_tpb, mixed_tpb, tile_nodes), "set_launch_config")
+
+    def set_contact_light_max(self, light_max):
+        """Nodes with at most light_max contact candidates take the lane-per-node kernel."""
+        self._check(self.lib.pdem_set_contact_light_max(self.ctx, int(light_max)), "set_contact_light_max")
 
     def tile_counts(self):
         """(tiles staged in LDS, tiles gathering from HBM) of the mixed bond kernels."""

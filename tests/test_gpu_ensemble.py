@@ -99,3 +99,29 @@ def test_autotune_keeps_results(precision):
     else:
         assert np.abs(fa - fb).max() <= 1e-6 * np.abs(fa).max()
     np.testing.assert_array_equal(a.field(capi.FIELD_BONDS), b.field(capi.FIELD_BONDS))
+
+
+def test_light_contact_kernel_2d_matches_oracle():
+    """The lane-per-node contact kernel in 2D (used by default for 2D models
+    of 2^20 nodes and more) on a small ensemble with contacts, forced through
+    pdem_set_contact_light_max: FP64 parity state equal to the oracle, and
+    the same as the wavefront-per-node kernel bit for bit."""
+    spec = _small_ensemble(n=4, seed=5)
+    K = 400
+    runs = {}
+    for lm in (16, 0):
+        g = capi.GpuEngine(spec, capi.PREC_FP64)
+        g.set_contact_light_max(lm)
+        g.integrate_begin()
+        g.integrate_cd_graph(K)
+        runs[lm] = g
+    r = oracle.OracleEngine(spec, 8)
+    r.integrate_begin()
+    r.integrate_cd(K)
+    out = compare_state(runs[16], r, tag="2D light contact kernel")
+    print(f"[light-2d] nodes={spec.n_nodes} {out}", flush=True)
+    assert runs[16].field(capi.FIELD_Z).max() > 0.0, "no contact deformation in the test window"
+    for fid in (capi.FIELD_F, capi.FIELD_U, capi.FIELD_V):
+        np.testing.assert_array_equal(runs[16].field(fid), runs[0].field(fid))
+    with pytest.raises(RuntimeError):
+        runs[0].set_contact_light_max(17)
