@@ -937,6 +937,11 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     if (nonwall.empty()) nonwall.push_back(0);
     s.wall_gid = c->upload(wgid.data(), wgid.size());
     s.nonwall_pid = c->upload(nonwall.data(), nonwall.size());
+    {
+      std::vector<int64_t> cn(nonwall.size());
+      for (size_t q = 0; q < nonwall.size(); q++) cn[q] = c->hp[nonwall[q]].center_node;
+      s.nonwall_center = c->upload(cn.data(), cn.size());
+    }
     s.wbits = c->dalloc<unsigned>((size_t)std::max(1, s.n_nonwall) * std::max(1, s.nwords));
     std::vector<int> lo_init(std::max(1, s.n_nonwall), INT_MAX), hi_init(std::max(1, s.n_nonwall), -1);
     s.wlo = c->upload(lo_init.data(), lo_init.size());

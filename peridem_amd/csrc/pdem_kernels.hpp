@@ -126,6 +126,7 @@ struct DS {
   // damping
   int n_nonwall;
   const int* nonwall_pid;
+  const int64_t* nonwall_center;  // centre node of each non-wall particle
   int nwall, nwords;
   const int* widx;
   const int* wall_gid;
@@ -437,7 +438,7 @@ __global__ void __launch_bounds__(1024) k_ctrl(DS s, int advance) {
   if (advance) return;  // centre data packed by the fused node update
   const int nn = s.n_nonwall;
   for (int q = blockIdx.x * blockDim.x + t; q < nn; q += gridDim.x * blockDim.x) {
-    const int64_t cn = s.P[s.nonwall_pid[q]].center;
+    const int64_t cn = s.nonwall_center[q];
     for (int d = 0; d < 3; d++) {
       s.pcx[d * nn + q] = s.x[d] ? s.x[d][cn] : 0.;
       s.pcv[d * nn + q] = s.v[d] ? s.v[d][cn] : 0.;
