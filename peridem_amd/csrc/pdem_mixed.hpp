@@ -105,49 +105,47 @@ __device__ __forceinline__ int block_id(const uint4& b, int t) {
 }
 
 // node data access by particle-relative id jp (node gs + jp): LDS window or HBM.
-// LDS arrays of the three components are W apart (the window class plus 16
-// entries), which shifts a component's banks against the next one's.
+// LDS displacement arrays of the three components are W apart (the window
+// class plus 16 entries), which shifts a component's banks against the next one's.
 template <int NODES>
 constexpr int lds_stride() { return NODES + 16; }
+// LDS per window node: (position offset, volume) as one float4 and the force
+// pass's (alpha, b) terms as one float2, one load each; displacements FP64 in
+// three component arrays.
 template <int DIM, bool LDS, int W>
 struct Win {
-  const float* lx;
+  const float4* lxv;
   const double* lu;
-  const float* lv;
-  const float* la;
-  const float* lb;
+  const float2* lab;
   const DS* s;
   const MixedArgs* m;
   int gs;   // particle's first node
   int off;  // gs - window start: LDS index of jp
   double org[3];
-  __device__ __forceinline__ float x(int d, int jp) const {
-    return LDS ? lx[d * W + jp + off] : (float)(s->xr[d][gs + jp] - org[d]);
+  __device__ __forceinline__ float4 xv(int jp) const {
+    if (LDS) return lxv[jp + off];
+    const int64_t j = gs + jp;
+    return make_float4((float)(s->xr[0][j] - org[0]), (float)(s->xr[1][j] - org[1]),
+                       DIM == 3 ? (float)(s->xr[2][j] - org[2]) : 0.f, (float)s->vol[j]);
   }
   __device__ __forceinline__ double u(int d, int jp) const {
     return LDS ? lu[d * W + jp + off] : s->u[d][gs + jp];
   }
-  __device__ __forceinline__ float v(int jp) const { return LDS ? lv[jp + off] : (float)s->vol[gs + jp]; }
-  __device__ __forceinline__ float a(int jp) const { return LDS ? la[jp + off] : m->fa[gs + jp]; }
-  __device__ __forceinline__ float b(int jp) const { return LDS ? lb[jp + off] : m->fb[gs + jp]; }
+  __device__ __forceinline__ float2 ab(int jp) const {
+    return LDS ? lab[jp + off] : make_float2(m->fa[gs + jp], m->fb[gs + jp]);
+  }
 };
 
 template <int DIM, bool FORCE, int W>
 __device__ __forceinline__ void stage_window(const DS& s, const MixedArgs& m, const DevTile& t,
-                                             float* lx, double* lu, float* lv, float* la,
-                                             float* lb, const double* org) {
+                                             float4* lxv, double* lu, float2* lab, const double* org) {
   const int nw = t.we - t.ws;
   for (int q = threadIdx.x; q < nw; q += blockDim.x) {
     const int j = t.ws + q;
-    for (int d = 0; d < DIM; d++) {
-      lx[d * W + q] = (float)(s.xr[d][j] - org[d]);
-      lu[d * W + q] = s.u[d][j];
-    }
-    lv[q] = (float)s.vol[j];
-    if (FORCE) {
-      la[q] = m.fa[j];
-      lb[q] = m.fb[j];
-    }
+    for (int d = 0; d < DIM; d++) lu[d * W + q] = s.u[d][j];
+    lxv[q] = make_float4((float)(s.xr[0][j] - org[0]), (float)(s.xr[1][j] - org[1]),
+                         DIM == 3 ? (float)(s.xr[2][j] - org[2]) : 0.f, (float)s.vol[j]);
+    if (FORCE) lab[q] = make_float2(m.fa[j], m.fb[j]);
   }
   __syncthreads();
 }
@@ -157,15 +155,14 @@ template <int DIM, int INF, bool LDS, int TPB, int NODES = LDS_NODES>
 __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_dilation(DS s, MixedArgs m) {
 #pragma clang fp contract(fast)
   constexpr int W = lds_stride<NODES>();
-  __shared__ float lx[LDS ? 3 * W : 1];
+  __shared__ float4 lxv[LDS ? NODES : 1];
   __shared__ double lu[LDS ? 3 * W : 1];
-  __shared__ float lv[LDS ? NODES : 1];
   const DevTile t = m.tiles[LDS ? blockIdx.x : m.ntiles_lds + blockIdx.x];
   const DevParticle& p = s.P[t.pid];
   const int gs = (int)p.glob_start;
-  Win<DIM, LDS, W> w{lx, lu, lv, nullptr, nullptr, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
+  Win<DIM, LDS, W> w{lxv, lu, nullptr, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
   for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][gs];
-  if (LDS) stage_window<DIM, false, W>(s, m, t, lx, lu, lv, nullptr, nullptr, w.org);
+  if (LDS) stage_window<DIM, false, W>(s, m, t, lxv, lu, nullptr, w.org);
   const float horizon = (float)p.horizon, h = (float)p.h;
   const float check_up = horizon + 0.5f * h, check_low = horizon - 0.5f * h;
   const float low_rel = check_low / REL_KEEP;             // r > low_rel: isGreater(r, check_low)
@@ -179,12 +176,10 @@ __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_dilation(DS
     const uint4* cb = (const uint4*)(m.col16 + m.sptr16[q / WAVE] + (q % WAVE) * KB);
     unsigned* bw = s.bits + s.bptr[i / WAVE] + i % WAVE;
     const int n = s.deg[i];
-    float xi[3];
+    const float4 xvi = w.xv(ip);
+    const float xi[3] = {xvi.x, xvi.y, xvi.z};
     double ui[3];
-    for (int d = 0; d < DIM; d++) {
-      xi[d] = w.x(d, ip);
-      ui[d] = w.u(d, ip);
-    }
+    for (int d = 0; d < DIM; d++) ui[d] = w.u(d, ip);
     double theta = 0.;
     unsigned word = 0, wnew = 0;
     uint4 blk = n > 0 ? cb[0] : make_uint4(0, 0, 0, 0);
@@ -200,12 +195,14 @@ __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_dilation(DS
         const int k = kb + tt;
         if (k < n) {
           const int jp = block_id(cur, tt);
+          const float4 xvj = w.xv(jp);
+          const float xj[3] = {xvj.x, xvj.y, xvj.z};
           float dx[3], du[3];
           for (int d = 0; d < DIM; d++) {
-            dx[d] = w.x(d, jp) - xi[d];
+            dx[d] = xj[d] - xi[d];
             du[d] = (float)(w.u(d, jp) - ui[d]);
           }
-          float volj = w.v(jp);
+          float volj = xvj.w;
           float xx = dx[0] * dx[0], xe = dx[0] * du[0], ee = du[0] * du[0];
           for (int d = 1; d < DIM; d++) {
             xx += dx[d] * dx[d];
@@ -250,17 +247,15 @@ template <int DIM, int INF, bool LDS, int TPB, int NODES = LDS_NODES>
 __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_force(DS s, MixedArgs m) {
 #pragma clang fp contract(fast)
   constexpr int W = lds_stride<NODES>();
-  __shared__ float lx[LDS ? 3 * W : 1];
+  __shared__ float4 lxv[LDS ? NODES : 1];
   __shared__ double lu[LDS ? 3 * W : 1];
-  __shared__ float lv[LDS ? NODES : 1];
-  __shared__ float la[LDS ? NODES : 1];
-  __shared__ float lb[LDS ? NODES : 1];
+  __shared__ float2 lab[LDS ? NODES : 1];
   const DevTile t = m.tiles[LDS ? blockIdx.x : m.ntiles_lds + blockIdx.x];
   const DevParticle& p = s.P[t.pid];
   const int gs = (int)p.glob_start;
-  Win<DIM, LDS, W> w{lx, lu, lv, la, lb, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
+  Win<DIM, LDS, W> w{lxv, lu, lab, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
   for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][gs];
-  if (LDS) stage_window<DIM, true, W>(s, m, t, lx, lu, lv, la, lb, w.org);
+  if (LDS) stage_window<DIM, true, W>(s, m, t, lxv, lu, lab, w.org);
   const float horizon = (float)p.horizon, h = (float)p.h;
   const float check_up = horizon + 0.5f * h, check_low = horizon - 0.5f * h;
   const float low_rel = check_low / REL_KEEP;  // r > low_rel: isGreater(r, check_low)
@@ -274,13 +269,12 @@ __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_force(DS s,
     const uint4* cb = (const uint4*)(m.col16 + m.sptr16[q / WAVE] + (q % WAVE) * KB);
     const unsigned* bw = s.bits + s.bptr[i / WAVE] + i % WAVE;
     const int n = s.deg[i];
-    float xi[3];
+    const float4 xvi = w.xv(ip);
+    const float xi[3] = {xvi.x, xvi.y, xvi.z};
     double ui[3];
-    for (int d = 0; d < DIM; d++) {
-      xi[d] = w.x(d, ip);
-      ui[d] = w.u(d, ip);
-    }
-    const float ai = w.a(ip), bi = w.b(ip);
+    for (int d = 0; d < DIM; d++) ui[d] = w.u(d, ip);
+    const float2 abi = w.ab(ip);
+    const float ai = abi.x, bi = abi.y;
     double force[3] = {0., 0., 0.};
     float Zs = 0.f;  // maximum of |S|; Z = Zs / s0
     unsigned word = 0;
@@ -294,13 +288,15 @@ __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_force(DS s,
         const int k = kb + tt;
         if (k < n) {
           const int jp = block_id(cur, tt);
+          const float4 xvj = w.xv(jp);
+          const float xj[3] = {xvj.x, xvj.y, xvj.z};
           float dx[3], du[3], y[3];
           for (int d = 0; d < DIM; d++) {
-            dx[d] = w.x(d, jp) - xi[d];
+            dx[d] = xj[d] - xi[d];
             du[d] = (float)(w.u(d, jp) - ui[d]);
             y[d] = dx[d] + du[d];
           }
-          float volj = w.v(jp);
+          float volj = xvj.w;
           float xx = dx[0] * dx[0], xe = dx[0] * du[0], ee = du[0] * du[0];
           for (int d = 1; d < DIM; d++) {
             xx += dx[d] * dx[d];
@@ -317,7 +313,8 @@ __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_force(DS s,
           if (!fs) {
             if (r > low_rel) volj *= up_h - r * inv_h;
             const float Jr = J(r, xx);
-            sf = Jr * (r * (bi + w.b(jp)) + cl * (ai + w.a(jp))) * volj * inv_ly;
+            const float2 abj = w.ab(jp);
+            sf = Jr * (r * (bi + abj.y) + cl * (ai + abj.x)) * volj * inv_ly;
           } else {
             sf = Kn * volj * (ly - Rc) * inv_ly;
             if (sf > 0.f) sf = 0.f;
