@@ -7,10 +7,10 @@
 // Work unit: a tile = up to TILE_NODES consecutive nodes of one particle.
 // Horizons never cross particles, so a tile's neighbors lie in a window of
 // node ids [ws, we) inside its particle.  LDS tiles (the window fits) stage
-// the window once per workgroup (structure of arrays, 48 B per node: FP32
+// the window once per workgroup (48 B per node: a float4 of the FP32
 // position offset from the particle's first reference node (independent of
-// the tile shape), FP64 displacement,
-// FP32 volume, FP32 dilation terms) and every bond reads LDS; the remaining
+// the tile shape) and the volume, the FP64 displacement in three component
+// arrays, a float2 of the dilation terms) and every bond reads LDS; the remaining
 // tiles (particles whose numbering spreads neighbors over more ids than the
 // window holds) gather from HBM through the caches: a separate
 // instantiation, launched on a second stream beside the LDS tiles.
