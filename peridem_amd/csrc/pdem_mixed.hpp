@@ -9,8 +9,8 @@
 // node ids [ws, we) inside its particle.  LDS tiles (the window fits) stage
 // the window once per workgroup (48 B per node: a float4 of the FP32
 // position offset from the particle's first reference node (independent of
-// the tile shape) and the volume, the FP64 displacement in three component
-// arrays, a float2 of the dilation terms) and every bond reads LDS; the remaining
+// the tile shape) and the volume, the FP64 displacement as a double2 and a
+// double, a float2 of the dilation terms) and every bond reads LDS; the remaining
 // tiles (particles whose numbering spreads neighbors over more ids than the
 // window holds) gather from HBM through the caches: a separate
 // instantiation, launched on a second stream beside the LDS tiles.
@@ -105,17 +105,11 @@ __device__ __forceinline__ int block_id(const uint4& b, int t) {
 }
 
 // node data access by particle-relative id jp (node gs + jp): LDS window or HBM.
-// LDS displacement arrays of the three components are W apart (the window
-// class plus 16 entries), which shifts a component's banks against the next one's.
-template <int NODES>
-constexpr int lds_stride() { return NODES + 16; }
-// LDS per window node: (position offset, volume) as one float4 and the force
-// pass's (alpha, b) terms as one float2, one load each; displacements FP64 in
-// three component arrays.
-template <int DIM, bool LDS, int W>
+template <int DIM, bool LDS>
 struct Win {
   const float4* lxv;
-  const double* lu;
+  const double2* lu01;
+  const double* lu2;
   const float2* lab;
   const DS* s;
   const MixedArgs* m;
@@ -128,21 +122,30 @@ struct Win {
     return make_float4((float)(s->xr[0][j] - org[0]), (float)(s->xr[1][j] - org[1]),
                        DIM == 3 ? (float)(s->xr[2][j] - org[2]) : 0.f, (float)s->vol[j]);
   }
-  __device__ __forceinline__ double u(int d, int jp) const {
-    return LDS ? lu[d * W + jp + off] : s->u[d][gs + jp];
+  __device__ __forceinline__ void u(int jp, double* out) const {
+    if (LDS) {
+      const double2 q = lu01[jp + off];
+      out[0] = q.x;
+      out[1] = q.y;
+      if (DIM == 3) out[2] = lu2[jp + off];
+    } else {
+      for (int d = 0; d < DIM; d++) out[d] = s->u[d][gs + jp];
+    }
   }
   __device__ __forceinline__ float2 ab(int jp) const {
     return LDS ? lab[jp + off] : make_float2(m->fa[gs + jp], m->fb[gs + jp]);
   }
 };
 
-template <int DIM, bool FORCE, int W>
+template <int DIM, bool FORCE>
 __device__ __forceinline__ void stage_window(const DS& s, const MixedArgs& m, const DevTile& t,
-                                             float4* lxv, double* lu, float2* lab, const double* org) {
+                                             float4* lxv, double2* lu01, double* lu2, float2* lab,
+                                             const double* org) {
   const int nw = t.we - t.ws;
   for (int q = threadIdx.x; q < nw; q += blockDim.x) {
     const int j = t.ws + q;
-    for (int d = 0; d < DIM; d++) lu[d * W + q] = s.u[d][j];
+    lu01[q] = make_double2(s.u[0][j], s.u[1][j]);
+    if (DIM == 3) lu2[q] = s.u[2][j];
     lxv[q] = make_float4((float)(s.xr[0][j] - org[0]), (float)(s.xr[1][j] - org[1]),
                          DIM == 3 ? (float)(s.xr[2][j] - org[2]) : 0.f, (float)s.vol[j]);
     if (FORCE) lab[q] = make_float2(m.fa[j], m.fb[j]);
@@ -154,15 +157,15 @@ __device__ __forceinline__ void stage_window(const DS& s, const MixedArgs& m, co
 template <int DIM, int INF, bool LDS, int TPB, int NODES = LDS_NODES>
 __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_dilation(DS s, MixedArgs m) {
 #pragma clang fp contract(fast)
-  constexpr int W = lds_stride<NODES>();
   __shared__ float4 lxv[LDS ? NODES : 1];
-  __shared__ double lu[LDS ? 3 * W : 1];
+  __shared__ double2 lu01[LDS ? NODES : 1];
+  __shared__ double lu2[LDS && DIM == 3 ? NODES : 1];
   const DevTile t = m.tiles[LDS ? blockIdx.x : m.ntiles_lds + blockIdx.x];
   const DevParticle& p = s.P[t.pid];
   const int gs = (int)p.glob_start;
-  Win<DIM, LDS, W> w{lxv, lu, nullptr, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
+  Win<DIM, LDS> w{lxv, lu01, lu2, nullptr, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
   for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][gs];
-  if (LDS) stage_window<DIM, false, W>(s, m, t, lxv, lu, nullptr, w.org);
+  if (LDS) stage_window<DIM, false>(s, m, t, lxv, lu01, lu2, nullptr, w.org);
   const float horizon = (float)p.horizon, h = (float)p.h;
   const float check_up = horizon + 0.5f * h, check_low = horizon - 0.5f * h;
   const float low_rel = check_low / REL_KEEP;             // r > low_rel: isGreater(r, check_low)
@@ -179,7 +182,7 @@ __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_dilation(DS
     const float4 xvi = w.xv(ip);
     const float xi[3] = {xvi.x, xvi.y, xvi.z};
     double ui[3];
-    for (int d = 0; d < DIM; d++) ui[d] = w.u(d, ip);
+    w.u(ip, ui);
     double theta = 0.;
     unsigned word = 0, wnew = 0;
     uint4 blk = n > 0 ? cb[0] : make_uint4(0, 0, 0, 0);
@@ -197,10 +200,12 @@ __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_dilation(DS
           const int jp = block_id(cur, tt);
           const float4 xvj = w.xv(jp);
           const float xj[3] = {xvj.x, xvj.y, xvj.z};
+          double uj[3];
+          w.u(jp, uj);
           float dx[3], du[3];
           for (int d = 0; d < DIM; d++) {
             dx[d] = xj[d] - xi[d];
-            du[d] = (float)(w.u(d, jp) - ui[d]);
+            du[d] = (float)(uj[d] - ui[d]);
           }
           float volj = xvj.w;
           float xx = dx[0] * dx[0], xe = dx[0] * du[0], ee = du[0] * du[0];
@@ -246,16 +251,16 @@ __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_dilation(DS
 template <int DIM, int INF, bool LDS, int TPB, int NODES = LDS_NODES>
 __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_force(DS s, MixedArgs m) {
 #pragma clang fp contract(fast)
-  constexpr int W = lds_stride<NODES>();
   __shared__ float4 lxv[LDS ? NODES : 1];
-  __shared__ double lu[LDS ? 3 * W : 1];
+  __shared__ double2 lu01[LDS ? NODES : 1];
+  __shared__ double lu2[LDS && DIM == 3 ? NODES : 1];
   __shared__ float2 lab[LDS ? NODES : 1];
   const DevTile t = m.tiles[LDS ? blockIdx.x : m.ntiles_lds + blockIdx.x];
   const DevParticle& p = s.P[t.pid];
   const int gs = (int)p.glob_start;
-  Win<DIM, LDS, W> w{lxv, lu, lab, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
+  Win<DIM, LDS> w{lxv, lu01, lu2, lab, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
   for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][gs];
-  if (LDS) stage_window<DIM, true, W>(s, m, t, lxv, lu, lab, w.org);
+  if (LDS) stage_window<DIM, true>(s, m, t, lxv, lu01, lu2, lab, w.org);
   const float horizon = (float)p.horizon, h = (float)p.h;
   const float check_up = horizon + 0.5f * h, check_low = horizon - 0.5f * h;
   const float low_rel = check_low / REL_KEEP;  // r > low_rel: isGreater(r, check_low)
@@ -272,7 +277,7 @@ __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_force(DS s,
     const float4 xvi = w.xv(ip);
     const float xi[3] = {xvi.x, xvi.y, xvi.z};
     double ui[3];
-    for (int d = 0; d < DIM; d++) ui[d] = w.u(d, ip);
+    w.u(ip, ui);
     const float2 abi = w.ab(ip);
     const float ai = abi.x, bi = abi.y;
     double force[3] = {0., 0., 0.};
@@ -290,10 +295,12 @@ __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_force(DS s,
           const int jp = block_id(cur, tt);
           const float4 xvj = w.xv(jp);
           const float xj[3] = {xvj.x, xvj.y, xvj.z};
+          double uj[3];
+          w.u(jp, uj);
           float dx[3], du[3], y[3];
           for (int d = 0; d < DIM; d++) {
             dx[d] = xj[d] - xi[d];
-            du[d] = (float)(w.u(d, jp) - ui[d]);
+            du[d] = (float)(uj[d] - ui[d]);
             y[d] = dx[d] + du[d];
           }
           float volj = xvj.w;
