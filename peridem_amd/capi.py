@@ -613,8 +613,9 @@ class Mlp:
         return a, a.ctypes.data_as(C.POINTER(C.c_float))
 
     def close(self):
+        """Free the network; raises while an engine context still has it enabled."""
         if self.h:
-            self.lib.pdem_mlp_destroy(self.h)
+            self._check(self.lib.pdem_mlp_destroy(self.h), "destroy")
             self.h = C.c_void_p()
 
     def __del__(self):

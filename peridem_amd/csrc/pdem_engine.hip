@@ -36,6 +36,7 @@ constexpr int64_t LIGHT_2D_MIN_NODES = int64_t(1) << 20;  // 2D models from this
 constexpr int64_t FUSE_MAX_NODES = (int64_t)1 << 20;
 
 struct pdem_ctx {
+  int device = 0;  // HIP device current at pdem_create
   int dim = 2, precision = 0, multi = 0;
   int64_t N = 0;
   int np = 0, nz = 0;
@@ -117,7 +118,16 @@ struct pdem_ctx {
     if (n) HIPCHK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
     return d;
   }
+  // Teardown order: every launch on both streams completes first (a replayed
+  // graph and its kernel arguments, events recorded by the profiling path and
+  // the buffers they read stay valid until then), then the graph and the
+  // events go, then the memory, then the streams.
   ~pdem_ctx() {
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    if (cur != device) (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (side) (void)hipStreamSynchronize(side);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     for (auto& e : ev_used) {
@@ -128,13 +138,14 @@ struct pdem_ctx {
       (void)hipEventDestroy(e.first);
       (void)hipEventDestroy(e.second);
     }
-    if (stream) (void)hipStreamSynchronize(stream);
-    for (void* p : allocs) (void)hipFree(p);
-    if (side) (void)hipStreamSynchronize(side);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
+    if (cl_net) cl_net->users.fetch_sub(1);
+    cl_net = nullptr;
+    for (void* p : allocs) (void)hipFree(p);
     if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
+    if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
   }
 };
 
@@ -637,6 +648,7 @@ void build(pdem_ctx* c, const pdem_desc* d) {
   c->dim = dim;
   c->precision = d->precision;
   c->multi = d->multi_particle;
+  HIPCHK(hipGetDevice(&c->device));
   c->np = d->n_particles;
   c->nz = d->n_zones;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1752,12 +1764,17 @@ int pdem_closure_enable(pdem_ctx* c, const pdem_mlp* net, double pen_max_h, doub
       if (dev != net->device) throw HipError{"closure network lives on another device"};
     }
     if (check_every < 0) throw HipError{"check_every must be >= 0"};
+    HIPCHK(hipStreamSynchronize(c->stream));  // steps in flight still read the previous network
     c->ca.use_net = (net && pen_max_h > 0.) ? 1 : 0;
     c->ca.net = net ? mlp::make_net(net->wb, net->wtb, net->p, net->norm) : mlp::Net{};
     c->ca.pen_max_h = pen_max_h;
     c->ca.dz_max = (float)dz_max;
     c->ca.check_every = check_every;
-    c->cl_net = net;
+    if (net != c->cl_net) {
+      if (net) net->users.fetch_add(1);
+      if (c->cl_net) c->cl_net->users.fetch_sub(1);
+      c->cl_net = net;
+    }
     if (net) HIPCHK(hipStreamSynchronize(net->stream));  // training finished before use
     HIPCHK(hipMemsetAsync(c->ca.stats, 0, 8 * sizeof(unsigned long long), c->stream));
     HIPCHK(hipMemsetAsync(c->ca.err_sum, 0, 2 * sizeof(double), c->stream));
@@ -1769,9 +1786,13 @@ int pdem_closure_enable(pdem_ctx* c, const pdem_mlp* net, double pen_max_h, doub
 
 int pdem_closure_disable(pdem_ctx* c) {
   return guarded(c, [&] {
+    HIPCHK(hipStreamSynchronize(c->stream));  // steps in flight still read the network
     c->ds.cl_on = 0;
-    c->cl_net = nullptr;
     drop_graph(c);
+    if (c->cl_net) c->cl_net->users.fetch_sub(1);
+    c->cl_net = nullptr;
+    c->ca.net = mlp::Net{};
+    c->ca.use_net = 0;
   });
 }
 

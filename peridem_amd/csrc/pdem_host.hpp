@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
+#include <atomic>
 #include <string>
 
 struct HipError {
@@ -36,5 +37,8 @@ struct pdem_mlp {
   float* partial = nullptr;  // per-workgroup gradient partials
   int64_t partial_wg = 0;
   int64_t adam_step = 0;
+  // engine contexts whose closure reads this network (their captured steps
+  // hold its buffers): pdem_mlp_destroy refuses while it is nonzero
+  mutable std::atomic<int> users{0};
   std::string err;
 };

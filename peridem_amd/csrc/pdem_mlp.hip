@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "pdem.h"
@@ -249,6 +250,11 @@ int pdem_mlp_create(uint64_t seed, pdem_mlp** out) {
 
 int pdem_mlp_destroy(pdem_mlp* m) {
   if (!m) return 0;
+  if (const int u = m->users.load()) {
+    m->err = "network still enabled in " + std::to_string(u) +
+             " engine context(s): call pdem_closure_disable or pdem_destroy first";
+    return -1;
+  }
   (void)hipSetDevice(m->device);
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   for (void* q : {(void*)m->p, (void*)m->m, (void*)m->v, (void*)m->wb, (void*)m->wtb, (void*)m->norm,
