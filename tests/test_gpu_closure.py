@@ -172,3 +172,32 @@ def test_closure_tracks_plain_without_friction():
     assert st["closure"] > 0
     assert st["check_err_mean"] < 0.2 * st["check_plain_mean"]
     assert dev < 0.05
+
+
+def test_contexts_with_closure_back_to_back():
+    """Create, enable the closure with a network, replay steps, destroy --
+    twice in a row on one process, with the network outliving both contexts;
+    destroying the network while a context has it enabled is refused."""
+    spec = _compressive()
+    net = capi.Mlp(seed=2)
+    try:
+        results = []
+        for rep in range(2):
+            print(f"[closure-life] round {rep}: create", flush=True)
+            e = capi.GpuEngine(spec, capi.PREC_FP64)
+            e.closure_setup(closure.particle_codes(spec))
+            e.closure_enable(net, pen_max_h=0.05, check_every=10)
+            if rep == 0:
+                with pytest.raises(RuntimeError, match="still enabled"):
+                    net.close()
+                assert net.h
+            e.integrate_begin()
+            e.integrate_cd_graph(200)  # returns without waiting for the replays
+            results.append(e.field(capi.FIELD_U))
+            e.integrate_cd_graph(50)
+            e.close()  # replays may still run: the context waits for them
+            print(f"[closure-life] round {rep}: destroyed", flush=True)
+        np.testing.assert_array_equal(results[0], results[1])
+    finally:
+        net.close()  # no context uses it any more
+    assert not net.h
