@@ -171,6 +171,33 @@ typedef struct pdem_contact_state {
 
 typedef struct pdem_ctx pdem_ctx;
 
+/* The deck values a particle's material constants derive from: the fields
+ * of the zone's inp::MaterialDeck (src/inp/decks/materialDeck.h:230-301) and
+ * the particle's horizon. */
+typedef struct pdem_material_deck {
+  int32_t material_type;          /* PDEM_MAT_* (Material.Type) */
+  int32_t dim;                    /* 2 or 3 */
+  int32_t compute_from_classical; /* d_computeParamsFromElastic */
+  int32_t influence_fn_type;      /* PDEM_INF_*: the process-global influence function */
+  int32_t n_influence_fn_params;  /* 0: the reference's defaults */
+  int32_t n_bond_potential_params;
+  double influence_fn_params[2];
+  double E, K, G, nu, Gc, KIc;    /* d_matData; negative: not given */
+  double horizon;                 /* the particle's horizon delta */
+  double check_sc_factor;         /* d_checkScFactor (RNP) */
+  double bond_potential_params[3];/* used when compute_from_classical is 0 */
+} pdem_material_deck;
+
+/* pdem_particle.mat for one particle, as the reference's material
+ * constructors compute their private constants (RnpMaterial, PdState,
+ * PmbMaterial, PdElastic: material.h:258-311, 541-620, 674-716, 925-1000,
+ * 1034-1076, 1260-1306, 1336-1380, 1580-1700), and K_contact, the bulk
+ * modulus of computeMaterialProperties(dim) that gives the particle's Kn
+ * (baseParticle.cpp:170-179).  Host code: needs no device and no context.
+ * Returns 0, or -1 with the reason in pdem_material_last_error(). */
+int pdem_material_constants(const pdem_material_deck *deck, double *mat, double *K_contact);
+const char *pdem_material_last_error(void);
+
 /* Build the device model: upload state, build the PD neighbor list on device
  * (replaces DEMModel::updatePeridynamicNeighborlist, demModel.cpp:1659-1694),
  * create the bond-state bits (geometry::Fracture ctor, fracture.cpp:20-49),
@@ -401,7 +428,8 @@ int pdem_checkpoint_load(pdem_ctx *ctx, const void *buf, int64_t bytes);
 int pdem_device_count(void);
 
 /* sizeof of the ABI structures, for binding checks:
- * 0 pdem_desc, 1 pdem_particle, 2 pdem_contact_pair, 3 pdem_bc_set, 4 pdem_contact_state */
+ * 0 pdem_desc, 1 pdem_particle, 2 pdem_contact_pair, 3 pdem_bc_set, 4 pdem_contact_state,
+ * 5 pdem_material_deck */
 int64_t pdem_abi_sizeof(int32_t which);
 
 #ifdef __cplusplus

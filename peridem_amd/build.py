@@ -26,7 +26,7 @@ def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     out = os.path.join(LIBDIR, "libpdem_hip.so")
     tmp = out + ".tmp"
-    srcs = [os.path.join(CSRC, f) for f in ("pdem_engine.hip", "pdem_mlp.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("pdem_engine.hip", "pdem_mlp.hip", "pdem_material.cpp")]
     cmd = [HIPCC] + FLAGS + srcs + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)

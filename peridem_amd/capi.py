@@ -81,6 +81,40 @@ class ContactState(C.Structure):
                 ("overflow", C.c_int64), ("capacity", C.c_int64)]
 
 
+class MaterialDeckC(C.Structure):
+    """pdem_material_deck: the deck values of pdem_material_constants."""
+    _fields_ = [("material_type", C.c_int32), ("dim", C.c_int32), ("compute_from_classical", C.c_int32),
+                ("influence_fn_type", C.c_int32), ("n_influence_fn_params", C.c_int32),
+                ("n_bond_potential_params", C.c_int32), ("influence_fn_params", C.c_double * 2),
+                ("E", C.c_double), ("K", C.c_double), ("G", C.c_double), ("nu", C.c_double),
+                ("Gc", C.c_double), ("KIc", C.c_double), ("horizon", C.c_double),
+                ("check_sc_factor", C.c_double), ("bond_potential_params", C.c_double * 3)]
+
+
+def material_constants(material_type, dim, horizon, E=-1.0, K=-1.0, G=-1.0, nu=-1.0, Gc=-1.0, KIc=-1.0,
+                       influence_fn_type=INF_CONST, influence_fn_params=(), check_sc_factor=1.0,
+                       compute_from_classical=True, bond_potential_params=()):
+    """pdem_material_constants: (mat[6], K_contact) of one particle, computed by
+    the engine library's host code (what a C++ DEMModel subclass calls)."""
+    lib = engine_library()
+    d = MaterialDeckC()
+    d.material_type, d.dim, d.compute_from_classical = material_type, dim, int(compute_from_classical)
+    d.influence_fn_type = influence_fn_type
+    d.n_influence_fn_params = len(influence_fn_params)
+    for k, v in enumerate(influence_fn_params):
+        d.influence_fn_params[k] = v
+    d.n_bond_potential_params = len(bond_potential_params)
+    for k, v in enumerate(bond_potential_params):
+        d.bond_potential_params[k] = v
+    d.E, d.K, d.G, d.nu, d.Gc, d.KIc = E, K, G, nu, Gc, KIc
+    d.horizon, d.check_sc_factor = horizon, check_sc_factor
+    mat = (C.c_double * 6)()
+    kc = C.c_double()
+    if lib.pdem_material_constants(C.byref(d), mat, C.byref(kc)) != 0:
+        raise ValueError(lib.pdem_material_last_error().decode())
+    return list(mat), kc.value
+
+
 def _ptr(a: np.ndarray, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
@@ -389,6 +423,11 @@ def engine_library():
         lib.pdem_pair_log.restype = C.c_int
         lib.pdem_pair_log.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.POINTER(C.c_float),
                                       C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int64)]
+        lib.pdem_material_constants.restype = C.c_int
+        lib.pdem_material_constants.argtypes = [C.POINTER(MaterialDeckC), C.POINTER(C.c_double),
+                                                C.POINTER(C.c_double)]
+        lib.pdem_material_last_error.restype = C.c_char_p
+        lib.pdem_material_last_error.argtypes = []
         lib.pdem_checkpoint_size.restype = C.c_int
         lib.pdem_checkpoint_size.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         lib.pdem_checkpoint_save.restype = C.c_int

@@ -1108,6 +1108,7 @@ int64_t pdem_abi_sizeof(int32_t which) {
     case 2: return (int64_t)sizeof(pdem_contact_pair);
     case 3: return (int64_t)sizeof(pdem_bc_set);
     case 4: return (int64_t)sizeof(pdem_contact_state);
+    case 5: return (int64_t)sizeof(pdem_material_deck);
     default: return -1;
   }
 }

@@ -31,7 +31,7 @@ def test_struct_layouts_match_binding():
     lib.pdem_abi_sizeof.restype = C.c_int64
     lib.pdem_abi_sizeof.argtypes = [C.c_int32]
     for which, st in enumerate([capi.Desc, capi.Particle, capi.ContactPair, capi.BcSet,
-                                capi.ContactState]):
+                                capi.ContactState, capi.MaterialDeckC]):
         assert lib.pdem_abi_sizeof(which) == C.sizeof(st), st.__name__
     assert lib.pdem_abi_sizeof(99) == -1
 
