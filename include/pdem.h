@@ -413,6 +413,19 @@ int pdem_closure_stats(pdem_ctx *ctx, int64_t *pairs_by_mode3, int64_t *checked,
 int pdem_pair_log(pdem_ctx *ctx, double band_h, int64_t cap, float *features, float *targets, float *baseline,
                   int64_t *n);
 
+/* pdem_pair_log without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight logging loops, hosts that forbid hipMalloc).  The
+ * rows are staged in a library-owned device array of
+ * PDEM_PAIR_LOG_NOALLOC_MAX_ROWS rows (6.0 MB per GPU, held until the process
+ * ends) instead of buffers allocated per call; cap above that maximum is an
+ * error.  Same rows as pdem_pair_log (in the order the kernel appends them,
+ * which may differ between calls).  Every call on one GPU shares the array:
+ * calls that overlap on one GPU, from several contexts or threads, race. */
+#define PDEM_PAIR_LOG_NOALLOC_MAX_ROWS 65536
+int pdem_pair_log_noalloc(pdem_ctx *ctx, double band_h, int64_t cap, float *features, float *targets,
+                          float *baseline, int64_t *n);
+
 /* Checkpoint / restore of the complete step state (node fields, bond bits,
  * damage, the cadence controller's counters, the candidate and partner lists
  * of the last rebuild, zero-set flags): resuming from a checkpoint continues

@@ -423,6 +423,8 @@ def engine_library():
         lib.pdem_pair_log.restype = C.c_int
         lib.pdem_pair_log.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.POINTER(C.c_float),
                                       C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int64)]
+        lib.pdem_pair_log_noalloc.restype = C.c_int
+        lib.pdem_pair_log_noalloc.argtypes = lib.pdem_pair_log.argtypes
         lib.pdem_material_constants.restype = C.c_int
         lib.pdem_material_constants.argtypes = [C.POINTER(MaterialDeckC), C.POINTER(C.c_double),
                                                 C.POINTER(C.c_double)]
@@ -578,16 +580,23 @@ class GpuEngine(Engine):
         return {"no_contact": modes[0], "closure": modes[1], "plain": modes[2], "checked": chk.value,
                 "check_err_mean": em.value, "check_err_max": ex.value, "check_plain_mean": pm.value}
 
-    def pair_log(self, band_h=0.5, cap=1 << 20, baseline=False):
+    PAIR_LOG_NOALLOC_MAX_ROWS = 65536
+
+    def pair_log(self, band_h=0.5, cap=1 << 20, baseline=False, noalloc=False):
         """Training rows of the current state: features [n][16], plain targets
-        [n][4] and, with baseline=True, the normal-contact baseline [n][3]."""
+        [n][4] and, with baseline=True, the normal-contact baseline [n][3].
+        noalloc=True stages the rows in the library's fixed device array
+        (pdem_pair_log_noalloc, cap <= PAIR_LOG_NOALLOC_MAX_ROWS) instead of
+        device buffers allocated for the call."""
         f = np.empty((cap, 16), np.float32)
         t = np.empty((cap, 4), np.float32)
         b = np.empty((cap, 3), np.float32)
         n = C.c_int64()
         P = C.POINTER(C.c_float)
-        self._check(self.lib.pdem_pair_log(self.ctx, float(band_h), int(cap), f.ctypes.data_as(P),
-                                           t.ctypes.data_as(P), b.ctypes.data_as(P), C.byref(n)), "pair_log")
+        fn = self.lib.pdem_pair_log_noalloc if noalloc else self.lib.pdem_pair_log
+        self._check(fn(self.ctx, float(band_h), int(cap), f.ctypes.data_as(P),
+                       t.ctypes.data_as(P), b.ctypes.data_as(P), C.byref(n)),
+                    "pair_log_noalloc" if noalloc else "pair_log")
         m = min(n.value, cap)
         if baseline:
             return f[:m].copy(), t[:m].copy(), b[:m].copy()

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -31,12 +32,46 @@ namespace {
 thread_local std::string g_create_err;
 }  // namespace
 
+// Staging of pdem_pair_log_noalloc: a row counter (first 256 bytes) and up to
+// PDEM_PAIR_LOG_NOALLOC_MAX_ROWS rows of features, targets and baseline.
+// One copy per GPU (6.0 MB), held until the process ends; every call on that
+// GPU shares it.
+constexpr int64_t NOALLOC_ROWS = PDEM_PAIR_LOG_NOALLOC_MAX_ROWS;
+constexpr size_t NOALLOC_BYTES = 256 + (size_t)NOALLOC_ROWS * (CL_NFEAT + 4 + 3) * sizeof(float);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+// Device address of `scratch` per device.  The array's name in host code is a
+// host-side stand-in, not a device address, so kernels and copies take the
+// address looked up here (with the device current) and never the name.
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;
+
+void* scratch_address(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw HipError{"no device address for the staging array"};
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+
+__global__ void k_zero_u32(unsigned* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+}  // namespace
+
 // multi-particle models up to this many nodes use the fused step kernels
 constexpr int64_t LIGHT_2D_MIN_NODES = int64_t(1) << 20;  // 2D models from this size use the light-node contact kernel
 constexpr int64_t FUSE_MAX_NODES = (int64_t)1 << 20;
 
 struct pdem_ctx {
   int device = 0;  // HIP device current at pdem_create
+  void* scratch_dev = nullptr;  // device address of `scratch` on `device`
   int dim = 2, precision = 0, multi = 0;
   int64_t N = 0;
   int np = 0, nz = 0;
@@ -649,6 +684,7 @@ void build(pdem_ctx* c, const pdem_desc* d) {
   c->precision = d->precision;
   c->multi = d->multi_particle;
   HIPCHK(hipGetDevice(&c->device));
+  c->scratch_dev = scratch_address(c->device);
   c->np = d->n_particles;
   c->nz = d->n_zones;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1816,37 +1852,76 @@ int pdem_closure_stats(pdem_ctx* c, int64_t* pairs_by_mode, int64_t* checked, do
   });
 }
 
+}  // extern "C"
+
+namespace {
+// The pair-log launches and copies of pdem_pair_log and
+// pdem_pair_log_noalloc: rows staged in df / dt / db (room for at least cap
+// rows), the row counter in dn (cleared here by zero_count, which enqueues on the
+// engine stream); rows beyond cap are dropped, *n gets the number found.
+template <class Zero>
+void pair_log_staged(pdem_ctx* c, double band_h, int64_t cap, float* df, float* dt, float* db, unsigned* dn,
+                     Zero zero_count, float* feat, float* targ, float* base, int64_t* n) {
+  closure_lists(c);
+  zero_count();
+  DS t = c->ds;
+  t.cl_on = 1;
+  hipLaunchKernelGGL(k_particle_radius, dim3((unsigned)std::min<int64_t>((t.n_nonwall + 3) / 4, 1024)),
+                     dim3(256), 0, c->stream, t);
+  hipLaunchKernelGGL(k_pair_log, dim3(CL_BLOCKS), dim3(256), 0, c->stream, t, c->ca, band_h, df, dt, db, dn,
+                     (int)std::min<int64_t>(cap, INT_MAX));
+  check_launch(c);
+  unsigned cnt = 0;
+  HIPCHK(hipMemcpyAsync(&cnt, dn, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int64_t m = std::min<int64_t>(cnt, cap);
+  if (m > 0) {
+    HIPCHK(hipMemcpy(feat, df, m * CL_NFEAT * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(targ, dt, m * 4 * sizeof(float), hipMemcpyDeviceToHost));
+    if (base) HIPCHK(hipMemcpy(base, db, m * 3 * sizeof(float), hipMemcpyDeviceToHost));
+  }
+  *n = (int64_t)cnt;
+}
+}  // namespace
+
+extern "C" {
+
 int pdem_pair_log(pdem_ctx* c, double band_h, int64_t cap, float* feat, float* targ, float* base, int64_t* n) {
   return guarded(c, [&] {
     if (!c->cl_ready) throw HipError{"call pdem_closure_setup before pdem_pair_log"};
     if (cap < 0) throw HipError{"negative capacity"};
-    closure_lists(c);
     float* df = c->dalloc<float>((size_t)std::max<int64_t>(cap, 1) * CL_NFEAT);
     float* dt = c->dalloc<float>((size_t)std::max<int64_t>(cap, 1) * 4);
     float* db = c->dalloc<float>((size_t)std::max<int64_t>(cap, 1) * 3);
     unsigned* dn = c->dalloc<unsigned>(1, false);
-    HIPCHK(hipMemsetAsync(dn, 0, sizeof(unsigned), c->stream));
-    DS t = c->ds;
-    t.cl_on = 1;
-    hipLaunchKernelGGL(k_particle_radius, dim3((unsigned)std::min<int64_t>((t.n_nonwall + 3) / 4, 1024)),
-                       dim3(256), 0, c->stream, t);
-    hipLaunchKernelGGL(k_pair_log, dim3(CL_BLOCKS), dim3(256), 0, c->stream, t, c->ca, band_h, df, dt, db, dn,
-                       (int)std::min<int64_t>(cap, INT_MAX));
-    check_launch(c);
-    unsigned cnt = 0;
-    HIPCHK(hipMemcpyAsync(&cnt, dn, 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    const int64_t m = std::min<int64_t>(cnt, cap);
-    if (m > 0) {
-      HIPCHK(hipMemcpy(feat, df, m * CL_NFEAT * sizeof(float), hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(targ, dt, m * 4 * sizeof(float), hipMemcpyDeviceToHost));
-      if (base) HIPCHK(hipMemcpy(base, db, m * 3 * sizeof(float), hipMemcpyDeviceToHost));
-    }
-    *n = (int64_t)cnt;
+    pair_log_staged(
+        c, band_h, cap, df, dt, db, dn,
+        [&] { HIPCHK(hipMemsetAsync(dn, 0, sizeof(unsigned), c->stream)); }, feat, targ, base, n);
     c->dfree(df);
     c->dfree(dt);
     c->dfree(db);
     c->dfree(dn);
+  });
+}
+
+int pdem_pair_log_noalloc(pdem_ctx* c, double band_h, int64_t cap, float* feat, float* targ, float* base,
+                          int64_t* n) {
+  return guarded(c, [&] {
+    if (!c->cl_ready) throw HipError{"call pdem_closure_setup before pdem_pair_log_noalloc"};
+    if (cap < 0) throw HipError{"negative capacity"};
+    if (cap > NOALLOC_ROWS)
+      throw HipError{"pdem_pair_log_noalloc: cap " + std::to_string(cap) + " exceeds the staging maximum of " +
+                     std::to_string(NOALLOC_ROWS) + " rows"};
+    if (!c->scratch_dev) throw HipError{"staging array not set up"};
+    unsigned char* sp = (unsigned char*)c->scratch_dev;
+    unsigned* dn = (unsigned*)sp;
+    float* df = (float*)(sp + 256);
+    float* dt = df + NOALLOC_ROWS * CL_NFEAT;
+    float* db = dt + NOALLOC_ROWS * 4;
+    // the counter keeps the previous call's value: cleared by a fill kernel
+    pair_log_staged(
+        c, band_h, cap, df, dt, db, dn,
+        [&] { hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, dn, 1); }, feat, targ, base, n);
   });
 }
 
