@@ -343,6 +343,12 @@ int pdem_set_contact_light_max(pdem_ctx *ctx, int32_t light_max);
  * finalises the nodes, which saves launches where the step is launch-bound.
  * Results are the same bit for bit either way. No counterpart in the reference. */
 int pdem_set_step_fusion(pdem_ctx *ctx, int32_t on);
+/* Fused state-based passes (mixed precision, on by default): a particle
+ * whose nodes all fit one LDS tile runs its dilation and force passes in one
+ * workgroup (window staged once, dilation terms kept in LDS); with 0 every
+ * tile runs the two-pass kernels.  Results are the same bit for bit either
+ * way. No counterpart in the reference (kernel selection). */
+int pdem_set_state_fusion(pdem_ctx *ctx, int32_t on);
 
 /* ---------------------------------------------------------------------
  * Learned pair-contact closure network.  No counterpart in the reference:

@@ -423,13 +423,13 @@ def engine_library():
         lib.pdem_pair_log.restype = C.c_int
         lib.pdem_pair_log.argtypes = [C.c_void_p, C.c_double, C.c_int64, C.POINTER(C.c_float),
                                       C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int64)]
-        lib.pdem_pair_log_noalloc.restype = C.c_int
-        lib.pdem_pair_log_noalloc.argtypes = lib.pdem_pair_log.argtypes
         lib.pdem_material_constants.restype = C.c_int
         lib.pdem_material_constants.argtypes = [C.POINTER(MaterialDeckC), C.POINTER(C.c_double),
                                                 C.POINTER(C.c_double)]
         lib.pdem_material_last_error.restype = C.c_char_p
         lib.pdem_material_last_error.argtypes = []
+        lib.pdem_pair_log_noalloc.restype = C.c_int
+        lib.pdem_pair_log_noalloc.argtypes = lib.pdem_pair_log.argtypes
         lib.pdem_checkpoint_size.restype = C.c_int
         lib.pdem_checkpoint_size.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         lib.pdem_checkpoint_save.restype = C.c_int
@@ -448,6 +448,8 @@ def engine_library():
         lib.pdem_get_tile_counts.argtypes = [C.c_void_p] + [C.POINTER(C.c_int32)] * 2
         lib.pdem_set_step_fusion.restype = C.c_int
         lib.pdem_set_step_fusion.argtypes = [C.c_void_p, C.c_int32]
+        lib.pdem_set_state_fusion.restype = C.c_int
+        lib.pdem_set_state_fusion.argtypes = [C.c_void_p, C.c_int32]
         _engine_lib = lib
     return _engine_lib
 
@@ -551,6 +553,11 @@ class GpuEngine(Engine):
     def set_step_fusion(self, on=True):
         """Fused step kernels for small multi-particle models (default on)."""
         self._check(self.lib.pdem_set_step_fusion(self.ctx, int(on)), "set_step_fusion")
+
+    def set_state_fusion(self, on=True):
+        """Fused dilation and force passes for particles that fit one LDS tile
+        (mixed precision; results unchanged)."""
+        self._check(self.lib.pdem_set_state_fusion(self.ctx, int(on)), "set_state_fusion")
 
     def closure_setup(self, codes=None):
         """Closure data; codes: one float per particle (shape/material class)."""

@@ -80,6 +80,10 @@ struct pdem_ctx {
   // beside the LDS tiles (forked from and joined back into `stream`)
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // second side stream: the cell list of a rebuild step, built beside the
+  // bond passes (multi-particle models)
+  hipStream_t side2 = nullptr;
+  hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
   DS ds{};
   std::vector<void*> allocs;
   std::vector<pdem_particle> hp;
@@ -115,6 +119,7 @@ struct pdem_ctx {
   RnpTileArgs rt{};
   bool rnp_tile = false;  // use it (set up by default, autotuned)
   bool step_fusion = true;  // fused step kernels for small multi-particle models
+  bool state_fusion = true;  // closed state-based tiles run both passes in one kernel
   bool light_used = false;  // the lane-per-node contact list may hold nodes
   double* pp_out = nullptr;  // two-particle test results
   // strain / stress post-processing (pdem_fe_setup)
@@ -163,6 +168,7 @@ struct pdem_ctx {
     if (cur != device) (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
     if (side) (void)hipStreamSynchronize(side);
+    if (side2) (void)hipStreamSynchronize(side2);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     for (auto& e : ev_used) {
@@ -175,10 +181,13 @@ struct pdem_ctx {
     }
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
+    if (ev_fork2) (void)hipEventDestroy(ev_fork2);
+    if (ev_join2) (void)hipEventDestroy(ev_join2);
     if (cl_net) cl_net->users.fetch_sub(1);
     cl_net = nullptr;
     for (void* p : allocs) (void)hipFree(p);
     if (side) (void)hipStreamDestroy(side);
+    if (side2) (void)hipStreamDestroy(side2);
     if (stream) (void)hipStreamDestroy(stream);
     if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
   }
@@ -213,9 +222,14 @@ struct Launch {
     HIPCHK(hipEventRecord(c->ev_fork, c->stream));
     HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
   }
+  static void mixed_umax(pdem_ctx* c, hipStream_t st) {
+    if (c->mixed.n_open_pids > 0)
+      hipLaunchKernelGGL(k_mixed_umax<DIM>, dim3(c->mixed.n_open_pids), dim3(256), 0, st, c->ds, c->mixed);
+  }
   template <int INF, int T>
   static void mixed_dilation(pdem_ctx* c) {
     const MixedArgs& ma = c->mixed;
+    mixed_umax(c, c->stream);
     if (ma.ntiles_hbm) {
       fork(c);
       hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, false, T>), dim3(ma.ntiles_hbm), dim3(T), 0, c->side,
@@ -247,66 +261,118 @@ struct Launch {
                          c->ds, ma);
     if (ma.ntiles_hbm) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   }
+  // The state-based passes of the mixed kernels: closed tiles (whole
+  // particles) in the fused kernel on the engine stream; open LDS tiles and
+  // HBM tiles in the two-pass kernels (dilation of all, then force of all,
+  // since their force pass reads other tiles' dilation terms) on the side
+  // stream beside them.  Without closed tiles the open LDS tiles keep the
+  // engine stream and the HBM tiles the side stream.
+  template <int INF, int T>
+  static void mixed_state(pdem_ctx* c) {
+    const MixedArgs& ma = c->mixed;
+    if (ma.ntiles_closed == 0) {
+      mixed_dilation<INF, T>(c);
+      mixed_force<INF, T>(c);
+      return;
+    }
+    const bool two_pass = ma.ntiles_lds || ma.ntiles_hbm;
+    if (two_pass) {
+      fork(c);
+      mixed_umax(c, c->side);
+      if (ma.ntiles_lds && ma.small)
+        hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, true, T, LDS_SMALL>), dim3(ma.ntiles_lds), dim3(T), 0,
+                           c->side, c->ds, ma);
+      else if (ma.ntiles_lds)
+        hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, true, T>), dim3(ma.ntiles_lds), dim3(T), 0, c->side, c->ds,
+                           ma);
+      if (ma.ntiles_hbm)
+        hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, false, T>), dim3(ma.ntiles_hbm), dim3(T), 0, c->side,
+                           c->ds, ma);
+      if (ma.ntiles_lds && ma.small)
+        hipLaunchKernelGGL((k_mixed_force<DIM, INF, true, T, LDS_SMALL>), dim3(ma.ntiles_lds), dim3(T), 0, c->side,
+                           c->ds, ma);
+      else if (ma.ntiles_lds)
+        hipLaunchKernelGGL((k_mixed_force<DIM, INF, true, T>), dim3(ma.ntiles_lds), dim3(T), 0, c->side, c->ds, ma);
+      if (ma.ntiles_hbm)
+        hipLaunchKernelGGL((k_mixed_force<DIM, INF, false, T>), dim3(ma.ntiles_hbm), dim3(T), 0, c->side, c->ds,
+                           ma);
+      HIPCHK(hipEventRecord(c->ev_join, c->side));
+    }
+    if (ma.small)
+      hipLaunchKernelGGL((k_mixed_state<DIM, INF, T, LDS_SMALL>), dim3(ma.ntiles_closed), dim3(T), 0, c->stream,
+                         c->ds, ma);
+    else
+      hipLaunchKernelGGL((k_mixed_state<DIM, INF, T>), dim3(ma.ntiles_closed), dim3(T), 0, c->stream, c->ds, ma);
+    if (two_pass) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  }
   template <int INF>
-  static void pd_inf(pdem_ctx* c, int bin) {
+  static void pd_inf(pdem_ctx* c) {
     hipStream_t st = c->stream;
     const unsigned nb = blocks(c->N);
     if (c->precision == PDEM_PREC_MIXED && c->ntiles > 0) {
-      hipLaunchKernelGGL(k_bin_zero<DIM>, dim3(std::min<unsigned>(nb, BZ_BLOCKS)), dim3(TPB), 0, st, c->ds, bin);
-      if (c->rnp_any) hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(nb), dim3(TPB), 0, st, c->ds, 0);
-      if (c->pmb_any) hipLaunchKernelGGL((k_pd_bond<DIM, INF, 2>), dim3(nb), dim3(TPB), 0, st, c->ds, 0);
-      if (c->elastic_any) hipLaunchKernelGGL((k_pd_bond<DIM, INF, 3>), dim3(nb), dim3(TPB), 0, st, c->ds, 0);
-      if (c->mixed_tpb == 256)
-        mixed_dilation<INF, 256>(c);
-      else
-        mixed_dilation<INF, 1024>(c);
+      if (c->ds.n_noforce > 0)
+        hipLaunchKernelGGL(k_zero_noforce<DIM>, dim3(std::min<unsigned>(nb, BZ_BLOCKS)), dim3(TPB), 0, st, c->ds);
+      if (c->rnp_any) hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(nb), dim3(TPB), 0, st, c->ds);
+      if (c->pmb_any) hipLaunchKernelGGL((k_pd_bond<DIM, INF, 2>), dim3(nb), dim3(TPB), 0, st, c->ds);
+      if (c->elastic_any) hipLaunchKernelGGL((k_pd_bond<DIM, INF, 3>), dim3(nb), dim3(TPB), 0, st, c->ds);
+      // timed: both state-based passes (dilation and force), all tiles
       auto ev = ev_begin(c);
       if (c->mixed_tpb == 256)
-        mixed_force<INF, 256>(c);
+        mixed_state<INF, 256>(c);
       else
-        mixed_force<INF, 1024>(c);
+        mixed_state<INF, 1024>(c);
       ev_end(c, ev);
       return;
     }
     const unsigned pb = (unsigned)c->pd_tpb, pnb = blocks(c->N, c->pd_tpb);
-    if (c->ds.state_global) {
-      hipLaunchKernelGGL((k_pd_dilation<DIM, INF>), dim3(pnb), dim3(pb), 0, st, c->ds, bin);
-      bin = 0;
-    }
+    if (c->ds.state_global) hipLaunchKernelGGL((k_pd_dilation<DIM, INF>), dim3(pnb), dim3(pb), 0, st, c->ds);
     auto ev = ev_begin(c);
-    if (c->rnp_tile) {
+    if (c->rnp_tile)
       hipLaunchKernelGGL((k_pd_rnp_tile<DIM, INF>), dim3(c->rt.ntiles), dim3(RT_THREADS), c->rt.lds_bytes, st,
-                         c->ds, c->rt, bin);
-      bin = 0;
-    } else if (c->rnp_any || !(c->state_any || c->pmb_any || c->elastic_any)) {
-      hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(pnb), dim3(pb), 0, st, c->ds, bin);
-      bin = 0;
-    }
-    if (c->pmb_any) {
-      hipLaunchKernelGGL((k_pd_bond<DIM, INF, 2>), dim3(pnb), dim3(pb), 0, st, c->ds, bin);
-      bin = 0;
-    }
-    if (c->elastic_any) {
-      hipLaunchKernelGGL((k_pd_bond<DIM, INF, 3>), dim3(pnb), dim3(pb), 0, st, c->ds, bin);
-      bin = 0;
-    }
-    if (c->state_any) hipLaunchKernelGGL((k_pd_state<DIM, INF>), dim3(pnb), dim3(pb), 0, st, c->ds, bin);
+                         c->ds, c->rt);
+    else if (c->rnp_any || !(c->state_any || c->pmb_any || c->elastic_any))
+      hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(pnb), dim3(pb), 0, st, c->ds);
+    if (c->pmb_any) hipLaunchKernelGGL((k_pd_bond<DIM, INF, 2>), dim3(pnb), dim3(pb), 0, st, c->ds);
+    if (c->elastic_any) hipLaunchKernelGGL((k_pd_bond<DIM, INF, 3>), dim3(pnb), dim3(pb), 0, st, c->ds);
+    if (c->state_any) hipLaunchKernelGGL((k_pd_state<DIM, INF>), dim3(pnb), dim3(pb), 0, st, c->ds);
     ev_end(c, ev);
   }
-  static void pd_forces(pdem_ctx* c, int bin) {
+  static void pd_forces(pdem_ctx* c) {
     if (c->ds.inf_type == 0)
-      pd_inf<0>(c, bin);
+      pd_inf<0>(c);
     else if (c->ds.inf_type == 1)
-      pd_inf<1>(c, bin);
+      pd_inf<1>(c);
     else
-      pd_inf<2>(c, bin);
+      pd_inf<2>(c);
   }
-  // standalone search: controller, binning, candidate query
+  // the cell list of a rebuild step (counting sort by cell key); every
+  // kernel returns at once on other steps
+  static void cells_build(pdem_ctx* c, hipStream_t st) {
+    const unsigned nb = (unsigned)std::min<int64_t>(blocks(c->N), 2048);
+    hipLaunchKernelGGL(k_cells_count<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+    hipLaunchKernelGGL(k_cells_scan, dim3((unsigned)c->ds.ncell_blocks), dim3(CELL_TPB), 0, st, c->ds);
+    hipLaunchKernelGGL(k_cells_scatter<DIM>, dim3((unsigned)std::min<int64_t>(blocks(c->N, CELL_TPB), 1024)),
+                       dim3(CELL_TPB), 0, st, c->ds);
+  }
+  // PD forces and, on a rebuild step, the cell list on the second side stream
+  // beside them (joined before the candidate query)
+  static void pd_forces_and_cells(pdem_ctx* c) {
+    if (!c->multi) {
+      pd_forces(c);
+      return;
+    }
+    HIPCHK(hipEventRecord(c->ev_fork2, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->side2, c->ev_fork2, 0));
+    cells_build(c, c->side2);
+    HIPCHK(hipEventRecord(c->ev_join2, c->side2));
+    pd_forces(c);
+    HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join2, 0));
+  }
+  // standalone search: controller, cell list, candidate query
   static void contact_search(pdem_ctx* c) {
     hipStream_t st = c->stream;
-    const unsigned nb = blocks(c->N);
     hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, st, c->ds, 0);
-    hipLaunchKernelGGL(k_bin<DIM>, dim3(nb), dim3(TPB), 0, st, c->ds);
+    cells_build(c, st);
     rebuild_lists(c);
     hipLaunchKernelGGL(k_cells_init, dim3((unsigned)std::min<int64_t>(blocks(c->ds.ncells), 4096)), dim3(TPB), 0,
                        st, c->ds);
@@ -385,7 +451,7 @@ struct Launch {
   // forces after the controller has run: PD (binning on a rebuild), contact
   // (query on a rebuild, then forces), damping, external forces
   static void forces_after_ctrl(pdem_ctx* c) {
-    pd_forces(c, c->multi ? 1 : 0);
+    pd_forces_and_cells(c);
     const bool fin = fuse_finalize(c);
     if (c->multi) {
       rebuild_lists(c);
@@ -452,9 +518,6 @@ void check_overflow(pdem_ctx* c) {
     throw HipError{"more than 256 damping partners within reach of one particle"};
   if (h.overflow == 3)
     throw HipError{"more than 64 active contact pairs at one node"};
-  if (h.overflow == 2)
-    throw HipError{"cell capacity exceeded (" + std::to_string(c->ds.cell_cap) +
-                   " nodes per search cell): nodes crowded far beyond their initial packing"};
   if (h.overflow == 5)
     throw HipError{"closure pair list capacity exceeded (" + std::to_string(c->ds.cpair_cap) + " pairs)"};
   if (h.overflow == 6) throw HipError{"closure: damping partner lists are not symmetric"};
@@ -626,7 +689,9 @@ void build_tiles(pdem_ctx* c, int64_t tile_max) {
       const int64_t te = std::min<int64_t>(q.glob_end, ts + tile);
       int ws, we;
       window(ts, te, ws, we);
-      tiles.push_back(DevTile{(int)ts, (int)te, ws, we, p, (we - ws) <= LDS_NODES ? 1 : 0});
+      const bool lds = (we - ws) <= LDS_NODES;
+      const bool closed = lds && ts == q.glob_start && te == q.glob_end;
+      tiles.push_back(DevTile{(int)ts, (int)te, ws, we, p, closed ? 2 : lds ? 1 : 0});
     }
   }
   // thread slots: inside every tile, nodes by 8-bond block count (most
@@ -659,12 +724,36 @@ void build_tiles(pdem_ctx* c, int64_t tile_max) {
   c->mixed.slot = c->d_slot;
   c->mixed.col16 = c->d_col16;
   c->mixed.sptr16 = c->d_sp16;
+  // closed tiles (fused kernel), then open LDS tiles, then HBM tiles
   std::stable_sort(tiles.begin(), tiles.end(), [](const DevTile& a, const DevTile& b) { return a.lds > b.lds; });
   if (tiles.size() > c->d_tiles_cap) throw HipError{"tile buffer too small"};
   c->ntiles = (int)tiles.size();
   c->ntiles_global = 0;
-  for (auto& t : tiles) c->ntiles_global += t.lds ? 0 : 1;
-  c->mixed.ntiles_lds = c->ntiles - c->ntiles_global;
+  int nclosed = 0;
+  for (auto& t : tiles) {
+    c->ntiles_global += t.lds ? 0 : 1;
+    nclosed += t.lds == 2 ? 1 : 0;
+  }
+  if (!c->state_fusion) {  // every LDS tile through the two-pass kernels
+    nclosed = 0;
+    for (auto& t : tiles) t.lds = t.lds ? 1 : 0;
+  }
+  // particles with open LDS tiles: their windows' fixed-point step comes from
+  // a per-particle maximum computed before the passes
+  {
+    std::vector<int> open;
+    for (auto& t : tiles)
+      if (t.lds == 1 && (open.empty() || open.back() != t.pid)) open.push_back(t.pid);
+    std::sort(open.begin(), open.end());
+    open.erase(std::unique(open.begin(), open.end()), open.end());
+    if (!c->mixed.open_pids) c->mixed.open_pids = c->dalloc<int>(std::max(1, c->np));
+    if (!c->mixed.pumax) c->mixed.pumax = c->dalloc<float>(std::max(1, c->np));
+    if (!open.empty())
+      HIPCHK(hipMemcpy((void*)c->mixed.open_pids, open.data(), open.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->mixed.n_open_pids = (int)open.size();
+  }
+  c->mixed.ntiles_closed = nclosed;
+  c->mixed.ntiles_lds = c->ntiles - c->ntiles_global - nclosed;
   int maxw = 0;
   for (auto& t : tiles)
     if (t.lds) maxw = std::max(maxw, t.we - t.ws);
@@ -930,31 +1019,21 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     }
     s.cell = cell;
     s.ncells = ncells;
-    // cell capacity from the densest cell of the reference configuration, with
-    // room for compaction
-    {
-      std::vector<int> cnt(ncells, 0);
-      for (int64_t i = 0; i < N; i++) {
-        int64_t cc = 0, mul = 1;
-        for (int k = 0; k < dim; k++) {
-          double q = std::floor((d->x_ref[3 * i + k] + d->u[3 * i + k] * 0 - s.glo[k]) / cell);
-          int64_t qi = q < 0 ? 0 : (q > s.gn[k] - 1 ? s.gn[k] - 1 : (int64_t)q);
-          cc += qi * mul;
-          mul *= s.gn[k];
-        }
-        cnt[cc]++;
-      }
-      int mx = 0;
-      for (int v : cnt) mx = std::max(mx, v);
-      int capc = std::max(64, ((2 * mx + 32) + 31) / 32 * 32);
-      s.cell_cap = capc;
-    }
+    s.ncell_blocks = (ncells + CELL_BLOCK - 1) / CELL_BLOCK;
     s.cell_count = c->dalloc<int>(ncells + 1);
     s.cell_owner = c->dalloc<int>(ncells + 1);
     hipLaunchKernelGGL(k_cells_init, dim3((unsigned)std::min<int64_t>(blocks(ncells), 4096)), dim3(TPB), 0,
                        c->stream, s);
-    s.cell_nodes = c->dalloc<int>((size_t)ncells * s.cell_cap, false);
-    s.cell_xf = c->dalloc<float4>((size_t)ncells * s.cell_cap, false);
+    s.cell_start = c->dalloc<int>(ncells + 1, false);
+    s.cell_btot = c->dalloc<int>(s.ncell_blocks, false);
+    s.cell_boff = c->dalloc<int>(s.ncell_blocks, false);
+    s.node_cell = c->dalloc<int>(N, false);
+    s.node_slot = c->dalloc<int>(N, false);
+    s.cell_nodes = c->dalloc<int>(N, false);
+    s.cell_xf = c->dalloc<float4>(N, false);
+    HIPCHK(hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming));
     // candidate capacity: nodes of a ball of the largest search radius at the
     // finest mesh spacing
     double hmin = 1e300;
@@ -1256,7 +1335,7 @@ int pdem_compute_forces(pdem_ctx* c) {
 
 int pdem_compute_peridynamic_forces(pdem_ctx* c) {
   return guarded(c, [&] {
-    dispatch(c, [&] { Launch<2>::pd_forces(c, 0); }, [&] { Launch<3>::pd_forces(c, 0); });
+    dispatch(c, [&] { Launch<2>::pd_forces(c); }, [&] { Launch<3>::pd_forces(c); });
     check_launch(c);
   });
 }
@@ -2003,7 +2082,7 @@ int pdem_autotune(pdem_ctx* c, int32_t reps, double* best_ms) {
       hipEvent_t a, b;
       HIPCHK(hipEventCreate(&a));
       HIPCHK(hipEventCreate(&b));
-      auto run = [&] { dispatch(c, [&] { Launch<2>::pd_forces(c, 0); }, [&] { Launch<3>::pd_forces(c, 0); }); };
+      auto run = [&] { dispatch(c, [&] { Launch<2>::pd_forces(c); }, [&] { Launch<3>::pd_forces(c); }); };
       run();  // warm-up
       HIPCHK(hipEventRecord(a, c->stream));
       for (int r = 0; r < reps; r++) run();
@@ -2114,7 +2193,7 @@ int pdem_set_contact_light_max(pdem_ctx* c, int32_t light_max) {
 int pdem_get_tile_counts(pdem_ctx* c, int32_t* lds_tiles, int32_t* hbm_tiles) {
   return guarded(c, [&] {
     if (!lds_tiles || !hbm_tiles) throw HipError{"null output pointer"};
-    *lds_tiles = c->mixed.ntiles_lds;
+    *lds_tiles = c->mixed.ntiles_closed + c->mixed.ntiles_lds;
     *hbm_tiles = c->mixed.ntiles_hbm;
   });
 }
@@ -2123,6 +2202,16 @@ int pdem_set_step_fusion(pdem_ctx* c, int32_t on) {
   return guarded(c, [&] {
     HIPCHK(hipStreamSynchronize(c->stream));
     c->step_fusion = on != 0;
+    drop_graph(c);
+  });
+}
+
+int pdem_set_state_fusion(pdem_ctx* c, int32_t on) {
+  return guarded(c, [&] {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->side) HIPCHK(hipStreamSynchronize(c->side));
+    c->state_fusion = on != 0;
+    if (c->precision == PDEM_PREC_MIXED && c->ntiles > 0) build_tiles(c, c->tile_nodes);
     drop_graph(c);
   });
 }

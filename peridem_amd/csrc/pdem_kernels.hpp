@@ -118,10 +118,19 @@ struct DS {
   int gn[3];
   int csub;  // cells per largest search radius: the query scans +-csub cells
   int64_t ncells;
-  int cell_cap;     // fixed capacity of a cell
+  // Cell list of the last rebuild: the nodes sorted by cell key (a counting
+  // sort: per-cell counts, an exclusive scan over the cells in blocks of
+  // CELL_BLOCK, a scatter).  Cell c holds the cell_count[c] entries from
+  // cell_boff[c / CELL_BLOCK] + cell_start[c] on; no cell has a capacity.
   int* cell_count;  // [ncells]
-  int* cell_nodes;  // [ncells][cell_cap], unordered inside a cell
-  float4* cell_xf;  // [ncells][cell_cap] beside cell_nodes: position - glo in FP32, particle id (bits)
+  int* cell_start;  // [ncells] exclusive prefix of the counts inside the cell's block
+  int* cell_btot;   // [nblocks] count total of each block of cells
+  int* cell_boff;   // [nblocks] exclusive prefix of the block totals
+  int64_t ncell_blocks;
+  int* node_cell;   // [N] cell of each node (this rebuild)
+  int* node_slot;   // [N] rank of the node inside its cell
+  int* cell_nodes;  // [N] node ids by cell
+  float4* cell_xf;  // [N] beside cell_nodes: position - glo in FP32, particle id (bits)
   int* cell_owner;  // [ncells] particle id when one particle fills the cell, -1 mixed, -2 empty
   // damping
   int n_nonwall;
@@ -204,9 +213,6 @@ __device__ __forceinline__ double dot3(const double* a, const double* b) {
 }
 
 __device__ __forceinline__ int64_t gid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
-
-template <int DIM>
-__device__ void bin_node(const DS& s, int64_t i);
 
 // ------------------------------------------------- PD neighbor list build
 // j in the same particle, j != i, 0 + dx^2 + dy^2 + dz^2 < horizon^2
@@ -669,10 +675,9 @@ __device__ __forceinline__ double infJ(const DS& s, double r_over_delta, double 
 // force sum runs over the neighbor list in order, as the reference's does.
 // The next neighbor's data is loaded while the current bond is evaluated.
 template <int DIM, int INF>
-__global__ void __launch_bounds__(256) k_pd_rnp(DS s, int bin) {
+__global__ void __launch_bounds__(256) k_pd_rnp(DS s) {
   int64_t i = gid();
   if (i >= s.N) return;
-  if (bin && s.ctrl->rebuild) bin_node<DIM>(s, i);
   const DevParticle& p = s.P[s.ptid[i]];
   if (!p.compute_force) {
     for (int d = 0; d < DIM; d++) s.f[d][i] = 0.;
@@ -766,10 +771,9 @@ __global__ void __launch_bounds__(256) k_pd_rnp(DS s, int bin) {
 // bonds give the short-range repulsion; damage max |S / Sc| with Sc = s0
 // (PMB) or the largest double (PdElastic).
 template <int DIM, int INF, int LAW>
-__global__ void __launch_bounds__(256) k_pd_bond(DS s, int bin) {
+__global__ void __launch_bounds__(256) k_pd_bond(DS s) {
   int64_t i = gid();
   if (i >= s.N) return;
-  if (bin && s.ctrl->rebuild) bin_node<DIM>(s, i);
   const DevParticle& p = s.P[s.ptid[i]];
   if (!p.compute_force) {
     for (int d = 0; d < DIM; d++) s.f[d][i] = 0.;
@@ -837,10 +841,9 @@ __global__ void __launch_bounds__(256) k_pd_bond(DS s, int bin) {
 // State-based dilation pass (demModel.cpp:610-679): bond breaking on the
 // actual stretch, theta_i = 3 sum r (|y|-r) J V^c / m_i.
 template <int DIM, int INF>
-__global__ void __launch_bounds__(256) k_pd_dilation(DS s, int bin) {
+__global__ void __launch_bounds__(256) k_pd_dilation(DS s) {
   int64_t i = gid();
   if (i >= s.N) return;
-  if (bin && s.ctrl->rebuild) bin_node<DIM>(s, i);
   const DevParticle& p = s.P[s.ptid[i]];
   if (!p.compute_force || p.material_type != 1) return;
   const double horizon = p.horizon, h = p.h;
@@ -890,10 +893,9 @@ __global__ void __launch_bounds__(256) k_pd_dilation(DS s, int bin) {
 
 // State-based force pass with PdState::getBondEF for both ends (material.h:1415-1429).
 template <int DIM, int INF>
-__global__ void __launch_bounds__(256) k_pd_state(DS s, int bin) {
+__global__ void __launch_bounds__(256) k_pd_state(DS s) {
   int64_t i = gid();
   if (i >= s.N) return;
-  if (bin && s.ctrl->rebuild) bin_node<DIM>(s, i);
   const DevParticle& p = s.P[s.ptid[i]];
   if (!p.compute_force) {
     for (int d = 0; d < DIM; d++) s.f[d][i] = 0.;
@@ -959,8 +961,18 @@ __global__ void __launch_bounds__(256) k_pd_state(DS s, int bin) {
 }
 
 // ---------------------------------------------------------- cell list
-// Uniform grid with cell >= the largest search radius; each cell holds up to
-// cell_cap node ids in arbitrary order (candidate sets do not depend on it).
+// Uniform grid with cell >= the largest search radius / csub (at most 2^25
+// cells: CELL_MAX_BLOCKS blocks of CELL_BLOCK).  At a rebuild
+// the nodes are sorted by cell key with a counting sort on the grid:
+// k_cells_count (per-cell counts and each node's rank in its cell),
+// k_cells_scan (exclusive scan of the counts inside blocks of CELL_BLOCK
+// cells, block totals), k_cells_scatter (block offsets, then every node to
+// its place).  The order inside a cell does not matter: candidate sets are
+// sets and contact sums run in ascending node id.
+constexpr int CELL_BLOCK = 4096;  // cells per scan block (1024 threads x 4)
+constexpr int CELL_TPB = 1024;
+constexpr int CELL_MAX_BLOCKS = (1 << 25) / CELL_BLOCK;
+
 __device__ __forceinline__ int cellc(double a, double lo, double cell, int n) {
   double k = floor((a - lo) / cell);
   int ki = k < 0. ? 0 : (k > (double)(n - 1) ? n - 1 : (int)k);
@@ -968,7 +980,7 @@ __device__ __forceinline__ int cellc(double a, double lo, double cell, int n) {
 }
 
 template <int DIM>
-__device__ void bin_node(const DS& s, int64_t i) {
+__device__ __forceinline__ void count_node(const DS& s, int64_t i) {
   int cx = cellc(s.x[0][i], s.glo[0], s.cell, s.gn[0]);
   int cy = cellc(s.x[1][i], s.glo[1], s.cell, s.gn[1]);
   int cz = DIM == 3 ? cellc(s.x[2][i], s.glo[2], s.cell, s.gn[2]) : 0;
@@ -981,7 +993,7 @@ __device__ void bin_node(const DS& s, int64_t i) {
     const int old = atomicCAS(&s.cell_owner[c], -2, pid);
     if (old != -2 && old != pid) s.cell_owner[c] = -1;
   }
-  // slots: one atomic per distinct cell among the wavefront's active lanes
+  // ranks: one atomic per distinct cell among the wavefront's active lanes
   // (the lanes of a wavefront bin neighboring nodes, mostly into a few cells)
   const int lane = (int)(threadIdx.x & (WAVE - 1));
   unsigned long long todo = __ballot(1);
@@ -996,21 +1008,90 @@ __device__ void bin_node(const DS& s, int64_t i) {
     if (c == cl) slot = base + (int)__popcll(same & ((1ull << lane) - 1ull));
     todo &= ~same;
   }
-  if (slot < s.cell_cap) {
-    const int64_t e = (int64_t)c * s.cell_cap + slot;
-    s.cell_nodes[e] = (int)i;
-    s.cell_xf[e] = make_float4((float)(s.x[0][i] - s.glo[0]), (float)(s.x[1][i] - s.glo[1]),
-                               DIM == 3 ? (float)(s.x[2][i] - s.glo[2]) : 0.f, __int_as_float(pid));
-  } else {
-    s.ctrl->overflow = 2;
-  }
+  s.node_cell[i] = c;
+  s.node_slot[i] = slot;
 }
 
+// grid-stride over the nodes (the launch stays small on steps without a rebuild)
 template <int DIM>
-__global__ void k_bin(DS s) {
+__global__ void k_cells_count(DS s) {
   if (!s.ctrl->rebuild) return;
-  int64_t i = gid();
-  if (i < s.N) bin_node<DIM>(s, i);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = gid(); i < s.N; i += stride) count_node<DIM>(s, i);
+}
+
+// inclusive scan of v over the workgroup (CELL_TPB threads); tot gets the sum
+__device__ __forceinline__ int block_scan_incl(int v, int* lds, int& tot) {
+  const int lane = (int)(threadIdx.x & (WAVE - 1)), w = (int)(threadIdx.x / WAVE);
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const int t = __shfl_up(v, o);
+    if (lane >= o) v += t;
+  }
+  if (lane == WAVE - 1) lds[w] = v;
+  __syncthreads();
+  if (w == 0) {
+    int x = lane < (int)(blockDim.x / WAVE) ? lds[lane] : 0;
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const int t = __shfl_up(x, o);
+      if (lane >= o) x += t;
+    }
+    if (lane < (int)(blockDim.x / WAVE)) lds[lane] = x;
+  }
+  __syncthreads();
+  tot = lds[blockDim.x / WAVE - 1];
+  const int r = v + (w > 0 ? lds[w - 1] : 0);
+  __syncthreads();
+  return r;
+}
+
+// one workgroup per block of CELL_BLOCK cells: exclusive prefix of the counts
+// inside the block and the block's total
+__global__ void __launch_bounds__(CELL_TPB) k_cells_scan(DS s) {
+  __shared__ int lds[CELL_TPB / WAVE];
+  if (!s.ctrl->rebuild) return;
+  const int64_t c0 = (int64_t)blockIdx.x * CELL_BLOCK + 4 * threadIdx.x;
+  int v[4];
+  for (int k = 0; k < 4; k++) v[k] = c0 + k < s.ncells ? s.cell_count[c0 + k] : 0;
+  const int mine = v[0] + v[1] + v[2] + v[3];
+  int tot;
+  const int incl = block_scan_incl(mine, lds, tot);
+  int run = incl - mine;
+  for (int k = 0; k < 4; k++) {
+    if (c0 + k < s.ncells) s.cell_start[c0 + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) s.cell_btot[blockIdx.x] = tot;
+}
+
+// block offsets (every workgroup scans the block totals; workgroup 0 stores
+// them for the query), then each node to its place in the sorted list
+template <int DIM>
+__global__ void __launch_bounds__(CELL_TPB) k_cells_scatter(DS s) {
+  __shared__ int lds[CELL_TPB / WAVE];
+  __shared__ int boff[CELL_MAX_BLOCKS];
+  if (!s.ctrl->rebuild) return;
+  const int nb = (int)s.ncell_blocks;
+  int carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += CELL_TPB) {
+    const int b = b0 + (int)threadIdx.x;
+    const int v = b < nb ? s.cell_btot[b] : 0;
+    int tot;
+    const int incl = block_scan_incl(v, lds, tot);
+    if (b < nb) {
+      boff[b] = carry + incl - v;
+      if (blockIdx.x == 0) s.cell_boff[b] = carry + incl - v;
+    }
+    carry += tot;
+  }
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = gid(); i < s.N; i += stride) {
+    const int c = s.node_cell[i];
+    const int64_t e = (int64_t)boff[c / CELL_BLOCK] + s.cell_start[c] + s.node_slot[i];
+    s.cell_nodes[e] = (int)i;
+    s.cell_xf[e] = make_float4((float)(s.x[0][i] - s.glo[0]), (float)(s.x[1][i] - s.glo[1]),
+                               DIM == 3 ? (float)(s.x[2][i] - s.glo[2]) : 0.f, __int_as_float(s.ptid[i]));
+  }
 }
 
 // radiusSearchExcludeTag for one searching node (demModel.cpp:1717-1756):
@@ -1062,10 +1143,10 @@ __device__ void query_node(const DS& s, int64_t i, int pi) {
         if (!edge && dmin2 > r2 * 1.0000001) continue;
         const int cc = (z * s.gn[1] + y) * s.gn[0] + x;
         if (s.cell_owner[cc] == pi) continue;  // only the node's own particle
-        int m = s.cell_count[cc];
-        m = m < s.cell_cap ? m : s.cell_cap;
-        const int* cn = s.cell_nodes + (int64_t)cc * s.cell_cap;
-        const float4* cx = s.cell_xf + (int64_t)cc * s.cell_cap;
+        const int m = s.cell_count[cc];
+        const int64_t e0 = (int64_t)s.cell_boff[cc / CELL_BLOCK] + s.cell_start[cc];
+        const int* cn = s.cell_nodes + e0;
+        const float4* cx = s.cell_xf + e0;
         for (int q = 0; q < m; q++) {
           const float4 e = cx[q];
           if (__float_as_int(e.w) == pi) continue;  // the node's own particle (and i itself)

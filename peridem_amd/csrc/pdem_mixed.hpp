@@ -53,12 +53,12 @@ struct DevTile {
   int ts, te;     // nodes [ts, te)
   int ws, we;     // neighbor window [ws, we)
   int pid;
-  int lds;        // window staged in LDS
+  int lds;        // window staged in LDS; 2: also closed (the tile is its whole particle)
 };
 
 struct MixedArgs {
-  const DevTile* tiles;   // LDS tiles first, then HBM tiles
-  int ntiles_lds, ntiles_hbm;
+  const DevTile* tiles;   // closed LDS tiles, then open LDS tiles, then HBM tiles
+  int ntiles_closed, ntiles_lds, ntiles_hbm;
   int small;              // every LDS window holds at most LDS_SMALL nodes
   // thread slot q runs node slot[q]; its k-th neighbor id is
   // col16[sptr16[q / 64] + (k / KB) * 64 * KB + (q % 64) * KB + k % KB]
@@ -67,6 +67,11 @@ struct MixedArgs {
   const int64_t* sptr16;
   float* fa;              // per node alpha = 15 G / m           (force pass)
   float* fb;              // per node theta ((3K/m) - alpha / 3)  (force pass)
+  // particles with open LDS tiles: max |u - u(first node)| over the particle
+  // (rounded up to float), for the fixed-point step of their windows
+  const int* open_pids;
+  int n_open_pids;
+  float* pumax;           // [np]
 };
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -104,17 +109,29 @@ __device__ __forceinline__ int block_id(const uint4& b, int t) {
   return (int)((w >> ((t & 1) * 16)) & 0xffffu);
 }
 
-// node data access by particle-relative id jp (node gs + jp): LDS window or HBM.
+// Node data by particle-relative id jp (node gs + jp): the LDS window or HBM.
+//
+// LDS window records, per window node: a float4 of the FP32 position offset
+// from the particle's first reference node and the volume; an int4 of the
+// displacement relative to the window's first node in fixed point (three
+// int32 in units of a power-of-two step `us` chosen per window so that every
+// value stays below 2^29 in magnitude) and, in the force pass, the node's
+// b term; the node's alpha term as a float.  A bond's relative displacement
+// is the exact integer difference of two records, rounded once to FP32 and
+// scaled by us (exact): as accurate as rounding the FP64 difference to FP32,
+// plus a quantisation of 2^-29 of the window's largest relative
+// displacement.  32 bytes per bond in the dilation pass, 36 in the force pass.
+// HBM tiles read the FP64 displacements (us = 1).
 template <int DIM, bool LDS>
 struct Win {
   const float4* lxv;
-  const double2* lu01;
-  const double* lu2;
-  const float2* lab;
+  const int4* lq;
+  const float* la;
   const DS* s;
   const MixedArgs* m;
-  int gs;   // particle's first node
-  int off;  // gs - window start: LDS index of jp
+  int gs;    // particle's first node
+  int off;   // gs - window start: LDS index of jp
+  float us;  // fixed-point step of the displacement records (1 for HBM tiles)
   double org[3];
   __device__ __forceinline__ float4 xv(int jp) const {
     if (LDS) return lxv[jp + off];
@@ -122,236 +139,384 @@ struct Win {
     return make_float4((float)(s->xr[0][j] - org[0]), (float)(s->xr[1][j] - org[1]),
                        DIM == 3 ? (float)(s->xr[2][j] - org[2]) : 0.f, (float)s->vol[j]);
   }
-  __device__ __forceinline__ void u(int jp, double* out) const {
+  // the node's own displacement record (fixed point) or displacement (FP64)
+  struct Own {
+    int q[3];
+    double u[3];
+    float b;  // the node's b term (force pass)
+  };
+  __device__ __forceinline__ Own own(int jp) const {
+    Own o{};
     if (LDS) {
-      const double2 q = lu01[jp + off];
-      out[0] = q.x;
-      out[1] = q.y;
-      if (DIM == 3) out[2] = lu2[jp + off];
+      const int4 r = lq[jp + off];
+      o.q[0] = r.x;
+      o.q[1] = r.y;
+      o.q[2] = r.z;
+      o.b = __int_as_float(r.w);
     } else {
-      for (int d = 0; d < DIM; d++) out[d] = s->u[d][gs + jp];
+      for (int d = 0; d < DIM; d++) o.u[d] = s->u[d][gs + jp];
+    }
+    return o;
+  }
+  // relative displacement of neighbor jp in units of us (FP32); with B, bj
+  // gets the neighbor's b term from the same LDS record (the dilation pass
+  // reads only the displacement words: in the fused kernel it runs while b
+  // terms are being written)
+  template <bool B>
+  __device__ __forceinline__ void du(int jp, const Own& o, float* out, float& bj) const {
+    if (LDS) {
+      if (B) {
+        const int4 r = lq[jp + off];
+        out[0] = (float)(r.x - o.q[0]);
+        out[1] = (float)(r.y - o.q[1]);
+        if (DIM == 3) out[2] = (float)(r.z - o.q[2]);
+        bj = __int_as_float(r.w);
+      } else {
+        const int* r = (const int*)(lq + jp + off);
+        out[0] = (float)(r[0] - o.q[0]);
+        out[1] = (float)(r[1] - o.q[1]);
+        if (DIM == 3) out[2] = (float)(r[2] - o.q[2]);
+        bj = 0.f;
+      }
+    } else {
+      for (int d = 0; d < DIM; d++) out[d] = (float)(s->u[d][gs + jp] - o.u[d]);
+      bj = 0.f;
     }
   }
-  __device__ __forceinline__ float2 ab(int jp) const {
-    return LDS ? lab[jp + off] : make_float2(m->fa[gs + jp], m->fb[gs + jp]);
+  __device__ __forceinline__ float2 ab(int jp, float bj) const {
+    return LDS ? make_float2(la[jp + off], bj) : make_float2(m->fa[gs + jp], m->fb[gs + jp]);
   }
 };
 
+// max |u - u(particle's first node)| over nodes [n0, n1) of the workgroup's
+// threads, as a float rounded up (the fixed-point step only has to bound it);
+// the maximum does not depend on the order, so every tile of a particle gets
+// the same value
+template <int DIM>
+__device__ __forceinline__ float particle_umax(const DS& s, int gs, int n0, int n1, float* red) {
+  double u0[3] = {0., 0., 0.};
+  for (int d = 0; d < DIM; d++) u0[d] = s.u[d][gs];
+  double mx = 0.;
+  for (int j = n0 + (int)threadIdx.x; j < n1; j += blockDim.x)
+    for (int d = 0; d < DIM; d++) mx = fmax(mx, fabs(s.u[d][j] - u0[d]));
+  float mf = (float)(mx * (1.0 + 0x1p-20));
+  for (int o = WAVE / 2; o > 0; o >>= 1) mf = fmaxf(mf, __shfl_xor(mf, o));
+  if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = mf;
+  __syncthreads();
+  mf = 0.f;
+  for (int w = 0; w < (int)(blockDim.x / WAVE); w++) mf = fmaxf(mf, red[w]);
+  __syncthreads();
+  return mf;
+}
+
+// the maximum for the particles with open LDS tiles: one workgroup each
+template <int DIM>
+__global__ void __launch_bounds__(256) k_mixed_umax(DS s, MixedArgs m) {
+  __shared__ float red[256 / WAVE];
+  const int pid = m.open_pids[blockIdx.x];
+  const DevParticle& p = s.P[pid];
+  const float mf = particle_umax<DIM>(s, (int)p.glob_start, (int)p.glob_start, (int)p.glob_end, red);
+  if (threadIdx.x == 0) m.pumax[pid] = mf;
+}
+
+// Stage a tile's window in LDS: positions and volumes, fixed-point
+// displacements relative to the particle's first node (the step us: the
+// smallest power of two with max |u - u_first| / us < 2^29 over the whole
+// particle, so that the records, and the results, do not depend on the tile
+// shape) and, for the force pass of an open tile, the dilation terms of
+// every window node.  A closed tile (the whole particle) takes the maximum
+// over its window; an open tile reads it from k_mixed_umax.  Returns us.
 template <int DIM, bool FORCE>
-__device__ __forceinline__ void stage_window(const DS& s, const MixedArgs& m, const DevTile& t,
-                                             float4* lxv, double2* lu01, double* lu2, float2* lab,
-                                             const double* org) {
+__device__ __forceinline__ float stage_window(const DS& s, const MixedArgs& m, const DevTile& t, float4* lxv,
+                                              int4* lq, float* la, const double* org, float* red) {
   const int nw = t.we - t.ws;
+  const int gs = (int)s.P[t.pid].glob_start;
+  double u0[3] = {0., 0., 0.};
+  for (int d = 0; d < DIM; d++) u0[d] = s.u[d][gs];
   for (int q = threadIdx.x; q < nw; q += blockDim.x) {
     const int j = t.ws + q;
-    lu01[q] = make_double2(s.u[0][j], s.u[1][j]);
-    if (DIM == 3) lu2[q] = s.u[2][j];
     lxv[q] = make_float4((float)(s.xr[0][j] - org[0]), (float)(s.xr[1][j] - org[1]),
                          DIM == 3 ? (float)(s.xr[2][j] - org[2]) : 0.f, (float)s.vol[j]);
-    if (FORCE) lab[q] = make_float2(m.fa[j], m.fb[j]);
+  }
+  const float mf = t.lds == 2 ? particle_umax<DIM>(s, gs, t.ws, t.we, red) : m.pumax[t.pid];
+  // us = 2^e with mf / us < 2^29 (e >= -126: a window at rest keeps all zeros)
+  int e = -126;
+  if (mf > 0.f) {
+    int ex;
+    frexpf(mf, &ex);  // mf < 2^ex
+    e = ex - 29 > -126 ? ex - 29 : -126;
+  }
+  const float us = ldexpf(1.f, e);
+  const double inv_us = ldexp(1., -e);
+  for (int q = threadIdx.x; q < nw; q += blockDim.x) {
+    const int j = t.ws + q;
+    int qq[3] = {0, 0, 0};
+    for (int d = 0; d < DIM; d++) qq[d] = (int)rint((s.u[d][j] - u0[d]) * inv_us);
+    lq[q] = make_int4(qq[0], qq[1], qq[2], FORCE ? __float_as_int(m.fb[j]) : 0);
+    if (FORCE) la[q] = m.fa[j];
   }
   __syncthreads();
+  return us;
 }
 
-// Dilation pass (demModel.cpp:610-679) in mixed precision.
+// Per-particle constants of the mixed bond loops (FP32, folded once per
+// workgroup).
+template <int INF>
+struct StateConsts {
+  float low_rel;   // r > low_rel: isGreater(r, horizon - h/2), the volume correction applies
+  float thr_rel;   // |S| > thr_rel: the bond breaks (isGreater(|S|, s0 + 1e-10))
+  float inv_h, up_h, inv_s0, Kn, Rc;
+  double K, G;
+  InfF<INF> J;
+  __device__ StateConsts(const DS& s, const DevParticle& p)
+      : J(s, rcpf((float)p.horizon)) {
+    const float horizon = (float)p.horizon, h = (float)p.h;
+    const float check_up = horizon + 0.5f * h, check_low = horizon - 0.5f * h;
+    low_rel = check_low / REL_KEEP;
+    thr_rel = (float)(p.mat[2] + 1.0e-10) / REL_KEEP;
+    inv_h = rcpf(h);
+    up_h = check_up * inv_h;
+    inv_s0 = (float)(1.0 / p.mat[2]);
+    Kn = (float)p.Kn;
+    Rc = (float)p.Rc;
+    K = p.mat[0];
+    G = p.mat[1];
+  }
+};
+
+// Dilation of the node in thread slot q (demModel.cpp:610-679) in mixed
+// precision: breaks bonds, writes theta and returns the node's force-pass
+// terms (alpha, theta (3K/m - alpha/3)) as FP32.
+template <int DIM, int INF, bool LDS>
+__device__ __forceinline__ float2 dilation_node(const DS& s, const MixedArgs& m, const Win<DIM, LDS>& w,
+                                                const StateConsts<INF>& c, int q, int& node) {
+#pragma clang fp contract(fast)
+  const int i = m.slot[q];
+  node = i;
+  const int ip = i - w.gs;
+  const uint4* cb = (const uint4*)(m.col16 + m.sptr16[q / WAVE] + (q % WAVE) * KB);
+  unsigned* bw = s.bits + s.bptr[i / WAVE] + i % WAVE;
+  const int n = s.deg[i];
+  const float4 xvi = w.xv(ip);
+  const float xi[3] = {xvi.x, xvi.y, xvi.z};
+  const auto own = w.own(ip);
+  const float us = w.us;
+  double theta = 0.;
+  unsigned word = 0, wnew = 0;
+  uint4 blk = n > 0 ? cb[0] : make_uint4(0, 0, 0, 0);
+  for (int kb = 0; kb < n; kb += KB) {
+    if ((kb & 31) == 0) {
+      if (kb > 0 && wnew != word) bw[(int64_t)((kb >> 5) - 1) * WAVE] = wnew;
+      word = bw[(int64_t)(kb >> 5) * WAVE];
+      wnew = word;
+    }
+    const uint4 cur = blk;
+    if (kb + KB < n) blk = cb[(int64_t)(kb / KB + 1) * WAVE];
+    auto bond = [&](const int tt) {
+      const int k = kb + tt;
+      if (k < n) {
+        const int jp = block_id(cur, tt);
+        const float4 xvj = w.xv(jp);
+        const float xj[3] = {xvj.x, xvj.y, xvj.z};
+        float dx[3], du[3], bj;
+        w.template du<false>(jp, own, du, bj);  // in units of us
+        for (int d = 0; d < DIM; d++) dx[d] = xj[d] - xi[d];
+        float volj = xvj.w;
+        float xx = dx[0] * dx[0], xe = dx[0] * du[0], ee = du[0] * du[0];
+        for (int d = 1; d < DIM; d++) {
+          xx += dx[d] * dx[d];
+          xe += dx[d] * du[d];
+          ee += du[d] * du[d];
+        }
+        const float t2 = us * fmaf(us, ee, 2.f * xe);  // |xi + eta|^2 - |xi|^2
+        const float inv_r = rsqf_hw(xx);
+        const float r = xx * inv_r;
+        const float cl = t2 * rcpf(sqrtf_hw(xx + t2) + r);
+        const unsigned bit = 1u << (k & 31);
+        bool fs = (word & bit) != 0u;
+        if (!fs && fabsf(cl) * inv_r > c.thr_rel) {
+          fs = true;
+          wnew |= bit;
+        }
+        if (!fs) {
+          if (r > c.low_rel) volj *= c.up_h - r * c.inv_h;
+          theta += (double)(r * cl * c.J(r, xx) * volj);
+        }
+      }
+    };
+#pragma unroll
+    for (int tt = 0; tt < KB; tt++) bond(tt);
+  }
+  if (n > 0 && wnew != word) bw[(int64_t)((n - 1) >> 5) * WAVE] = wnew;
+  const double mi = s.mx[i];
+  const double th = 3. * theta / mi;
+  s.theta[i] = th;
+  const double alpha = 15. * c.G / mi;
+  return make_float2((float)alpha, (float)(th * ((3. * c.K / mi) - alpha / 3.)));
+}
+
+// Force of the node in thread slot q (demModel.cpp:682-788,
+// PdState::getBondEF material.h:1415-1429) in mixed precision:
+// ef_i + ef_j = J (r (b_i + b_j) + cl (a_i + a_j)).  The damage Z_i is the
+// plain maximum of |S| / s0 over the list (the reference's running compare
+// with a 1e-5 relative slack keeps a value at most 1e-5 below it; FP32 bond
+// arithmetic is the larger difference).
+template <int DIM, int INF, bool LDS>
+__device__ __forceinline__ void force_node(const DS& s, const MixedArgs& m, const Win<DIM, LDS>& w,
+                                           const StateConsts<INF>& c, int q) {
+#pragma clang fp contract(fast)
+  const int i = m.slot[q];
+  const int ip = i - w.gs;
+  const uint4* cb = (const uint4*)(m.col16 + m.sptr16[q / WAVE] + (q % WAVE) * KB);
+  const unsigned* bw = s.bits + s.bptr[i / WAVE] + i % WAVE;
+  const int n = s.deg[i];
+  const float4 xvi = w.xv(ip);
+  const float xi[3] = {xvi.x, xvi.y, xvi.z};
+  const auto own = w.own(ip);
+  const float us = w.us;
+  const float2 abi = w.ab(ip, own.b);
+  const float ai = abi.x, bi = abi.y;
+  double force[3] = {0., 0., 0.};
+  float Zs = 0.f;  // maximum of |S|; Z = Zs / s0
+  unsigned word = 0;
+  uint4 blk = n > 0 ? cb[0] : make_uint4(0, 0, 0, 0);
+  for (int kb = 0; kb < n; kb += KB) {
+    if ((kb & 31) == 0) word = bw[(int64_t)(kb >> 5) * WAVE];
+    const uint4 cur = blk;
+    if (kb + KB < n) blk = cb[(int64_t)(kb / KB + 1) * WAVE];
+    float fblk[3] = {0.f, 0.f, 0.f};
+    auto bond = [&](const int tt) {
+      const int k = kb + tt;
+      if (k < n) {
+        const int jp = block_id(cur, tt);
+        const float4 xvj = w.xv(jp);
+        const float xj[3] = {xvj.x, xvj.y, xvj.z};
+        float dx[3], du[3], y[3], bj;
+        w.template du<true>(jp, own, du, bj);  // in units of us
+        for (int d = 0; d < DIM; d++) {
+          dx[d] = xj[d] - xi[d];
+          y[d] = fmaf(us, du[d], dx[d]);
+        }
+        float volj = xvj.w;
+        float xx = dx[0] * dx[0], xe = dx[0] * du[0], ee = du[0] * du[0];
+        for (int d = 1; d < DIM; d++) {
+          xx += dx[d] * dx[d];
+          xe += dx[d] * du[d];
+          ee += du[d] * du[d];
+        }
+        const float t2 = us * fmaf(us, ee, 2.f * xe);  // |y|^2 - |xi|^2
+        const float yy = xx + t2;
+        const float inv_r = rsqf_hw(xx), inv_ly = rsqf_hw(yy);
+        const float r = xx * inv_r, ly = yy * inv_ly;
+        const float cl = t2 * rcpf(ly + r);
+        const bool fs = (word & (1u << (k & 31))) != 0u;
+        float sf;
+        if (!fs) {
+          if (r > c.low_rel) volj *= c.up_h - r * c.inv_h;
+          const float Jr = c.J(r, xx);
+          const float2 abj = w.ab(jp, bj);
+          sf = Jr * (r * (bi + abj.y) + cl * (ai + abj.x)) * volj * inv_ly;
+        } else {
+          sf = c.Kn * volj * (ly - c.Rc) * inv_ly;
+          if (sf > 0.f) sf = 0.f;
+        }
+        for (int d = 0; d < DIM; d++) fblk[d] += sf * y[d];
+        Zs = fmaxf(Zs, fabsf(cl) * inv_r);
+      }
+    };
+#pragma unroll
+    for (int tt = 0; tt < KB; tt++) bond(tt);
+    for (int d = 0; d < DIM; d++) force[d] += (double)fblk[d];
+  }
+  for (int d = 0; d < DIM; d++) s.f[d][i] = force[d];
+  s.Z[i] = Zs * c.inv_s0;
+}
+
+// Tile ranges of MixedArgs::tiles: closed LDS tiles (whole particles, fused
+// kernel), open LDS tiles, HBM tiles.
+__device__ __forceinline__ const DevTile& tile_of(const MixedArgs& m, bool lds) {
+  return m.tiles[(lds ? m.ntiles_closed : m.ntiles_closed + m.ntiles_lds) + blockIdx.x];
+}
+
+// LDS of the window records (LDS tiles) or a dummy (HBM tiles)
+#define PDEM_WINDOW_LDS(LDS_ON, NODES)                 \
+  __shared__ float4 lxv[(LDS_ON) ? (NODES) : 1];       \
+  __shared__ int4 lq[(LDS_ON) ? (NODES) : 1];          \
+  __shared__ float la[(LDS_ON) ? (NODES) : 1];         \
+  __shared__ float red[1024 / WAVE];
+
+// Dilation pass of the open tiles (tiles that do not hold a whole particle:
+// the force pass of a tile reads the dilation terms of nodes of other tiles).
 template <int DIM, int INF, bool LDS, int TPB, int NODES = LDS_NODES>
 __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_dilation(DS s, MixedArgs m) {
-#pragma clang fp contract(fast)
-  __shared__ float4 lxv[LDS ? NODES : 1];
-  __shared__ double2 lu01[LDS ? NODES : 1];
-  __shared__ double lu2[LDS && DIM == 3 ? NODES : 1];
-  const DevTile t = m.tiles[LDS ? blockIdx.x : m.ntiles_lds + blockIdx.x];
+  PDEM_WINDOW_LDS(LDS, NODES)
+  const DevTile t = tile_of(m, LDS);
   const DevParticle& p = s.P[t.pid];
   const int gs = (int)p.glob_start;
-  Win<DIM, LDS> w{lxv, lu01, lu2, nullptr, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
+  Win<DIM, LDS> w{lxv, lq, la, &s, &m, gs, gs - t.ws, 1.f, {0., 0., 0.}};
   for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][gs];
-  if (LDS) stage_window<DIM, false>(s, m, t, lxv, lu01, lu2, nullptr, w.org);
-  const float horizon = (float)p.horizon, h = (float)p.h;
-  const float check_up = horizon + 0.5f * h, check_low = horizon - 0.5f * h;
-  const float low_rel = check_low / REL_KEEP;             // r > low_rel: isGreater(r, check_low)
-  const float thr_rel = (float)(p.mat[2] + 1.0e-10) / REL_KEEP;  // |S| > thr_rel: breaks
-  const float inv_h = rcpf(h), inv_delta = rcpf(horizon), up_h = check_up * inv_h;
-  const InfF<INF> J(s, inv_delta);
-  const double K = p.mat[0], G = p.mat[1];
+  if (LDS) w.us = stage_window<DIM, false>(s, m, t, lxv, lq, la, w.org, red);
+  const StateConsts<INF> c(s, p);
   for (int q = t.ts + threadIdx.x; q < t.te; q += blockDim.x) {
-    const int i = m.slot[q];
-    const int ip = i - gs;
-    const uint4* cb = (const uint4*)(m.col16 + m.sptr16[q / WAVE] + (q % WAVE) * KB);
-    unsigned* bw = s.bits + s.bptr[i / WAVE] + i % WAVE;
-    const int n = s.deg[i];
-    const float4 xvi = w.xv(ip);
-    const float xi[3] = {xvi.x, xvi.y, xvi.z};
-    double ui[3];
-    w.u(ip, ui);
-    double theta = 0.;
-    unsigned word = 0, wnew = 0;
-    uint4 blk = n > 0 ? cb[0] : make_uint4(0, 0, 0, 0);
-    for (int kb = 0; kb < n; kb += KB) {
-      if ((kb & 31) == 0) {
-        if (kb > 0 && wnew != word) bw[(int64_t)((kb >> 5) - 1) * WAVE] = wnew;
-        word = bw[(int64_t)(kb >> 5) * WAVE];
-        wnew = word;
-      }
-      const uint4 cur = blk;
-      if (kb + KB < n) blk = cb[(int64_t)(kb / KB + 1) * WAVE];
-      auto bond = [&](const int tt) {
-        const int k = kb + tt;
-        if (k < n) {
-          const int jp = block_id(cur, tt);
-          const float4 xvj = w.xv(jp);
-          const float xj[3] = {xvj.x, xvj.y, xvj.z};
-          double uj[3];
-          w.u(jp, uj);
-          float dx[3], du[3];
-          for (int d = 0; d < DIM; d++) {
-            dx[d] = xj[d] - xi[d];
-            du[d] = (float)(uj[d] - ui[d]);
-          }
-          float volj = xvj.w;
-          float xx = dx[0] * dx[0], xe = dx[0] * du[0], ee = du[0] * du[0];
-          for (int d = 1; d < DIM; d++) {
-            xx += dx[d] * dx[d];
-            xe += dx[d] * du[d];
-            ee += du[d] * du[d];
-          }
-          const float t2 = 2.f * xe + ee;  // |xi + eta|^2 - |xi|^2
-          const float inv_r = rsqf_hw(xx);
-          const float r = xx * inv_r;
-          const float cl = t2 * rcpf(sqrtf_hw(xx + t2) + r);
-          const unsigned bit = 1u << (k & 31);
-          bool fs = (word & bit) != 0u;
-          if (!fs && fabsf(cl) * inv_r > thr_rel) {
-            fs = true;
-            wnew |= bit;
-          }
-          if (!fs) {
-            if (r > low_rel) volj *= up_h - r * inv_h;
-            theta += (double)(r * cl * J(r, xx) * volj);
-          }
-        }
-      };
-#pragma unroll
-      for (int tt = 0; tt < KB; tt++) bond(tt);
-    }
-    if (n > 0 && wnew != word) bw[(int64_t)((n - 1) >> 5) * WAVE] = wnew;
-    const double mi = s.mx[i];
-    const double th = 3. * theta / mi;
-    s.theta[i] = th;
-    const double alpha = 15. * G / mi;
-    m.fa[i] = (float)alpha;
-    m.fb[i] = (float)(th * ((3. * K / mi) - alpha / 3.));
+    int i;
+    const float2 ab = dilation_node<DIM, INF, LDS>(s, m, w, c, q, i);
+    m.fa[i] = ab.x;
+    m.fb[i] = ab.y;
   }
 }
 
-// Force pass (demModel.cpp:682-788, PdState::getBondEF material.h:1415-1429)
-// in mixed precision: ef_i + ef_j = J (r (b_i + b_j) + cl (a_i + a_j)).
-// The damage Z_i is the plain maximum of |S| / s0 over the list (the
-// reference's running compare with a 1e-5 relative slack keeps a value at
-// most 1e-5 below it; FP32 bond arithmetic is the larger difference).
+// Force pass of the open tiles.
 template <int DIM, int INF, bool LDS, int TPB, int NODES = LDS_NODES>
 __global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_force(DS s, MixedArgs m) {
-#pragma clang fp contract(fast)
-  __shared__ float4 lxv[LDS ? NODES : 1];
-  __shared__ double2 lu01[LDS ? NODES : 1];
-  __shared__ double lu2[LDS && DIM == 3 ? NODES : 1];
-  __shared__ float2 lab[LDS ? NODES : 1];
-  const DevTile t = m.tiles[LDS ? blockIdx.x : m.ntiles_lds + blockIdx.x];
+  PDEM_WINDOW_LDS(LDS, NODES)
+  const DevTile t = tile_of(m, LDS);
   const DevParticle& p = s.P[t.pid];
   const int gs = (int)p.glob_start;
-  Win<DIM, LDS> w{lxv, lu01, lu2, lab, &s, &m, gs, gs - t.ws, {0., 0., 0.}};
+  Win<DIM, LDS> w{lxv, lq, la, &s, &m, gs, gs - t.ws, 1.f, {0., 0., 0.}};
   for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][gs];
-  if (LDS) stage_window<DIM, true>(s, m, t, lxv, lu01, lu2, lab, w.org);
-  const float horizon = (float)p.horizon, h = (float)p.h;
-  const float check_up = horizon + 0.5f * h, check_low = horizon - 0.5f * h;
-  const float low_rel = check_low / REL_KEEP;  // r > low_rel: isGreater(r, check_low)
-  const float inv_h = rcpf(h), inv_delta = rcpf(horizon), inv_s0 = (float)(1.0 / p.mat[2]);
-  const float up_h = check_up * inv_h;
-  const InfF<INF> J(s, inv_delta);
-  const float Kn = (float)p.Kn, Rc = (float)p.Rc;
-  for (int q = t.ts + threadIdx.x; q < t.te; q += blockDim.x) {
-    const int i = m.slot[q];
-    const int ip = i - gs;
-    const uint4* cb = (const uint4*)(m.col16 + m.sptr16[q / WAVE] + (q % WAVE) * KB);
-    const unsigned* bw = s.bits + s.bptr[i / WAVE] + i % WAVE;
-    const int n = s.deg[i];
-    const float4 xvi = w.xv(ip);
-    const float xi[3] = {xvi.x, xvi.y, xvi.z};
-    double ui[3];
-    w.u(ip, ui);
-    const float2 abi = w.ab(ip);
-    const float ai = abi.x, bi = abi.y;
-    double force[3] = {0., 0., 0.};
-    float Zs = 0.f;  // maximum of |S|; Z = Zs / s0
-    unsigned word = 0;
-    uint4 blk = n > 0 ? cb[0] : make_uint4(0, 0, 0, 0);
-    for (int kb = 0; kb < n; kb += KB) {
-      if ((kb & 31) == 0) word = bw[(int64_t)(kb >> 5) * WAVE];
-      const uint4 cur = blk;
-      if (kb + KB < n) blk = cb[(int64_t)(kb / KB + 1) * WAVE];
-      float fblk[3] = {0.f, 0.f, 0.f};
-      auto bond = [&](const int tt) {
-        const int k = kb + tt;
-        if (k < n) {
-          const int jp = block_id(cur, tt);
-          const float4 xvj = w.xv(jp);
-          const float xj[3] = {xvj.x, xvj.y, xvj.z};
-          double uj[3];
-          w.u(jp, uj);
-          float dx[3], du[3], y[3];
-          for (int d = 0; d < DIM; d++) {
-            dx[d] = xj[d] - xi[d];
-            du[d] = (float)(uj[d] - ui[d]);
-            y[d] = dx[d] + du[d];
-          }
-          float volj = xvj.w;
-          float xx = dx[0] * dx[0], xe = dx[0] * du[0], ee = du[0] * du[0];
-          for (int d = 1; d < DIM; d++) {
-            xx += dx[d] * dx[d];
-            xe += dx[d] * du[d];
-            ee += du[d] * du[d];
-          }
-          const float t2 = 2.f * xe + ee;  // |y|^2 - |xi|^2
-          const float yy = xx + t2;
-          const float inv_r = rsqf_hw(xx), inv_ly = rsqf_hw(yy);
-          const float r = xx * inv_r, ly = yy * inv_ly;
-          const float cl = t2 * rcpf(ly + r);
-          const bool fs = (word & (1u << (k & 31))) != 0u;
-          float sf;
-          if (!fs) {
-            if (r > low_rel) volj *= up_h - r * inv_h;
-            const float Jr = J(r, xx);
-            const float2 abj = w.ab(jp);
-            sf = Jr * (r * (bi + abj.y) + cl * (ai + abj.x)) * volj * inv_ly;
-          } else {
-            sf = Kn * volj * (ly - Rc) * inv_ly;
-            if (sf > 0.f) sf = 0.f;
-          }
-          for (int d = 0; d < DIM; d++) fblk[d] += sf * y[d];
-          Zs = fmaxf(Zs, fabsf(cl) * inv_r);
-        }
-      };
-#pragma unroll
-      for (int tt = 0; tt < KB; tt++) bond(tt);
-      for (int d = 0; d < DIM; d++) force[d] += (double)fblk[d];
-    }
-    for (int d = 0; d < DIM; d++) s.f[d][i] = force[d];
-    s.Z[i] = Zs * inv_s0;
-  }
+  if (LDS) w.us = stage_window<DIM, true>(s, m, t, lxv, lq, la, w.org, red);
+  const StateConsts<INF> c(s, p);
+  for (int q = t.ts + threadIdx.x; q < t.te; q += blockDim.x) force_node<DIM, INF, LDS>(s, m, w, c, q);
 }
 
-// the force reset of nodes that take no PD force (a list built at set-up)
-// and, on a rebuild step, the binning of every node; grid-stride, so that a
-// step without a rebuild launches few workgroups
+// Both passes (demModel.cpp:610-679 and :682-788) of a closed tile, a whole
+// particle, so that every neighbor of its nodes is one of its nodes, in one
+// workgroup: the window is staged in LDS once, the dilation pass leaves each
+// node's force-pass terms in its LDS record, and after a barrier the force
+// pass reads them there.  Nothing of the dilation pass goes through HBM but
+// theta.
+template <int DIM, int INF, int TPB, int NODES = LDS_NODES>
+__global__ void __launch_bounds__(TPB, MIXED_WAVES_PER_SIMD) k_mixed_state(DS s, MixedArgs m) {
+  PDEM_WINDOW_LDS(true, NODES)
+  const DevTile t = m.tiles[blockIdx.x];
+  const DevParticle& p = s.P[t.pid];
+  const int gs = (int)p.glob_start;
+  Win<DIM, true> w{lxv, lq, la, &s, &m, gs, gs - t.ws, 1.f, {0., 0., 0.}};
+  for (int d = 0; d < DIM; d++) w.org[d] = s.xr[d][gs];
+  w.us = stage_window<DIM, false>(s, m, t, lxv, lq, la, w.org, red);
+  const StateConsts<INF> c(s, p);
+  for (int q = t.ts + threadIdx.x; q < t.te; q += blockDim.x) {
+    int i;
+    const float2 ab = dilation_node<DIM, INF, true>(s, m, w, c, q, i);
+    la[i - t.ws] = ab.x;
+    lq[i - t.ws].w = __float_as_int(ab.y);
+  }
+  __syncthreads();
+  for (int q = t.ts + threadIdx.x; q < t.te; q += blockDim.x) force_node<DIM, INF, true>(s, m, w, c, q);
+}
+
+#undef PDEM_WINDOW_LDS
+
+// the force reset of nodes that take no PD force (a list built at set-up);
+// grid-stride
 constexpr int BZ_BLOCKS = 1024;
 template <int DIM>
-__global__ void k_bin_zero(DS s, int bin) {
+__global__ void k_zero_noforce(DS s) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t k = gid(); k < s.n_noforce; k += stride) {
     const int64_t i = s.noforce_nodes[k];
     for (int d = 0; d < DIM; d++) s.f[d][i] = 0.;
   }
-  if (bin && s.ctrl->rebuild)
-    for (int64_t i = gid(); i < s.N; i += stride) bin_node<DIM>(s, i);
 }
 
 }  // namespace pdem

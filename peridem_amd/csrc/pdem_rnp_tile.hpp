@@ -37,7 +37,7 @@ constexpr uint16_t RT_OUT = 0xFFFF;
 struct RtTile {
   int t0, t1;  // node range [t0, t1), inside one particle
   int npairs;  // in-tile pairs (i < j)
-  int work;    // 1: RNP particle with forces; 0: zero the force (and bin) only
+  int work;    // 1: RNP particle with forces; 0: zero the force only
   int64_t pbase;  // first pair of the tile in the pair arrays
 };
 
@@ -51,14 +51,13 @@ struct RnpTileArgs {
 };
 
 template <int DIM, int INF>
-__global__ void __launch_bounds__(RT_THREADS) k_pd_rnp_tile(DS s, RnpTileArgs a, int bin) {
+__global__ void __launch_bounds__(RT_THREADS) k_pd_rnp_tile(DS s, RnpTileArgs a) {
   extern __shared__ double lds[];
   const RtTile t = a.tiles[blockIdx.x];
   const int n_t = t.t1 - t.t0;
   if (!t.work) {
     for (int q = threadIdx.x; q < n_t; q += blockDim.x) {
       const int64_t i = t.t0 + q;
-      if (bin && s.ctrl->rebuild) bin_node<DIM>(s, i);
       if (!s.P[s.ptid[i]].compute_force)
         for (int d = 0; d < DIM; d++) s.f[d][i] = 0.;
     }
@@ -112,7 +111,6 @@ __global__ void __launch_bounds__(RT_THREADS) k_pd_rnp_tile(DS s, RnpTileArgs a,
   for (int qq = threadIdx.x; qq < n_t; qq += blockDim.x) {
     const int q = a.perm[t.t0 + qq];
     const int64_t i = t.t0 + q;
-    if (bin && s.ctrl->rebuild) bin_node<DIM>(s, i);
     const int64_t sl = i / WAVE, lane = i % WAVE;
     const int* __restrict__ c = s.col + s.sptr[sl] + lane;
     const uint16_t* __restrict__ ps = a.ps + s.sptr[sl] + lane;

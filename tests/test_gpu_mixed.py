@@ -17,28 +17,109 @@ def _errors(fg, fr):
     return float(d.max() / den), float(np.sqrt((d ** 2).sum(1).mean()) / den)
 
 
-@pytest.mark.parametrize("steps", [0, 40])
-def test_mixed_state_based_force_error(steps):
+def _smooth_displacement(xr, amp, lam):
+    """A smooth non-rigid displacement field (peak strain ~ 2 pi amp / lam):
+    every bond carries force."""
+    u = np.zeros_like(xr)
+    for d in range(xr.shape[1]):
+        u[:, d] = amp * np.sin(2 * np.pi * xr[:, (d + 1) % xr.shape[1]] / lam + d)
+    return u
+
+
+@pytest.mark.parametrize("case", ["deformed", "steps40"])
+def test_mixed_state_based_force_error(case):
     spec = configs.packing_3d(n_spheres=6, nodes_per_sphere=300, L=5.0e-3, seed=2,
                               gap_factor=1.0, init_speed=2.0)
-    print(f"[mixed] nodes={spec.n_nodes} steps={steps}", flush=True)
+    print(f"[mixed] nodes={spec.n_nodes} case={case}", flush=True)
     g = capi.GpuEngine(spec, capi.PREC_MIXED)
     r = oracle.OracleEngine(spec, 8)
     g.integrate_begin()
     r.integrate_begin()
-    if steps:
-        g.integrate_cd(steps)
-        r.integrate_cd(steps)
-    emax, erms = _errors(g.field(capi.FIELD_F), r.field(capi.FIELD_F))
+    if case == "deformed":
+        u = _smooth_displacement(spec.x_ref, 3.0e-7, 2.0e-3)
+        for e in (g, r):
+            e.set_field(capi.FIELD_U, u)
+            e.set_field(capi.FIELD_X, spec.x_ref + u)
+            e.compute_forces()
+    else:
+        g.integrate_cd(40)
+        r.integrate_cd(40)
+    fr = r.field(capi.FIELD_F)
+    emax, erms = _errors(g.field(capi.FIELD_F), fr)
     th = g.field(capi.FIELD_THETA)
     thr = r.field(capi.FIELD_THETA)
     eth = float(np.abs(th - thr).max() / max(np.abs(thr).max(), 1e-300))
-    print(f"[mixed] steps={steps} force max err {emax:.3e} rms {erms:.3e} theta err {eth:.3e}",
-          flush=True)
-    assert emax < 1e-4 and erms < 1e-5
+    print(f"[mixed] {case}: max |f| {np.abs(fr).max():.3e} force max err {emax:.3e} rms {erms:.3e} "
+          f"theta err {eth:.3e}", flush=True)
+    assert np.abs(thr).max() > 0.0
+    assert 0.0 < emax < 1e-4 and erms < 1e-5
+    assert eth < 1e-5
     # neighbor sets are the same integer construction in both modes
     a, b = g.neighbors(capi.NEIGH_PD), r.neighbors(capi.NEIGH_PD)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_mixed_2d_large_window_against_oracle():
+    """2D particles of more than 384 nodes (mesh size R/12): the mixed kernels'
+    large-window instantiation in 2D, on a deformed state and after contact
+    steps, against the oracle."""
+    spec = configs.ensemble_2d(n_instances=2, n_min=3, n_max=4, ratios=(12,), speed_lo=2.0, speed_hi=4.0,
+                               pitch_factor=1.0, R_lo=1.0e-3, R_hi=1.0e-3, seed=3)
+    sizes = [p["glob_end"] - p["glob_start"] for p in spec.particles]
+    print(f"[mixed-2d] nodes={spec.n_nodes} particle sizes {min(sizes)}..{max(sizes)}", flush=True)
+    assert min(sizes) > 384
+    g = capi.GpuEngine(spec, capi.PREC_MIXED)
+    r = oracle.OracleEngine(spec, 8)
+    for e in (g, r):
+        e.integrate_begin()
+    u = _smooth_displacement(spec.x_ref, 2.0e-7, 1.5e-3)
+    for e in (g, r):
+        e.set_field(capi.FIELD_U, u)
+        e.set_field(capi.FIELD_X, spec.x_ref + u)
+        e.compute_peridynamic_forces()
+    emax, erms = _errors(g.field(capi.FIELD_F), r.field(capi.FIELD_F))
+    print(f"[mixed-2d] deformed: force max err {emax:.3e} rms {erms:.3e}", flush=True)
+    assert 0.0 < emax < 1e-4 and erms < 1e-5
+    g.integrate_cd(200)
+    r.integrate_cd(200)
+    emax, erms = _errors(g.field(capi.FIELD_F), r.field(capi.FIELD_F))
+    print(f"[mixed-2d] after 200 steps: force max err {emax:.3e} rms {erms:.3e}", flush=True)
+    assert emax < 1e-4 and erms < 1e-5
+
+
+@pytest.mark.parametrize("which", ["packing", "hbm_wall", "ensemble"])
+def test_state_fusion_is_bit_exact(which):
+    """The fused state-based kernel (whole particles in one workgroup:
+    dilation, barrier, force) against the two-pass kernels: identical forces,
+    dilations, bond states and damage after deforming steps, eager and
+    captured."""
+    if which == "packing":
+        spec = configs.packing_3d(n_spheres=6, nodes_per_sphere=300, L=5.0e-3, seed=2, gap_factor=1.0,
+                                  init_speed=2.0)
+    elif which == "hbm_wall":
+        spec = configs.packing_3d(n_spheres=2, nodes_per_sphere=1000, L=20.0e-3, seed=2, gap_factor=1.0,
+                                  init_speed=2.0)
+    else:
+        spec = configs.ensemble_2d(n_instances=4, n_min=2, n_max=6, ratios=(6, 10), speed_lo=2.0,
+                                   speed_hi=4.0, pitch_factor=1.0, R_lo=1.0e-3, R_hi=1.0e-3, seed=5)
+    out = []
+    for on in (True, False):
+        g = capi.GpuEngine(spec, capi.PREC_MIXED)
+        g.set_state_fusion(on)
+        g.integrate_begin()
+        u = _smooth_displacement(spec.x_ref, 3.0e-7, 2.0e-3)
+        g.set_field(capi.FIELD_U, u)
+        g.set_field(capi.FIELD_X, spec.x_ref + u)
+        g.integrate_cd(20)
+        g.integrate_cd_graph(40)
+        out.append({f: g.field(f) for f in (capi.FIELD_F, capi.FIELD_THETA, capi.FIELD_BONDS, capi.FIELD_Z,
+                                              capi.FIELD_U)})
+        print(f"[state-fusion] {which} fused={on} tiles {g.tile_counts()} config {g.launch_config()}",
+              flush=True)
+        g.close()
+    assert np.abs(out[0][capi.FIELD_F]).max() > 0.0
+    for k in out[0]:
+        np.testing.assert_array_equal(out[0][k], out[1][k])
 
 
 def test_mixed_graph_matches_eager():
