@@ -71,7 +71,7 @@ def two_particle(R=1.0e-3, h=1.0e-4, gap=1.0e-4, speed=0.1, dt=1.0e-7, n_steps=2
 def compressive_2d(n_particles=500, R_lo=0.8e-3, R_hi=1.2e-3, nodes_per_particle=400,
                    box=None, seed=0, dt=5.0e-8, n_steps=50000, wall_speed=0.1,
                    kn_factor=1.0e-3, epsilon=0.7, mu=0.5, interval=40, pitch_factor=1.08,
-                   init_speed=0.0):
+                   init_speed=0.0, friction=False):
     """Config 1: 2D compressive test, 50/50 circles and hexagons, R ~ U[0.8, 1.2] mm,
     random orientation, seeded jittered grid; ~400 nodes per particle; RNP
     (E = 2 GPa, nu = 0.25, rho = 1200, Gc = 500), horizon 3h; contact with
@@ -80,7 +80,13 @@ def compressive_2d(n_particles=500, R_lo=0.8e-3, R_hi=1.2e-3, nodes_per_particle
 
     Deviations: the box is enlarged from 40 x 40 mm to the grid the particles
     need without overlap (the stated box would be 90% full); Kn_Factor = 1e-3
-    for explicit stability at dt = 5e-8 (see two_particle)."""
+    for explicit stability at dt = 5e-8 (see two_particle); friction off
+    (Friction_Coeff 0.5 with Friction_On false, as the reference's own
+    compressive deck examples/PeriDEM/compressive_test/n500_circ_hex sets it):
+    the reference's friction term pushes node i against the relative sliding
+    velocity (demModel.cpp:896-897), which adds energy to every sliding
+    contact, and the wall-driven packing then blows up within tens of
+    microseconds of contact at any time step.  friction=True turns it on."""
     rng = np.random.default_rng(seed)
     mat = rnp_material()
     wall_mat = rnp_material()
@@ -104,7 +110,7 @@ def compressive_2d(n_particles=500, R_lo=0.8e-3, R_hi=1.2e-3, nodes_per_particle
     delta_ref = 3.0 * h_ref
     cp = ContactPairDeck(contact_radius_factor=0.95, Kn=_kn(mat, delta_ref * R_lo / 1e-3),
                          Kn_factor=kn_factor, epsilon=epsilon, beta_n_factor=1.0,
-                         friction_coeff=mu, damping_on=True, friction_on=True)
+                         friction_coeff=mu, damping_on=True, friction_on=friction)
     contact = {(a, b): cp for a in range(4) for b in range(a, 4)}
     b = ModelBuilder(2, dt, zones, contact, search_factor=5.0, neigh_update_interval=interval,
                      gravity=(0.0, -10.0, 0.0))
@@ -145,7 +151,12 @@ def compressive_2d(n_particles=500, R_lo=0.8e-3, R_hi=1.2e-3, nodes_per_particle
     b.ubc.append(BcDeck(particles=[top_id], directions=[2], time_fn=capi.UFN_LINEAR,
                         time_params=[-wall_speed]))
     spec = b.build(name=f"compressive_2d_n{n_particles}", n_steps=n_steps, wall_id=top_id, f_dir=1)
-    spec.info.update(box=(W, H), seed=seed, shape_of_zone=[0.0, 1.0, 0.0, 0.0])
+    spec.info.update(box=(W, H), seed=seed, shape_of_zone=[0.0, 1.0, 0.0, 0.0], wall_speed=wall_speed,
+                     args=dict(n_particles=n_particles, R_lo=R_lo, R_hi=R_hi,
+                               nodes_per_particle=nodes_per_particle, box=box, seed=seed, dt=dt,
+                               n_steps=n_steps, wall_speed=wall_speed, kn_factor=kn_factor,
+                               epsilon=epsilon, mu=mu, interval=interval, pitch_factor=pitch_factor,
+                               init_speed=init_speed, friction=friction))
     return spec
 
 
@@ -375,3 +386,53 @@ def ensemble_instance(spec, k):
     s.info = dict(spec.info, instance=k)
     s.name = f"{spec.name}_i{k}"
     return s
+
+
+# Prepared state of the 2D compressive test: the particles packed against the
+# top wall, with contacts everywhere and the first bonds breaking.  The
+# reference's own compressive example runs in stages restarted from an
+# initial configuration (examples/PeriDEM/compressive_test/n500_circ_hex:
+# init_config, run1, run2); this is that first stage, driven faster (a wall
+# at 0.5 m/s stays stable; at 2 m/s it crushes the packing within 0.5 ms).
+COMPACT_PREP_SPEED = 0.5      # m/s: top-wall speed of the preparation run
+COMPACT_PREP_STEPS = 160000   # preparation steps at the deck's dt
+
+
+def compacted_state(spec, prep_speed=COMPACT_PREP_SPEED, prep_steps=COMPACT_PREP_STEPS, precision=None,
+                    chunk=20000):
+    """Drive the top wall of a compressive test (built by compressive_2d) at
+    prep_speed for prep_steps steps on the GPU (FP64 unless precision says
+    otherwise), from the same seeded configuration, and return the state to
+    restart the deck's own run from: positions, displacements, velocities,
+    bond states and the restart step at which the deck's wall (speed
+    spec.info['wall_speed']) stands where the preparation left it.  Applied
+    with apply_state() to a GPU engine or the oracle alike."""
+    ws = spec.info["wall_speed"]
+    ratio = prep_speed / ws
+    if abs(ratio - round(ratio)) > 1e-12:
+        raise ValueError("the preparation speed must be a whole multiple of the deck's wall speed")
+    prep = compressive_2d(**dict(spec.info["args"], wall_speed=prep_speed))
+    e = capi.GpuEngine(prep, capi.PREC_FP64 if precision is None else precision)
+    e.integrate_begin()
+    done = 0
+    while done < prep_steps:
+        k = min(chunk, prep_steps - done)
+        e.integrate_cd_graph(k)
+        done += k
+    st = {"step": int(round(prep_steps * ratio)), "prep_speed": prep_speed, "prep_steps": prep_steps,
+          "x": e.field(capi.FIELD_X), "u": e.field(capi.FIELD_U), "v": e.field(capi.FIELD_V),
+          "bonds": e.field(capi.FIELD_BONDS)}
+    st["penetration"], st["reaction"] = e.pp_compressive_test(prep.wall_id, prep.f_dir)
+    st["broken_bonds"] = e.broken_bonds()
+    e.close()
+    return st
+
+
+def apply_state(engine, st):
+    """Load a compacted_state() into an engine of the deck's own model, as a
+    restart at st['step'] (call before integrate_begin)."""
+    engine.set_field(capi.FIELD_X, st["x"])
+    engine.set_field(capi.FIELD_U, st["u"])
+    engine.set_field(capi.FIELD_V, st["v"])
+    engine.set_field(capi.FIELD_BONDS, st["bonds"])
+    engine.restart(st["step"])

@@ -16,7 +16,7 @@ FIELDS = (capi.FIELD_X, capi.FIELD_U, capi.FIELD_V, capi.FIELD_F, capi.FIELD_Z, 
 def test_restore_continues_bit_for_bit(which):
     if which == "compressive_fp64":
         spec = configs.compressive_2d(n_particles=12, nodes_per_particle=150, seed=3, init_speed=1.0,
-                                      pitch_factor=1.01, R_lo=1.2e-3, R_hi=1.2e-3)
+                                      pitch_factor=1.01, R_lo=1.2e-3, R_hi=1.2e-3, friction=True)
         prec = capi.PREC_FP64
     else:
         spec = configs.packing_3d(n_spheres=6, nodes_per_sphere=300, L=5.0e-3, seed=2, gap_factor=1.0,
@@ -50,7 +50,7 @@ def test_restart_from_vtu_matches_oracle(tmp_path):
     from peridem_amd import output
     import oracle
     spec = configs.compressive_2d(n_particles=12, nodes_per_particle=120, seed=5, interval=10, wall_speed=2.0,
-                                  init_speed=1.0)
+                                  init_speed=1.0, friction=True)
     a = oracle.OracleEngine(spec, 8)
     a.integrate_begin()
     a.integrate_cd(120)

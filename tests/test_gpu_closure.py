@@ -28,7 +28,7 @@ def _compressive():
     # equal radii on a tight grid: particle-particle and particle-wall contact
     # within the test window (stable at 1 m/s for this seed)
     return configs.compressive_2d(n_particles=12, nodes_per_particle=150, seed=3, init_speed=1.0,
-                                  pitch_factor=1.01, R_lo=1.2e-3, R_hi=1.2e-3)
+                                  pitch_factor=1.01, R_lo=1.2e-3, R_hi=1.2e-3, friction=True)
 
 
 @pytest.mark.parametrize("which", ["ensemble", "compressive"])

@@ -69,7 +69,7 @@ def test_two_particle_bond_breaking():
 def test_compressive_small_fp64():
     spec = configs.compressive_2d(n_particles=12, nodes_per_particle=120, seed=3, interval=10,
                                   wall_speed=2.0, R_lo=1e-3, R_hi=1e-3, pitch_factor=1.03,
-                                  init_speed=2.0, dt=2e-8)
+                                  init_speed=2.0, dt=2e-8, friction=True)
     g, r = _run_compare(spec, chunks=5, chunk_steps=40, tag="compressive12")
     assert r.neighbor_count(capi.NEIGH_CONTACT) > 0
     pg = g.pp_compressive_test(spec.wall_id, spec.f_dir)
@@ -83,7 +83,7 @@ def test_compressive_rebuild_every_step():
     # rebuilds every step
     spec = configs.compressive_2d(n_particles=9, nodes_per_particle=100, seed=4, interval=1,
                                   wall_speed=1.0, R_lo=1e-3, R_hi=1e-3, pitch_factor=1.03,
-                                  init_speed=1.0, dt=2e-8)
+                                  init_speed=1.0, dt=2e-8, friction=True)
     _run_compare(spec, chunks=3, chunk_steps=30, tag="interval1")
 
 
@@ -96,7 +96,7 @@ def test_packing_3d_state_fp64():
 
 def test_graph_replay_matches_eager():
     spec = configs.compressive_2d(n_particles=12, nodes_per_particle=120, seed=5, interval=10,
-                                  wall_speed=2.0)
+                                  wall_speed=2.0, friction=True)
     a = capi.GpuEngine(spec)
     b = capi.GpuEngine(spec)
     a.integrate_begin()
@@ -117,7 +117,7 @@ def test_step_fusion_is_bit_exact(which):
     gravity and the moving wall (compressive) or a 3D mixed-precision run."""
     if which == "compressive":
         spec = configs.compressive_2d(n_particles=12, nodes_per_particle=120, seed=5, interval=10,
-                                      wall_speed=2.0, init_speed=1.0)
+                                      wall_speed=2.0, init_speed=1.0, friction=True)
         prec = capi.PREC_FP64
     else:
         spec = configs.packing_3d(n_spheres=6, nodes_per_sphere=200, L=5.0e-3, seed=3, gap_factor=1.0,
@@ -158,7 +158,7 @@ def test_two_particle_test_observables():
 
 def test_check_stop_quantities():
     spec = configs.compressive_2d(n_particles=12, nodes_per_particle=120, seed=5, interval=10, wall_speed=2.0,
-                                  init_speed=1.0)
+                                  init_speed=1.0, friction=True)
     g = capi.GpuEngine(spec)
     r = oracle.OracleEngine(spec, 8)
     g.integrate_begin()

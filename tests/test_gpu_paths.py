@@ -179,7 +179,7 @@ def test_rnp_kernels_agree(case):
     R = 1e-3
     if case == "compressive":
         spec = configs.compressive_2d(n_particles=12, nodes_per_particle=150, seed=3, init_speed=1.0,
-                                      pitch_factor=1.01, R_lo=1.2e-3, R_hi=1.2e-3)
+                                      pitch_factor=1.01, R_lo=1.2e-3, R_hi=1.2e-3, friction=True)
     elif case == "rnp2d_pull_break":
         m = configs.rnp_material(Gc=5.0)
         spec = _single(2, m, ubc=[BcDeck([], [1], capi.UFN_LINEAR, [-20.0],

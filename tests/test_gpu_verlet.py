@@ -45,7 +45,7 @@ def _cases():
         "two_particle": (configs.two_particle(gap=0.0, speed=0.5), 3, 40),
         "compressive_small": (configs.compressive_2d(n_particles=12, nodes_per_particle=150, seed=3,
                                                      init_speed=1.0, pitch_factor=1.01, R_lo=1.2e-3,
-                                                     R_hi=1.2e-3), 3, 200),
+                                                     R_hi=1.2e-3, friction=True), 3, 200),
     }
 
 

@@ -239,7 +239,7 @@ def test_damping_is_equal_and_opposite_for_mirror_particles():
 
 
 def test_linear_displacement_bc_and_reaction_at_rest():
-    spec = configs.compressive_2d(n_particles=4, nodes_per_particle=80, seed=2, wall_speed=0.3)
+    spec = configs.compressive_2d(n_particles=4, nodes_per_particle=80, seed=2, wall_speed=0.3, friction=True)
     o = oracle.OracleEngine(spec, 4)
     o.integrate_begin()
     pen0, reac0 = o.pp_compressive_test(spec.wall_id, spec.f_dir)
