@@ -285,8 +285,13 @@ struct StateConsts {
 };
 
 // Dilation of the node in thread slot q (demModel.cpp:610-679) in mixed
-// precision: breaks bonds, writes theta and returns the node's force-pass
-// terms (alpha, theta (3K/m - alpha/3)) as FP32.
+// precision: breaks bonds, writes theta and the damage Z and returns the
+// node's force-pass terms (alpha, theta (3K/m - alpha/3)) as FP32.  The
+// damage Z_i (demModel.cpp:770-773) is the plain maximum of |S| / s0 over
+// the list, broken bonds included: |S| is the break test's value, taken
+// here rather than in the force pass (the reference's running compare with
+// a 1e-5 relative slack keeps a value at most 1e-5 below it; FP32 bond
+// arithmetic is the larger difference).
 template <int DIM, int INF, bool LDS>
 __device__ __forceinline__ float2 dilation_node(const DS& s, const MixedArgs& m, const Win<DIM, LDS>& w,
                                                 const StateConsts<INF>& c, int q, int& node) {
@@ -302,6 +307,7 @@ __device__ __forceinline__ float2 dilation_node(const DS& s, const MixedArgs& m,
   const auto own = w.own(ip);
   const float us = w.us;
   double theta = 0.;
+  float Zs = 0.f;  // maximum of |S| over all bonds, broken ones included; Z = Zs / s0
   unsigned word = 0, wnew = 0;
   uint4 blk = n > 0 ? cb[0] : make_uint4(0, 0, 0, 0);
   for (int kb = 0; kb < n; kb += KB) {
@@ -334,7 +340,9 @@ __device__ __forceinline__ float2 dilation_node(const DS& s, const MixedArgs& m,
         const float cl = t2 * rcpf(sqrtf_hw(xx + t2) + r);
         const unsigned bit = 1u << (k & 31);
         bool fs = (word & bit) != 0u;
-        if (!fs && fabsf(cl) * inv_r > c.thr_rel) {
+        const float sa = fabsf(cl) * inv_r;  // |S|
+        Zs = fmaxf(Zs, sa);
+        if (!fs && sa > c.thr_rel) {
           fs = true;
           wnew |= bit;
         }
@@ -351,16 +359,14 @@ __device__ __forceinline__ float2 dilation_node(const DS& s, const MixedArgs& m,
   const double mi = s.mx[i];
   const double th = 3. * theta / mi;
   s.theta[i] = th;
+  s.Z[i] = Zs * c.inv_s0;
   const double alpha = 15. * c.G / mi;
   return make_float2((float)alpha, (float)(th * ((3. * c.K / mi) - alpha / 3.)));
 }
 
 // Force of the node in thread slot q (demModel.cpp:682-788,
 // PdState::getBondEF material.h:1415-1429) in mixed precision:
-// ef_i + ef_j = J (r (b_i + b_j) + cl (a_i + a_j)).  The damage Z_i is the
-// plain maximum of |S| / s0 over the list (the reference's running compare
-// with a 1e-5 relative slack keeps a value at most 1e-5 below it; FP32 bond
-// arithmetic is the larger difference).
+// ef_i + ef_j = J (r (b_i + b_j) + cl (a_i + a_j)).
 template <int DIM, int INF, bool LDS>
 __device__ __forceinline__ void force_node(const DS& s, const MixedArgs& m, const Win<DIM, LDS>& w,
                                            const StateConsts<INF>& c, int q) {
@@ -377,7 +383,6 @@ __device__ __forceinline__ void force_node(const DS& s, const MixedArgs& m, cons
   const float2 abi = w.ab(ip, own.b);
   const float ai = abi.x, bi = abi.y;
   double force[3] = {0., 0., 0.};
-  float Zs = 0.f;  // maximum of |S|; Z = Zs / s0
   unsigned word = 0;
   uint4 blk = n > 0 ? cb[0] : make_uint4(0, 0, 0, 0);
   for (int kb = 0; kb < n; kb += KB) {
@@ -421,7 +426,6 @@ __device__ __forceinline__ void force_node(const DS& s, const MixedArgs& m, cons
           if (sf > 0.f) sf = 0.f;
         }
         for (int d = 0; d < DIM; d++) fblk[d] += sf * y[d];
-        Zs = fmaxf(Zs, fabsf(cl) * inv_r);
       }
     };
 #pragma unroll
@@ -429,7 +433,6 @@ __device__ __forceinline__ void force_node(const DS& s, const MixedArgs& m, cons
     for (int d = 0; d < DIM; d++) force[d] += (double)fblk[d];
   }
   for (int d = 0; d < DIM; d++) s.f[d][i] = force[d];
-  s.Z[i] = Zs * c.inv_s0;
 }
 
 // Tile ranges of MixedArgs::tiles: closed LDS tiles (whole particles, fused
