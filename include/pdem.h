@@ -337,11 +337,12 @@ int pdem_get_tile_counts(pdem_ctx *ctx, int32_t *lds_tiles, int32_t *hbm_tiles);
  * results do not depend on it (both kernels sum a node's active pairs in
  * ascending node id). No counterpart in the reference (kernel selection). */
 int pdem_set_contact_light_max(pdem_ctx *ctx, int32_t light_max);
-/* Step fusion for multi-particle models of up to 2^20 nodes (on by default):
- * the node update also applies the displacement BC and packs the particle
- * centres, the controller runs as one workgroup and the damping pass
- * finalises the nodes, which saves launches where the step is launch-bound.
- * Results are the same bit for bit either way. No counterpart in the reference. */
+/* Step fusion for multi-particle models: the node update also applies the
+ * displacement BC and packs the particle centres, the controller runs as one
+ * workgroup and the damping pass finalises the nodes, which saves launches
+ * where the step is launch-bound.  on: 0 never, 1 (default) models of up to
+ * 2^20 nodes, 2 any size.  Results are the same bit for bit in every mode.
+ * No counterpart in the reference. */
 int pdem_set_step_fusion(pdem_ctx *ctx, int32_t on);
 /* Fused state-based passes (mixed precision, on by default): a particle
  * whose nodes all fit one LDS tile runs its dilation and force passes in one

@@ -551,7 +551,8 @@ class GpuEngine(Engine):
         return tuple(x.value for x in v)
 
     def set_step_fusion(self, on=True):
-        """Fused step kernels for small multi-particle models (default on)."""
+        """Fused step kernels: False never, True (default) for multi-particle
+        models of up to 2^20 nodes, 2 at any size (results unchanged)."""
         self._check(self.lib.pdem_set_step_fusion(self.ctx, int(on)), "set_step_fusion")
 
     def set_state_fusion(self, on=True):

@@ -118,7 +118,7 @@ struct pdem_ctx {
   // bond-based RNP pair-tile kernel (FP64, all force-computing particles RNP)
   RnpTileArgs rt{};
   bool rnp_tile = false;  // use it (set up by default, autotuned)
-  bool step_fusion = true;  // fused step kernels for small multi-particle models
+  int step_fusion = 1;  // fused step kernels: 0 never, 1 models up to FUSE_MAX_NODES, 2 any size
   bool state_fusion = true;  // closed state-based tiles run both passes in one kernel
   bool light_used = false;  // the lane-per-node contact list may hold nodes
   double* pp_out = nullptr;  // two-particle test results
@@ -399,7 +399,7 @@ struct Launch {
   // models small enough that the step is launch-bound; the unfused kernels
   // stream large models at full occupancy.
   static bool fuse_step(pdem_ctx* c) {
-    if (!c->multi || !c->step_fusion || c->N > FUSE_MAX_NODES) return false;
+    if (!c->multi || c->step_fusion == 0 || (c->step_fusion == 1 && c->N > FUSE_MAX_NODES)) return false;
     for (size_t k = 0; k < c->ubc.size(); k++)
       if (c->ubc_zero[k] && !c->zero_done[k]) return false;  // must apply in set order first
     return true;
@@ -501,6 +501,15 @@ int set_err(pdem_ctx* c, const std::string& m) {
 }
 
 void check_launch(pdem_ctx* c) { HIPCHK(hipGetLastError()); }
+
+// every node's candidate-position record from the node arrays (after the
+// positions were set from the host or restored)
+void refresh_xa(pdem_ctx* c) {
+  if (!c->ds.xa) return;
+  dispatch(
+      c, [&] { hipLaunchKernelGGL(k_xa_all<2>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds); },
+      [&] { hipLaunchKernelGGL(k_xa_all<3>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds); });
+}
 
 Ctrl read_ctrl(pdem_ctx* c) {
   Ctrl h;
@@ -1031,6 +1040,8 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     s.node_slot = c->dalloc<int>(N, false);
     s.cell_nodes = c->dalloc<int>(N, false);
     s.cell_xf = c->dalloc<float4>(N, false);
+    s.xa = c->dalloc<double4>(N, false);
+    s.xa_epoch = c->dalloc<uint8_t>(N);
     HIPCHK(hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming));
@@ -1143,6 +1154,7 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     const size_t nn = (size_t)std::max(1, s.n_nonwall);
     c->persist.push_back({s.dcount, nn * 4});
     c->persist.push_back({s.dlist, nn * s.dcap * 4});
+    c->persist.push_back({s.xa_epoch, (size_t)N});
   }
   // weighted volume for every node (computeStateMx), then the init-time
   // contact search (DEMModel::init calls updateContactNeighborlist once)
@@ -1157,6 +1169,7 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     h2.radius = d->search_factor * maxR;
     HIPCHK(hipMemcpyAsync(s.ctrl, &h2, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream));
     dispatch(c, [&] { Launch<2>::contact_search(c); }, [&] { Launch<3>::contact_search(c); });
+    refresh_xa(c);
     check_launch(c);
     check_overflow(c);
   }
@@ -1439,7 +1452,11 @@ int pdem_set_field(pdem_ctx* c, int32_t field, const void* host) {
       }
     };
     switch (field) {
-      case PDEM_FIELD_X: setp(s.x); break;
+      case PDEM_FIELD_X:
+        setp(s.x);
+        refresh_xa(c);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        break;
       case PDEM_FIELD_U: setp(s.u); break;
       case PDEM_FIELD_V: setp(s.v); break;
       case PDEM_FIELD_F: setp(s.f); break;
@@ -2070,6 +2087,8 @@ int pdem_checkpoint_load(pdem_ctx* c, const void* buf, int64_t bytes) {
       HIPCHK(hipMemcpy(q.first, o, q.second, hipMemcpyHostToDevice));
       o += q.second;
     }
+    refresh_xa(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
   });
 }
 
@@ -2200,8 +2219,9 @@ int pdem_get_tile_counts(pdem_ctx* c, int32_t* lds_tiles, int32_t* hbm_tiles) {
 
 int pdem_set_step_fusion(pdem_ctx* c, int32_t on) {
   return guarded(c, [&] {
+    if (on < 0 || on > 2) throw HipError{"step fusion mode must be 0, 1 or 2"};
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->step_fusion = on != 0;
+    c->step_fusion = on;
     drop_graph(c);
   });
 }

@@ -132,6 +132,19 @@ struct DS {
   int* cell_nodes;  // [N] node ids by cell
   float4* cell_xf;  // [N] beside cell_nodes: position - glo in FP32, particle id (bits)
   int* cell_owner;  // [ncells] particle id when one particle fills the cell, -1 mixed, -2 empty
+  // Candidate positions for the contact loops of the steps between rebuilds:
+  // one 32-byte record per node (x, y, z in FP64 and the particle id's bits
+  // in the fourth slot), so a candidate costs one gather instead of four
+  // arrays.  Every node update (k_integrate, k_ubc) rewrites, after moving
+  // them, the records of the nodes of particles that do not search (walls:
+  // any of their nodes may be a candidate) and of the nodes marked at the
+  // last rebuild (xa_epoch[i] equal to the low byte of the rebuild count: the
+  // query marks each searching node that has candidates, which by the
+  // symmetry of the FP64 distance test are the searching nodes that are
+  // candidates of another).  On a rebuild step the contact loops read the
+  // node arrays.
+  double4* xa;        // [N]
+  uint8_t* xa_epoch;  // [N]
   // damping
   int n_nonwall;
   const int* nonwall_pid;
@@ -507,6 +520,24 @@ __device__ __forceinline__ void apply_ubc(const DS& s, int time_fn, int dir_mask
     }
 }
 
+// the candidate-position record of node i from the node arrays
+template <int DIM>
+__device__ __forceinline__ void xa_store(const DS& s, int64_t i, int pid) {
+  s.xa[i] = make_double4(s.x[0][i], s.x[1][i], DIM == 3 ? s.x[2][i] : 0., __longlong_as_double((long long)pid));
+}
+// after a node update: rewrite the record of a wall node or of a node marked
+// at the last rebuild
+template <int DIM>
+__device__ __forceinline__ void xa_refresh(const DS& s, int64_t i, int pid, bool searches) {
+  if (s.xa && (!searches || s.xa_epoch[i] == (uint8_t)s.ctrl->n_rebuilds)) xa_store<DIM>(s, i, pid);
+}
+// every node's record (after positions are set from the host or restored)
+template <int DIM>
+__global__ void k_xa_all(DS s) {
+  const int64_t i = gid();
+  if (i < s.N) xa_store<DIM>(s, i, s.ptid[i]);
+}
+
 // ------------------------------------------------------------ node update
 // integrateCD (demModel.cpp:349-390): semi-implicit Euler on free dofs;
 // with `half`, the first half kick and drift of integrateVerlet
@@ -526,26 +557,42 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
     const int pid = s.ptid[i];
     const DevParticle& p = s.P[pid];
     const uint8_t fx = s.fix[i];
+    // the node's candidate-position record is rewritten below (read before
+    // any store: a store could alias the flag)
+    const bool rec = multi && s.xa && (!p.searches || s.xa_epoch[i] == (uint8_t)ct->n_rebuilds);
+    double xs[3] = {0., 0., 0.};
+    bool xs_ok = false;
     if (p.compute_force) {
       const double rho = p.rho;
       const double a = half ? 0.5 * (dt / rho) : dt / rho;
-      double vv[3] = {0., 0., 0.};
+      // every input is loaded before the first store: the stores go through
+      // double pointers that may alias the arrays read, so a load after a
+      // store would wait for the stores to complete
+      double vd[3] = {0., 0., 0.}, fd[3] = {0., 0., 0.}, ud[3] = {0., 0., 0.};
       for (int d = 0; d < DIM; d++) {
-        double vd = s.v[d][i];
+        vd[d] = s.v[d][i];
+        xs[d] = s.x[d][i];
         if (!(fx >> d & 1)) {
-          vd += a * s.f[d][i];
-          s.v[d][i] = vd;
-          s.u[d][i] += dt * vd;
-          s.x[d][i] += dt * vd;
+          fd[d] = s.f[d][i];
+          ud[d] = s.u[d][i];
         }
-        vv[d] = vd;
       }
-      vm = len3<DIM>(vv);
+      for (int d = 0; d < DIM; d++)
+        if (!(fx >> d & 1)) {
+          vd[d] += a * fd[d];
+          s.v[d][i] = vd[d];
+          s.u[d][i] = ud[d] + dt * vd[d];
+          xs[d] = xs[d] + dt * vd[d];
+          s.x[d][i] = xs[d];
+        }
+      xs_ok = true;
+      vm = len3<DIM>(vd);
       s.vmag[i] = vm;
     }
     if (fused) {
       const double t_new = ct->time + dt;
       const int e1 = s.bc_ptr[i + 1];
+      if (e1 > s.bc_ptr[i]) xs_ok = false;  // moved by the displacement BC below
       for (int e = s.bc_ptr[i]; e < e1; e++) {
         const BcEntry be = s.bc_ent[e];
         const BcSetParams& bs = s.bc_sets[be.set];
@@ -559,6 +606,11 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
         }
         if (s.cl_on) s.pflags[q] = 0;
       }
+    }
+    if (rec) {
+      if (!xs_ok)
+        for (int d = 0; d < DIM; d++) xs[d] = s.x[d][i];
+      s.xa[i] = make_double4(xs[0], xs[1], xs[2], __longlong_as_double((long long)pid));
     }
   }
   if (!multi) {
@@ -628,7 +680,12 @@ template <int DIM>
 __global__ void k_ubc(DS s, DevBc bc) {
   int64_t k = gid();
   if (k >= bc.n) return;
-  apply_ubc<DIM>(s, bc.time_fn, bc.dir_mask, bc.tp, bc.amp[k], bc.nodes[k], s.ctrl->time);
+  const int64_t i = bc.nodes[k];
+  apply_ubc<DIM>(s, bc.time_fn, bc.dir_mask, bc.tp, bc.amp[k], i, s.ctrl->time);
+  if (s.xa) {
+    const int pid = s.ptid[i];
+    xa_refresh<DIM>(s, i, pid, s.P[pid].searches != 0);
+  }
 }
 
 // ParticleFLoading::apply (particleFLoading.cpp:85-173)
@@ -1105,7 +1162,7 @@ __global__ void __launch_bounds__(CELL_TPB) k_cells_scatter(DS s) {
 // node i always passes, with a factor of four to spare.  Entries that pass
 // take the FP64 test below, which alone decides the candidate set.
 template <int DIM>
-__device__ void query_node(const DS& s, int64_t i, int pi) {
+__device__ int query_node(const DS& s, int64_t i, int pi) {
   const double r = s.ctrl->radius, r2 = r * r;
   double xi[3] = {0., 0., 0.};
   for (int d = 0; d < DIM; d++) xi[d] = s.x[d][i];
@@ -1170,6 +1227,7 @@ __device__ void query_node(const DS& s, int64_t i, int pi) {
     cnt = cap;
   }
   s.ccount[i] = cnt;
+  return cnt;
 }
 
 // one active contact pair's two force terms on node i (demModel.cpp:868-904):
@@ -1244,8 +1302,8 @@ __device__ __forceinline__ void query_one(const DS& s, int64_t i) {
   const int pid = s.ptid[i];
   const DevParticle& pi = s.P[pid];
   if (!pi.searches) return;
-  query_node<DIM>(s, i, pid);
-  const int cnt = s.ccount[i];
+  const uint8_t ep = (uint8_t)s.ctrl->n_rebuilds;
+  const int cnt = query_node<DIM>(s, i, pid);
   if (s.cl_on && pi.nonwall_index >= 0)
     for (int k = 0; k < cnt; k++)
       if (s.pmeta[s.ptid[s.cand[(int64_t)k * s.N + i]]] >> 16) {
@@ -1259,6 +1317,27 @@ __device__ __forceinline__ void query_one(const DS& s, int64_t i) {
   const int pa = wave_append(&s.ctrl->nactive, ac);
   if (lt) s.light[pl] = (int)i;
   if (ac) s.active[pa] = (int)i;
+  // node i is a candidate of each of its searching candidates (the FP64 test
+  // is symmetric): its record is kept current until the next rebuild.  The
+  // byte store comes last: a store through uint8_t may alias any earlier
+  // load, which would otherwise have to be repeated after it.
+  if (cnt > 0) s.xa_epoch[i] = ep;
+}
+
+// candidate j seen from node position yi: yji = x_j - yi, and j's particle id
+// when the position came from j's record (between rebuilds), else -1
+template <int DIM>
+__device__ __forceinline__ void cand_pos(const DS& s, bool rec, int j, const double* yi, double* yji, int& pj) {
+  if (rec) {
+    const double4 a = s.xa[j];
+    yji[0] = a.x - yi[0];
+    yji[1] = a.y - yi[1];
+    if (DIM == 3) yji[2] = a.z - yi[2];
+    pj = (int)__double_as_longlong(a.w);
+  } else {
+    for (int d = 0; d < DIM; d++) yji[d] = s.x[d][j] - yi[d];
+    pj = -1;
+  }
 }
 
 // computeContactForces node loop for nodes with few candidates: one lane per
@@ -1269,6 +1348,7 @@ constexpr int NACT = LIGHT_MAX_C;  // every candidate of a light node fits
 template <int DIM>
 __global__ void __launch_bounds__(256) k_contact_light(DS s) {
   const int nl = (int)s.ctrl->nlight;
+  const bool rec = !s.ctrl->rebuild;
   for (int a = (int)gid(); a < nl; a += gridDim.x * blockDim.x) {
     const int i = s.light[a];
     const int n = s.ccount[i];
@@ -1294,7 +1374,8 @@ __global__ void __launch_bounds__(256) k_contact_light(DS s) {
     for (int k = 0; k < n; k++) {
       const int j = s.cand[(int64_t)k * s.N + i];
       double yji[3];
-      for (int d = 0; d < DIM; d++) yji[d] = s.x[d][j] - yi[d];
+      int pj;
+      cand_pos<DIM>(s, rec, j, yi, yji, pj);
       if (dot3<DIM>(yji, yji) < s.rc2_max) pass |= 1u << k;
     }
     while (pass) {
@@ -1302,9 +1383,10 @@ __global__ void __launch_bounds__(256) k_contact_light(DS s) {
       pass &= pass - 1u;
       const int j = s.cand[(int64_t)k * s.N + i];
       double yji[3];
-      for (int d = 0; d < DIM; d++) yji[d] = s.x[d][j] - yi[d];
+      int pjid;
+      cand_pos<DIM>(s, rec, j, yi, yji, pjid);
       const double d2 = dot3<DIM>(yji, yji);
-      const int pjid = s.ptid[j];
+      if (pjid < 0) pjid = s.ptid[j];
       const int mj = s.pmeta[pjid];
       const bool wj = (mj >> 16) != 0;
       if (j == i || pjid == pid || (wi && wj)) continue;
@@ -1336,10 +1418,11 @@ __global__ void __launch_bounds__(256) k_contact_light(DS s) {
 #pragma unroll
       for (int q = 0; q < NACT; q++)
         if (q == r) j = act[q];
-      const int mj = s.pmeta[s.ptid[j]];
-      const DevContact ct = s.CT[zi * s.nz + (mj & 0xffff)];
       double yji[3], fn[3], ft[3];
-      for (int d = 0; d < DIM; d++) yji[d] = s.x[d][j] - yi[d];
+      int pj;
+      cand_pos<DIM>(s, rec, j, yi, yji, pj);
+      const int mj = s.pmeta[pj >= 0 ? pj : s.ptid[j]];
+      const DevContact ct = s.CT[zi * s.nz + (mj & 0xffff)];
       contact_pair<DIM>(s, vi, j, ct, len3<DIM>(yji), yji, fn, ft);
       for (int d = 0; d < DIM; d++) force[d] += fn[d];
       for (int d = 0; d < DIM; d++) force[d] += ft[d];
@@ -1366,6 +1449,7 @@ __global__ void __launch_bounds__(CWAVES * WAVE) k_contact_force(DS s) {
   const int wv = threadIdx.x / WAVE;
   const int nwaves = gridDim.x * CWAVES;
   const int nact_nodes = (int)s.ctrl->nactive;
+  const bool rec = !s.ctrl->rebuild;
   for (int a = blockIdx.x * CWAVES + wv; a < nact_nodes; a += nwaves) {
     const int i = s.active[a];
     const int n = s.ccount[i];
@@ -1390,9 +1474,10 @@ __global__ void __launch_bounds__(CWAVES * WAVE) k_contact_force(DS s) {
         j = s.cand[(int64_t)k * s.N + i];
         // distance prefilter (largest contact radius) before any particle data is loaded
         double yji[3];
-        for (int d = 0; d < DIM; d++) yji[d] = s.x[d][j] - yi[d];
+        int pj;
+        cand_pos<DIM>(s, rec, j, yi, yji, pj);
         const double d2 = dot3<DIM>(yji, yji);
-        const int pjid = d2 < s.rc2_max ? s.ptid[j] : pid;
+        const int pjid = d2 < s.rc2_max ? (pj >= 0 ? pj : s.ptid[j]) : pid;
         const int mj = s.pmeta[pjid];
         const bool wj = (mj >> 16) != 0;
         if (!(j == i || pjid == pid || (wi && wj))) {
