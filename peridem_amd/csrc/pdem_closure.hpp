@@ -27,9 +27,11 @@
 // the normalised difference.  k_pair_log writes the same features and the
 // plain targets for training (pairs within a band around contact).
 //
-// Lanes: a wavefront takes 16 pairs; the 4 lanes l, l+16, l+32, l+48 of a
-// pair split its node loops and meet through shuffles, and hold the 4
-// feature quarters the MFMA input fragment needs.
+// Lanes: one wavefront per pair: the 64 lanes split its node loops (b's
+// contact patch staged in LDS) and meet through shuffles; every lane ends
+// with the pair's features, so the 16 rows of the network's MFMA fragment
+// are copies of the pair.  Only pairs whose particles had a candidate node
+// pair at the last rebuild are evaluated.
 #pragma once
 #include "pdem_kernels.hpp"
 #include "pdem_mlp.hpp"
@@ -52,67 +54,49 @@ struct ClosureArgs {
 constexpr int CL_NFEAT = 16;
 constexpr double CL_V0 = 1.0;  // velocity scale of the features, m/s
 
-__device__ __forceinline__ double sub_max(double v) {
-  v = fmax(v, __shfl_xor(v, 16));
-  return fmax(v, __shfl_xor(v, 32));
+// reductions over the 64 lanes of a wavefront (fixed butterfly order)
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = WAVE / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
 }
-__device__ __forceinline__ double sub_sum(double v) {
-  v += __shfl_xor(v, 16);
-  return v + __shfl_xor(v, 32);
-}
-
-// Per non-wall particle: largest node distance from the centre node and
-// largest Z, one wavefront per particle (coalesced node loops).
-__global__ void __launch_bounds__(256) k_particle_radius(DS s) {
-  const int lane = threadIdx.x & 63;
-  const int nwaves = gridDim.x * 4;
-  for (int q = blockIdx.x * 4 + (threadIdx.x >> 6); q < s.n_nonwall; q += nwaves) {
-    const DevParticle& P = s.P[s.nonwall_pid[q]];
-    const double c0 = s.x[0][P.center], c1 = s.x[1][P.center];
-    double r2 = 0.;
-    float z = 0.f;
-    for (int64_t i = P.glob_start + lane; i < P.glob_end; i += 64) {
-      const double d0 = s.x[0][i] - c0, d1 = s.x[1][i] - c1;
-      r2 = fmax(r2, d0 * d0 + d1 * d1);
-      z = fmaxf(z, s.Z[i]);
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      r2 = fmax(r2, __shfl_xor(r2, off));
-      z = fmaxf(z, __shfl_xor(z, off));
-    }
-    if (lane == 0) {
-      s.prmax[q] = sqrt(r2);
-      s.pzmax[q] = z;
-    }
-  }
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
 }
 
 struct PairGeo {
   int a, b, pa, pb, sa;
   double ca[2], cb[2], n[2], d;
   double ext_a, ext_b, gap, rc, ha, hb;
+  float za, zb;  // largest damage Z of each particle (this step)
   DevContact ct;
 };
 
 // largest (x_i - c).dir over the surface nodes of non-wall particle q
-__device__ __forceinline__ double extent(const DS& s, int q, const double* c, const double* dir, int sub,
-                                         bool on) {
+__device__ __forceinline__ double extent(const DS& s, int q, const double* c, const double* dir, int lane) {
   double m = -1e300;
-  if (on)
-#pragma unroll 4
-    for (int k = s.surf_ptr[q] + sub; k < s.surf_ptr[q + 1]; k += 4) {
-      const int i = s.surf_idx[k];
-      const double e = (s.x[0][i] - c[0]) * dir[0] + (s.x[1][i] - c[1]) * dir[1];
-      m = fmax(m, e);
-    }
-  return sub_max(m);
+  for (int k = s.surf_ptr[q] + lane; k < s.surf_ptr[q + 1]; k += WAVE) {
+    const int i = s.surf_idx[k];
+    m = fmax(m, (s.x[0][i] - c[0]) * dir[0] + (s.x[1][i] - c[1]) * dir[1]);
+  }
+  return wave_max(m);
 }
 
-__device__ __forceinline__ void pair_geometry(const DS& s, int k, bool valid, int sub, PairGeo& g) {
+// largest Z over the nodes of particle P
+__device__ __forceinline__ float zmax(const DS& s, const DevParticle& P, int lane) {
+  float z = 0.f;
+  for (int64_t i = P.glob_start + lane; i < P.glob_end; i += WAVE) z = fmaxf(z, s.Z[i]);
+  for (int o = WAVE / 2; o > 0; o >>= 1) z = fmaxf(z, __shfl_xor(z, o));
+  return z;
+}
+
+// Geometry of listed pair k in the pair frame; every lane ends with the same
+// values.  Every node pair (i in a, j in b) is at least gap apart.
+__device__ __forceinline__ void pair_geometry(const DS& s, int k, int lane, PairGeo& g) {
   const int nn = s.n_nonwall;
-  g.a = valid ? s.cpairs[2 * (int64_t)k] : 0;
-  g.sa = valid ? s.cpairs[2 * (int64_t)k + 1] : 0;
-  g.b = valid ? s.dlist[(int64_t)g.a * s.dcap + g.sa] : 0;
+  g.a = s.cpairs[2 * (int64_t)k];
+  g.sa = s.cpairs[2 * (int64_t)k + 1];
+  g.b = s.dlist[(int64_t)g.a * s.dcap + g.sa];
   g.pa = s.nonwall_pid[g.a];
   g.pb = s.nonwall_pid[g.b];
   for (int d = 0; d < 2; d++) {
@@ -121,24 +105,21 @@ __device__ __forceinline__ void pair_geometry(const DS& s, int k, bool valid, in
   }
   const double dx = g.cb[0] - g.ca[0], dy = g.cb[1] - g.ca[1];
   g.d = sqrt(dx * dx + dy * dy);
-  const bool ok = valid && g.d > 0.;
+  const bool ok = g.d > 0.;
   g.n[0] = ok ? dx / g.d : 1.;
   g.n[1] = ok ? dy / g.d : 0.;
   const double nb[2] = {-g.n[0], -g.n[1]};
   g.ct = s.CT[s.pzone[g.a] * s.nz + s.pzone[g.b]];
   g.rc = g.ct.contact_r;
-  g.ha = s.P[g.pa].h;
-  g.hb = s.P[g.pb].h;
-  // every node lies within prmax of its centre node (this step's positions):
-  // d - prmax_a - prmax_b >= Rc means no node pair within Rc
-  const bool far = ok && g.d - s.prmax[g.a] - s.prmax[g.b] >= g.rc;
-  g.ext_a = extent(s, g.a, g.ca, g.n, sub, ok && !far);
-  g.ext_b = extent(s, g.b, g.cb, nb, sub, ok && !far);
-  if (far) {
-    g.ext_a = s.prmax[g.a];
-    g.ext_b = s.prmax[g.b];
-  }
+  const DevParticle& Pa = s.P[g.pa];
+  const DevParticle& Pb = s.P[g.pb];
+  g.ha = Pa.h;
+  g.hb = Pb.h;
+  g.ext_a = extent(s, g.a, g.ca, g.n, lane);
+  g.ext_b = extent(s, g.b, g.cb, nb, lane);
   g.gap = ok ? g.d - g.ext_a - g.ext_b : -1e300;  // coincident centres: plain
+  g.za = zmax(s, Pa, lane);
+  g.zb = zmax(s, Pb, lane);
 }
 
 __device__ __forceinline__ double omega(const DS& s, const ClosureArgs& ca, int q, const double* c) {
@@ -166,54 +147,82 @@ struct PatchSums {
   double dmin, W, Wn, Wt, Wm, sa[2], saa, sb[2], sbb;
 };
 
-__device__ __forceinline__ void patch_contact(const DS& s, const PairGeo& g, int sub, bool on, PatchSums& P) {
+// b's patch nodes staged per wavefront (position and volume); a pair with a
+// larger patch is left to the plain method
+constexpr int CL_PATCH = 128;
+struct PatchStage {
+  double x[CL_PATCH], y[CL_PATCH], v[CL_PATCH];
+};
+
+// Patch sums of the pair with b's patch staged in `st`; false when b's patch
+// does not fit (the caller then treats the pair as plain)
+__device__ __forceinline__ bool patch_contact(const DS& s, const PairGeo& g, int lane, PatchStage& st,
+                                              PatchSums& P) {
+  const double thr_a = g.ext_a - g.rc, thr_b = g.ext_b - g.rc;
+  int nb = 0;
+  for (int m = s.surf_ptr[g.b] + lane; m - lane < s.surf_ptr[g.b + 1]; m += WAVE) {
+    bool in = false;
+    double xj0 = 0., xj1 = 0.;
+    int j = 0;
+    if (m < s.surf_ptr[g.b + 1]) {
+      j = s.surf_idx[m];
+      xj0 = s.x[0][j];
+      xj1 = s.x[1][j];
+      in = -((xj0 - g.cb[0]) * g.n[0] + (xj1 - g.cb[1]) * g.n[1]) >= thr_b;
+    }
+    const unsigned long long msk = __ballot(in);
+    const int pos = nb + __popcll(msk & ((1ull << lane) - 1ull));
+    if (in && pos < CL_PATCH) {
+      st.x[pos] = xj0;
+      st.y[pos] = xj1;
+      st.v[pos] = s.vol[j];
+    }
+    nb += __popcll(msk);
+  }
+  if (nb > CL_PATCH) return false;
+  __builtin_amdgcn_wave_barrier();  // LDS writes of the wavefront before its reads
   double dm = 1e300, w = 0., wn = 0., wt = 0., wm = 0., a0 = 0., a1 = 0., aa = 0., b0s = 0., b1s = 0., bb = 0.;
-  if (on) {
-    const double thr_a = g.ext_a - g.rc, thr_b = g.ext_b - g.rc;
-    const int b0 = s.surf_ptr[g.b], b1 = s.surf_ptr[g.b + 1];
-    for (int k = s.surf_ptr[g.a] + sub; k < s.surf_ptr[g.a + 1]; k += 4) {
-      const int i = s.surf_idx[k];
-      const double xi0 = s.x[0][i], xi1 = s.x[1][i];
-      const double ri0 = xi0 - g.ca[0], ri1 = xi1 - g.ca[1];
-      if (ri0 * g.n[0] + ri1 * g.n[1] < thr_a) continue;
-      const double Vi = s.vol[i];
-#pragma unroll 4
-      for (int m = b0; m < b1; m++) {
-        const int j = s.surf_idx[m];
-        const double xj0 = s.x[0][j], xj1 = s.x[1][j];
-        const double rj0 = xj0 - g.cb[0], rj1 = xj1 - g.cb[1];
-        if (-(rj0 * g.n[0] + rj1 * g.n[1]) < thr_b) continue;
-        const double dx = xj0 - xi0, dy = xj1 - xi1;
-        const double R = sqrt(dx * dx + dy * dy);
-        dm = fmin(dm, R);
-        if (R < g.rc) {
-          const double c = (1. - R / g.rc) * Vi * s.vol[j];
-          w += c;
-          // direction of the node pair (i to j) in the pair frame
-          wn += c * (dx * g.n[0] + dy * g.n[1]) / R;
-          wt += c * (-dx * g.n[1] + dy * g.n[0]) / R;
-          wm += c * (ri0 * dy - ri1 * dx) / R;
-          a0 += c * ri0;
-          a1 += c * ri1;
-          aa += c * (ri0 * ri0 + ri1 * ri1);
-          b0s += c * rj0;
-          b1s += c * rj1;
-          bb += c * (rj0 * rj0 + rj1 * rj1);
-        }
+  for (int k = s.surf_ptr[g.a] + lane; k < s.surf_ptr[g.a + 1]; k += WAVE) {
+    const int i = s.surf_idx[k];
+    const double xi0 = s.x[0][i], xi1 = s.x[1][i];
+    const double ri0 = xi0 - g.ca[0], ri1 = xi1 - g.ca[1];
+    if (ri0 * g.n[0] + ri1 * g.n[1] < thr_a) continue;
+    const double Vi = s.vol[i];
+    for (int m = 0; m < nb; m++) {
+      const double xj0 = st.x[m], xj1 = st.y[m];
+      const double rj0 = xj0 - g.cb[0], rj1 = xj1 - g.cb[1];
+      const double dx = xj0 - xi0, dy = xj1 - xi1;
+      const double R = sqrt(dx * dx + dy * dy);
+      dm = fmin(dm, R);
+      if (R < g.rc) {
+        const double c = (1. - R / g.rc) * Vi * st.v[m];
+        w += c;
+        // direction of the node pair (i to j) in the pair frame
+        wn += c * (dx * g.n[0] + dy * g.n[1]) / R;
+        wt += c * (-dx * g.n[1] + dy * g.n[0]) / R;
+        wm += c * (ri0 * dy - ri1 * dx) / R;
+        a0 += c * ri0;
+        a1 += c * ri1;
+        aa += c * (ri0 * ri0 + ri1 * ri1);
+        b0s += c * rj0;
+        b1s += c * rj1;
+        bb += c * (rj0 * rj0 + rj1 * rj1);
       }
     }
   }
-  P.dmin = -sub_max(-dm);
-  P.W = sub_sum(w);
-  P.Wn = sub_sum(wn);
-  P.Wt = sub_sum(wt);
-  P.Wm = sub_sum(wm);
-  P.sa[0] = sub_sum(a0);
-  P.sa[1] = sub_sum(a1);
-  P.saa = sub_sum(aa);
-  P.sb[0] = sub_sum(b0s);
-  P.sb[1] = sub_sum(b1s);
-  P.sbb = sub_sum(bb);
+  __builtin_amdgcn_wave_barrier();  // the stage is reused by the wavefront's next pair
+  P.dmin = -wave_max(-dm);
+  P.W = wave_sum(w);
+  P.Wn = wave_sum(wn);
+  P.Wt = wave_sum(wt);
+  P.Wm = wave_sum(wm);
+  P.sa[0] = wave_sum(a0);
+  P.sa[1] = wave_sum(a1);
+  P.saa = wave_sum(aa);
+  P.sb[0] = wave_sum(b0s);
+  P.sb[1] = wave_sum(b1s);
+  P.sbb = wave_sum(bb);
+  return true;
 }
 
 __device__ __forceinline__ void pair_features(const DS& s, const ClosureArgs& ca, const PairGeo& g, double dmin,
@@ -260,45 +269,43 @@ __device__ __forceinline__ double pair_unit(const ClosureArgs& ca, const PairGeo
 // b: the node-level terms of computeContactForces (demModel.cpp:868-904)
 // summed with the nodal volumes, from the candidate lists; zloc = largest Z
 // of a node of a with an active contact with b.
-__device__ __forceinline__ void plain_pair(const DS& s, const PairGeo& g, int sub, bool on, double* F, double& M,
+__device__ __forceinline__ void plain_pair(const DS& s, const PairGeo& g, int lane, double* F, double& M,
                                            double& zloc) {
   double f0 = 0., f1 = 0., m = 0., z = 0.;
-  if (on) {
-    const DevParticle& P = s.P[g.pa];
-    const DevContact ct = g.ct;
-    for (int64_t i = P.glob_start + sub; i < P.glob_end; i += 4) {
-      const int nc = s.ccount[i];
-      if (nc == 0) continue;
-      double yi[2] = {s.x[0][i], s.x[1][i]}, vi[2] = {s.v[0][i], s.v[1][i]};
-      double fi[2] = {0., 0.};
-      bool any = false;
-      for (int k = 0; k < nc; k++) {
-        const int j = s.cand[(int64_t)k * s.N + i];
-        if (s.ptid[j] != g.pb) continue;
-        double yji[2] = {s.x[0][j] - yi[0], s.x[1][j] - yi[1]};
-        const double R = len3<2>(yji);
-        if (!isLess(R, ct.contact_r)) continue;
-        double fn[2], ft[2];
-        contact_pair<2>(s, vi, j, ct, R, yji, fn, ft);
-        fi[0] += fn[0];
-        fi[1] += fn[1];
-        fi[0] += ft[0];
-        fi[1] += ft[1];
-        any = true;
-      }
-      if (any) {
-        const double V = s.vol[i];
-        f0 += V * fi[0];
-        f1 += V * fi[1];
-        m += V * ((yi[0] - g.ca[0]) * fi[1] - (yi[1] - g.ca[1]) * fi[0]);
-        z = fmax(z, (double)s.Z[i]);
-      }
+  const DevParticle& P = s.P[g.pa];
+  const DevContact ct = g.ct;
+  for (int64_t i = P.glob_start + lane; i < P.glob_end; i += WAVE) {
+    const int nc = s.ccount[i];
+    if (nc == 0) continue;
+    double yi[2] = {s.x[0][i], s.x[1][i]}, vi[2] = {s.v[0][i], s.v[1][i]};
+    double fi[2] = {0., 0.};
+    bool any = false;
+    for (int k = 0; k < nc; k++) {
+      const int j = s.cand[(int64_t)k * s.N + i];
+      if (s.ptid[j] != g.pb) continue;
+      double yji[2] = {s.x[0][j] - yi[0], s.x[1][j] - yi[1]};
+      const double R = len3<2>(yji);
+      if (!isLess(R, ct.contact_r)) continue;
+      double fn[2], ft[2];
+      contact_pair<2>(s, vi, j, ct, R, yji, fn, ft);
+      fi[0] += fn[0];
+      fi[1] += fn[1];
+      fi[0] += ft[0];
+      fi[1] += ft[1];
+      any = true;
+    }
+    if (any) {
+      const double V = s.vol[i];
+      f0 += V * fi[0];
+      f1 += V * fi[1];
+      m += V * ((yi[0] - g.ca[0]) * fi[1] - (yi[1] - g.ca[1]) * fi[0]);
+      z = fmax(z, (double)s.Z[i]);
     }
   }
-  F[0] = sub_sum(f0);
-  F[1] = sub_sum(f1);
-  M = sub_sum(m);
-  zloc = sub_max(z);
+  F[0] = wave_sum(f0);
+  F[1] = wave_sum(f1);
+  M = wave_sum(m);
+  zloc = wave_max(z);
 }
 
 // Record of one side of a closure pair (CL_REC doubles): the pair direction
@@ -337,51 +344,64 @@ __device__ __forceinline__ void closure_record(double* rec, const double* dir, d
   rec[9] = rc;
 }
 
-__device__ __forceinline__ int find_slot(const DS& s, int q, int target) {
+// slot of particle `target` in q's partner list, or -1 (all lanes)
+__device__ __forceinline__ int find_slot(const DS& s, int q, int target, int lane) {
   const int* dl = s.dlist + (int64_t)q * s.dcap;
-  int lo = 0, hi = s.dcount[q] - 1;
-  while (lo <= hi) {
-    const int mid = (lo + hi) >> 1;
-    if (dl[mid] == target) return mid;
-    if (dl[mid] < target)
-      lo = mid + 1;
-    else
-      hi = mid - 1;
+  const int n = s.dcount[q];
+  for (int base = 0; base < n; base += WAVE) {
+    const unsigned long long m = __ballot(base + lane < n && dl[base + lane] == target);
+    if (m) return base + __ffsll((long long)m) - 1;
   }
   return -1;
 }
 
-__global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, sub = lane >> 4, pl = lane & 15;
+// One wavefront per listed pair (grid-stride).  A pair whose particles had no
+// candidate node pair at the last rebuild has no contact force until the next
+// one (the plain method only evaluates candidates): mode 0 without geometry
+// (its cmode entries were cleared at that rebuild).
+constexpr int CL_WAVES = 4;  // wavefronts per workgroup of the pair kernels
+__global__ void __launch_bounds__(CL_WAVES * WAVE) k_closure(DS s, ClosureArgs ca) {
+  __shared__ PatchStage stage[CL_WAVES];
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
   const int npairs = min((int)s.ctrl->npairs, s.cpair_cap);
+  const int ep = (int)s.ctrl->n_rebuilds;
   const bool check = ca.check_every > 0 && (s.ctrl->n % ca.check_every) == 0;
-  for (int base = (blockIdx.x * 4 + wave) * 16; base < npairs; base += gridDim.x * 64) {
-    const int k = base + pl;
-    const bool valid = k < npairs;
+  for (int k = blockIdx.x * CL_WAVES + wave; k < npairs; k += gridDim.x * CL_WAVES) {
+    const int a0 = s.cpairs[2 * (int64_t)k];
+    const int64_t ia = (int64_t)a0 * s.dcap + s.cpairs[2 * (int64_t)k + 1];
+    if (s.cconn[ia] != ep) {
+      if (lane == 0) atomicAdd(&ca.stats[0], 1ull);
+      continue;
+    }
     PairGeo g;
-    pair_geometry(s, k, valid, sub, g);
-    bool near = valid && g.gap < g.rc;
+    pair_geometry(s, k, lane, g);
+    bool near = g.gap < g.rc;
+    PatchSums P{};
+    bool fits = true;
     // patches: nearest node pair (exact no-contact test), overlap weights
-    PatchSums P;
-    patch_contact(s, g, sub, near && g.gap >= 0., P);
-    if (near && g.gap >= 0. && P.dmin >= g.rc) near = false;  // no node pair within Rc
+    if (near && g.gap >= 0.) {
+      fits = patch_contact(s, g, lane, stage[wave], P);
+      if (fits && P.dmin >= g.rc) near = false;  // no node pair within Rc
+    }
     const double dmin = (near && g.gap >= 0.) ? P.dmin : g.gap;
-    const double za = s.pzmax[g.a], zb = s.pzmax[g.b];
     int mode = 0;
     if (near)
-      mode = (ca.use_net && g.gap >= 0. && g.rc - dmin <= ca.pen_max_h * fmin(g.ha, g.hb) && za < 1. && zb < 1.)
+      mode = (ca.use_net && fits && g.gap >= 0. && g.rc - dmin <= ca.pen_max_h * fmin(g.ha, g.hb) &&
+              g.za < 1.f && g.zb < 1.f)
                  ? 1
                  : 2;
     double Fa[2] = {0., 0.}, Ma = 0.;
     const double unit = pair_unit(ca, g);
-    if (ca.use_net && __ballot(mode == 1)) {
+    if (mode == 1) {
+      // every lane holds the pair's features: the 16 rows of the network's
+      // fragment are copies of the pair, row 0 (lane 0) is read
       float x16[CL_NFEAT];
       pair_features(s, ca, g, dmin, P, x16);
       mlp::s4 h1[8], h2[8], h3[8];
       const mlp::f4 y = mlp::forward(ca.net, mlp::input_frag(ca.net, x16, lane), h1, h2, h3, lane);
       double ys[4];
-      for (int r = 0; r < 4; r++) ys[r] = (double)(__shfl(y[r], pl) / ca.net.out_scale[r]);
-      if (mode == 1 && ys[3] > (double)ca.dz_max) mode = 2;
+      for (int r = 0; r < 4; r++) ys[r] = (double)(__shfl(y[r], 0) / ca.net.out_scale[r]);
+      if (ys[3] > (double)ca.dz_max) mode = 2;
       double b3[3];
       pair_baseline(ca, g, P, b3);
       const double Fn = (ys[0] + b3[0]) * unit, Ft = (ys[1] + b3[1]) * unit;
@@ -389,10 +409,8 @@ __global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
       Fa[1] = Fn * g.n[1] + Ft * g.n[0];
       Ma = (ys[2] + b3[2]) * unit * g.ha;
     }
-    const double nb[2] = {-g.n[0], -g.n[1]};
-    const int sb = valid ? find_slot(s, g.b, g.a) : -1;
-    if (valid && sub == 0) {
-      const int64_t ia = (int64_t)g.a * s.dcap + g.sa;
+    const int sb = find_slot(s, g.b, g.a, lane);
+    if (lane == 0) {
       s.cmode[ia] = (int8_t)mode;
       if (sb >= 0) s.cmode[(int64_t)g.b * s.dcap + sb] = (int8_t)mode;
       if (mode == 2 || (mode == 1 && sb >= 0)) {
@@ -401,6 +419,7 @@ __global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
         atomicOr(&s.pflags[g.b], bit);
       }
       if (mode == 1 && sb >= 0) {
+        const double nb[2] = {-g.n[0], -g.n[1]};
         const double Fb[2] = {-Fa[0], -Fa[1]};
         // torque balance: M_b = -M_a + (c_b - c_a) x F_a
         const double Mb = -Ma + ((g.cb[0] - g.ca[0]) * Fa[1] - (g.cb[1] - g.ca[1]) * Fa[0]);
@@ -412,10 +431,10 @@ __global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
       if (sb < 0) s.ctrl->overflow = 6;
       atomicAdd(&ca.stats[mode], 1ull);
     }
-    if (check) {
+    if (check && mode == 1) {
       double Fp[2], Mp, zl;
-      plain_pair(s, g, sub, valid && mode == 1, Fp, Mp, zl);
-      if (valid && sub == 0 && mode == 1) {
+      plain_pair(s, g, lane, Fp, Mp, zl);
+      if (lane == 0) {
         const double e = sqrt((Fa[0] - Fp[0]) * (Fa[0] - Fp[0]) + (Fa[1] - Fp[1]) * (Fa[1] - Fp[1])) / unit;
         const double fp = sqrt(Fp[0] * Fp[0] + Fp[1] * Fp[1]) / unit;
         atomicAdd(&ca.err_sum[0], e);
@@ -431,22 +450,22 @@ __global__ void __launch_bounds__(256) k_closure(DS s, ClosureArgs ca) {
 // features, the plain targets (F.n, F.t, M_a / h_a in force units, local Z)
 // and the normal-contact baseline of the patches in the same units (zero when
 // no patch pair is within Rc); the network learns targets minus baseline.
-__global__ void __launch_bounds__(256) k_pair_log(DS s, ClosureArgs ca, double band_h, float* feat, float* targ,
-                                                  float* basel, unsigned* count, int cap) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, sub = lane >> 4, pl = lane & 15;
+// Pairs whose partner patch does not fit the stage are not logged.
+__global__ void __launch_bounds__(CL_WAVES * WAVE) k_pair_log(DS s, ClosureArgs ca, double band_h, float* feat,
+                                                              float* targ, float* basel, unsigned* count, int cap) {
+  __shared__ PatchStage stage[CL_WAVES];
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
   const int npairs = min((int)s.ctrl->npairs, s.cpair_cap);
-  for (int base = (blockIdx.x * 4 + wave) * 16; base < npairs; base += gridDim.x * 64) {
-    const int k = base + pl;
-    const bool valid = k < npairs;
+  for (int k = blockIdx.x * CL_WAVES + wave; k < npairs; k += gridDim.x * CL_WAVES) {
     PairGeo g;
-    pair_geometry(s, k, valid, sub, g);
-    const bool take = valid && g.gap < g.rc + band_h * fmin(g.ha, g.hb);
-    PatchSums P;
-    patch_contact(s, g, sub, take && g.gap >= 0., P);
-    const double dmin = (take && g.gap >= 0.) ? P.dmin : g.gap;
+    pair_geometry(s, k, lane, g);
+    if (!(g.gap < g.rc + band_h * fmin(g.ha, g.hb))) continue;
+    PatchSums P{};
+    if (g.gap >= 0. && !patch_contact(s, g, lane, stage[wave], P)) continue;
+    const double dmin = g.gap >= 0. ? P.dmin : g.gap;
     double Fp[2], Mp, zl;
-    plain_pair(s, g, sub, take, Fp, Mp, zl);
-    if (take && sub == 0) {
+    plain_pair(s, g, lane, Fp, Mp, zl);
+    if (lane == 0) {
       const unsigned row = atomicAdd(count, 1u);
       if ((int)row < cap) {
         float x16[CL_NFEAT];
@@ -466,7 +485,43 @@ __global__ void __launch_bounds__(256) k_pair_log(DS s, ClosureArgs ca, double b
   }
 }
 
-constexpr int CL_BLOCKS = 256;  // closure kernel grid (grid-stride over pair chunks)
+// Pair connections at a rebuild (every step when `force`: set-up): each
+// candidate node pair (i in non-wall particle a, j in non-wall particle b)
+// stamps the partner slot of max(a, b) in min(a, b)'s list with the rebuild
+// count.  One lane per node; consecutive candidates of one particle share
+// the search.
+__global__ void k_cl_connect(DS s, int force) {
+  if (!force && !s.ctrl->rebuild) return;
+  const int ep = (int)s.ctrl->n_rebuilds;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = gid(); i < s.N; i += stride) {
+    const int qa = s.P[s.ptid[i]].nonwall_index;
+    const int cnt = qa >= 0 ? s.ccount[i] : 0;
+    int prev = -1;
+    for (int k = 0; k < cnt; k++) {
+      const int qb = s.P[s.ptid[s.cand[(int64_t)k * s.N + i]]].nonwall_index;
+      if (qb < 0 || qb == prev) continue;
+      prev = qb;
+      const int lo = qa < qb ? qa : qb, hi = qa < qb ? qb : qa;
+      const int* dl = s.dlist + (int64_t)lo * s.dcap;
+      int l = 0, h = s.dcount[lo] - 1;
+      while (l <= h) {
+        const int mid = (l + h) >> 1;
+        const int v = dl[mid];
+        if (v == hi) {
+          s.cconn[(int64_t)lo * s.dcap + mid] = ep;
+          break;
+        }
+        if (v < hi)
+          l = mid + 1;
+        else
+          h = mid - 1;
+      }
+    }
+  }
+}
+
+constexpr int CL_CONNECT_BLOCKS = 1024;  // grid of k_cl_connect (grid-stride over nodes)
 
 // Closure lists from the current candidate and partner lists (set-up and
 // logging; afterwards the rebuilds maintain them): wall_near per particle

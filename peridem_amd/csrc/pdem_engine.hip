@@ -195,6 +195,13 @@ struct pdem_ctx {
 
 namespace {
 
+// grid of the per-pair closure kernels: one wavefront per listed pair (at
+// most cpair_cap), grid-stride beyond 16384 wavefronts
+unsigned closure_blocks(pdem_ctx* c) {
+  const int64_t waves = std::min<int64_t>(std::max(c->ds.cpair_cap, 1), 16384);
+  return (unsigned)((waves + CL_WAVES - 1) / CL_WAVES);
+}
+
 template <int DIM>
 struct Launch {
   // HIP events around the bond-force kernel when profiling
@@ -412,15 +419,14 @@ struct Launch {
     const int db = (int)std::min<int64_t>((std::max(c->ds.n_nonwall, 1) + 3) / 4, 1024);
     hipLaunchKernelGGL(k_rebuild_lists<DIM>, dim3((unsigned)(qb + db)), dim3(TPB), 0, c->stream, c->ds, qb);
   }
-  // closure pairs: modes, network, patch records (2D, closure mode only)
+  // closure pairs: connections (rebuild steps), modes, network, patch
+  // records (2D, closure mode only)
   static void closure(pdem_ctx* c) {
     if (DIM == 2 && c->ds.cl_on) {
-      hipLaunchKernelGGL(k_particle_radius, dim3(radius_blocks(c)), dim3(256), 0, c->stream, c->ds);
-      hipLaunchKernelGGL(k_closure, dim3(CL_BLOCKS), dim3(256), 0, c->stream, c->ds, c->ca);
+      hipLaunchKernelGGL(k_cl_connect, dim3((unsigned)std::min<int64_t>(blocks(c->N), CL_CONNECT_BLOCKS)), dim3(TPB),
+                         0, c->stream, c->ds, 0);
+      hipLaunchKernelGGL(k_closure, dim3(closure_blocks(c)), dim3(CL_WAVES * WAVE), 0, c->stream, c->ds, c->ca);
     }
-  }
-  static unsigned radius_blocks(pdem_ctx* c) {
-    return (unsigned)std::min<int64_t>((std::max(c->ds.n_nonwall, 1) + 3) / 4, 1024);
   }
   static void contact_forces(pdem_ctx* c, bool apply_damping) {
     hipStream_t st = c->stream;
@@ -1802,8 +1808,11 @@ void closure_lists(pdem_ctx* c) {
   HIPCHK(hipMemcpyAsync(&s.ctrl->npairs, &zero, 4, hipMemcpyHostToDevice, c->stream));
   DS t = s;
   t.cl_on = 1;
+  HIPCHK(hipMemsetAsync(s.cmode, 0, (size_t)nn * s.dcap, c->stream));
   hipLaunchKernelGGL(k_cl_wall_near, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, t);
   hipLaunchKernelGGL(k_cl_pairs, dim3(blocks(nn)), dim3(TPB), 0, c->stream, t);
+  hipLaunchKernelGGL(k_cl_connect, dim3((unsigned)std::min<int64_t>(blocks(c->N), CL_CONNECT_BLOCKS)), dim3(TPB), 0,
+                     c->stream, t, 1);
   check_launch(c);
   check_overflow(c);
 }
@@ -1868,8 +1877,8 @@ void closure_setup(pdem_ctx* c, const float* particle_code) {
   s.cout = c->dalloc<double>((size_t)nn * s.dcap * CL_REC);
   s.pflags = c->dalloc<int>(nn);
   s.wall_near = c->dalloc<int>(nn);
-  s.prmax = c->dalloc<double>(nn);
-  s.pzmax = c->dalloc<float>(nn);
+  s.cconn = c->dalloc<int>((size_t)nn * s.dcap, false);
+  HIPCHK(hipMemsetAsync(s.cconn, 0xff, (size_t)nn * s.dcap * sizeof(int), c->stream));  // -1: never connected
   s.cpair_cap = (int)std::min<int64_t>((int64_t)nn * 16, (int64_t)nn * s.dcap);
   s.cpair_cap = std::max(s.cpair_cap, 1);
   s.cpairs = c->dalloc<int>(2 * (size_t)s.cpair_cap);
@@ -1877,6 +1886,8 @@ void closure_setup(pdem_ctx* c, const float* particle_code) {
   c->ca.err_sum = c->dalloc<double>(2);
   c->persist.push_back({s.cpairs, 2 * (size_t)s.cpair_cap * 4});
   c->persist.push_back({s.wall_near, (size_t)nn * 4});
+  c->persist.push_back({s.cmode, (size_t)nn * s.dcap});
+  c->persist.push_back({s.cconn, (size_t)nn * s.dcap * 4});
   c->cl_ready = true;
 }
 
@@ -1962,9 +1973,8 @@ void pair_log_staged(pdem_ctx* c, double band_h, int64_t cap, float* df, float* 
   zero_count();
   DS t = c->ds;
   t.cl_on = 1;
-  hipLaunchKernelGGL(k_particle_radius, dim3((unsigned)std::min<int64_t>((t.n_nonwall + 3) / 4, 1024)),
-                     dim3(256), 0, c->stream, t);
-  hipLaunchKernelGGL(k_pair_log, dim3(CL_BLOCKS), dim3(256), 0, c->stream, t, c->ca, band_h, df, dt, db, dn,
+  hipLaunchKernelGGL(k_pair_log, dim3(closure_blocks(c)), dim3(CL_WAVES * WAVE), 0, c->stream, t, c->ca, band_h, df,
+                     dt, db, dn,
                      (int)std::min<int64_t>(cap, INT_MAX));
   check_launch(c);
   unsigned cnt = 0;

@@ -177,14 +177,13 @@ struct DS {
   int* cpairs;          // [cpair_cap][2] (a, slot): pair a < dlist[a][slot], listed at rebuilds
   int cpair_cap;
   int8_t* cmode;        // [n_nonwall][dcap] 0: no contact possible, 1: closure, 2: plain
+  int* cconn;           // [n_nonwall][dcap] rebuild count at which a candidate node pair joined a and dlist[a][slot]
   double* cout;         // [n_nonwall][dcap][CL_REC] closure force records
   int* pflags;          // [n_nonwall] this step: bit 0 some pair is plain, bit 1 some pair is closure
   int* wall_near;       // [n_nonwall] rebuild count at which a node had a wall candidate
   const uint8_t* surf;  // [N] surface node flag
   const int* surf_ptr;  // [n_nonwall + 1] surface nodes per non-wall particle
   const int* surf_idx;
-  double* prmax;        // [n_nonwall] largest node distance from the centre node (this step)
-  float* pzmax;         // [n_nonwall] largest damage Z of the particle (this step)
   // node-update block maxima of |v|
   unsigned long long* blockmax;
   int nblockmax;
@@ -1571,6 +1570,8 @@ __device__ void damp_pairs_one(const DS& s, int q, int lane) {
       const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
       if (pos < s.dcap) {
         s.dlist[(int64_t)q * s.dcap + pos] = qq;
+        // a pair the closure kernel does not visit this interval has no contact
+        if (s.cl_on) s.cmode[(int64_t)q * s.dcap + pos] = 0;
         if (s.cl_on && qq > q) {
           const unsigned k = atomicAdd(&s.ctrl->npairs, 1u);
           if ((int)k < s.cpair_cap) {
