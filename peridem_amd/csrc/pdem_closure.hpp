@@ -90,12 +90,12 @@ __device__ __forceinline__ float zmax(const DS& s, const DevParticle& P, int lan
   return z;
 }
 
-// Geometry of listed pair k in the pair frame; every lane ends with the same
-// values.  Every node pair (i in a, j in b) is at least gap apart.
-__device__ __forceinline__ void pair_geometry(const DS& s, int k, int lane, PairGeo& g) {
+// Geometry of the pair (a, dlist[a][sa]) in the pair frame; every lane ends
+// with the same values.  Every node pair (i in a, j in b) is at least gap apart.
+__device__ __forceinline__ void pair_geometry(const DS& s, int a, int sa, int lane, PairGeo& g) {
   const int nn = s.n_nonwall;
-  g.a = s.cpairs[2 * (int64_t)k];
-  g.sa = s.cpairs[2 * (int64_t)k + 1];
+  g.a = a;
+  g.sa = sa;
   g.b = s.dlist[(int64_t)g.a * s.dcap + g.sa];
   g.pa = s.nonwall_pid[g.a];
   g.pb = s.nonwall_pid[g.b];
@@ -118,8 +118,13 @@ __device__ __forceinline__ void pair_geometry(const DS& s, int k, int lane, Pair
   g.ext_a = extent(s, g.a, g.ca, g.n, lane);
   g.ext_b = extent(s, g.b, g.cb, nb, lane);
   g.gap = ok ? g.d - g.ext_a - g.ext_b : -1e300;  // coincident centres: plain
-  g.za = zmax(s, Pa, lane);
-  g.zb = zmax(s, Pb, lane);
+  g.za = g.zb = 0.f;  // pair_damage fills them where the gating needs them
+}
+
+// largest damage Z of both particles of the pair
+__device__ __forceinline__ void pair_damage(const DS& s, int lane, PairGeo& g) {
+  g.za = zmax(s, s.P[g.pa], lane);
+  g.zb = zmax(s, s.P[g.pb], lane);
 }
 
 __device__ __forceinline__ double omega(const DS& s, const ClosureArgs& ca, int q, const double* c) {
@@ -355,26 +360,39 @@ __device__ __forceinline__ int find_slot(const DS& s, int q, int target, int lan
   return -1;
 }
 
-// One wavefront per listed pair (grid-stride).  A pair whose particles had no
-// candidate node pair at the last rebuild has no contact force until the next
-// one (the plain method only evaluates candidates): mode 0 without geometry
-// (its cmode entries were cleared at that rebuild).
+// Pending record of a pair for the network (CL_PEND doubles): slot indices
+// of b in a's list and of a in b's list (-1: none), the 16 features, the
+// baseline (3), the force unit, n, c_b - c_a, the patch thresholds
+// ext_a - Rc and ext_b - Rc, Rc, h_a, h_b, the patch sums W, sa (2), saa,
+// sb (2), sbb, and on check steps the plain pair force (2) and moment.
+constexpr int CL_PEND = 42;
+constexpr int CLP_IA = 0, CLP_IB = 1, CLP_X = 2, CLP_B3 = 18, CLP_UNIT = 21, CLP_N = 22, CLP_D = 24, CLP_TA = 26,
+              CLP_TB = 27, CLP_RC = 28, CLP_HA = 29, CLP_HB = 30, CLP_W = 31, CLP_SA = 32, CLP_SAA = 34,
+              CLP_SB = 35, CLP_SBB = 37, CLP_FP = 38, CLP_MP = 40, CLP_CHK = 41;
+
+// Geometry and gating, one wavefront per connected pair (grid-stride; the
+// listed pairs without a candidate node pair at the last rebuild have no
+// contact force until the next one, as the plain method only evaluates
+// candidates: mode 0, their cmode entries were cleared at that rebuild).
+// Modes 0 and 2 are final here; a pair in the closure band goes to the
+// pending list, where k_cl_net runs the network and decides.
 constexpr int CL_WAVES = 4;  // wavefronts per workgroup of the pair kernels
-__global__ void __launch_bounds__(CL_WAVES * WAVE) k_closure(DS s, ClosureArgs ca) {
+__global__ void __launch_bounds__(CL_WAVES * WAVE, 4) k_closure(DS s, ClosureArgs ca) {
   __shared__ PatchStage stage[CL_WAVES];
   const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
-  const int npairs = min((int)s.ctrl->npairs, s.cpair_cap);
-  const int ep = (int)s.ctrl->n_rebuilds;
+  const int nconn = min((int)s.ctrl->nconn, s.cpair_cap);
   const bool check = ca.check_every > 0 && (s.ctrl->n % ca.check_every) == 0;
-  for (int k = blockIdx.x * CL_WAVES + wave; k < npairs; k += gridDim.x * CL_WAVES) {
-    const int a0 = s.cpairs[2 * (int64_t)k];
-    const int64_t ia = (int64_t)a0 * s.dcap + s.cpairs[2 * (int64_t)k + 1];
-    if (s.cconn[ia] != ep) {
-      if (lane == 0) atomicAdd(&ca.stats[0], 1ull);
-      continue;
-    }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int npairs = min((int)s.ctrl->npairs, s.cpair_cap);
+    if (npairs > nconn) atomicAdd(&ca.stats[0], (unsigned long long)(npairs - nconn));
+  }
+  __shared__ unsigned counts[CL_WAVES][3];
+  unsigned cnt[3] = {0u, 0u, 0u};  // pairs per final mode (lane 0), summed per workgroup at the end
+  for (int k = blockIdx.x * CL_WAVES + wave; k < nconn; k += gridDim.x * CL_WAVES) {
+    const int ia = s.conn_list[k];
+    const int prev = s.cmode[ia];
     PairGeo g;
-    pair_geometry(s, k, lane, g);
+    pair_geometry(s, ia / s.dcap, ia % s.dcap, lane, g);
     bool near = g.gap < g.rc;
     PatchSums P{};
     bool fits = true;
@@ -385,63 +403,128 @@ __global__ void __launch_bounds__(CL_WAVES * WAVE) k_closure(DS s, ClosureArgs c
     }
     const double dmin = (near && g.gap >= 0.) ? P.dmin : g.gap;
     int mode = 0;
-    if (near)
-      mode = (ca.use_net && fits && g.gap >= 0. && g.rc - dmin <= ca.pen_max_h * fmin(g.ha, g.hb) &&
-              g.za < 1.f && g.zb < 1.f)
-                 ? 1
-                 : 2;
-    double Fa[2] = {0., 0.}, Ma = 0.;
-    const double unit = pair_unit(ca, g);
-    if (mode == 1) {
-      // every lane holds the pair's features: the 16 rows of the network's
-      // fragment are copies of the pair, row 0 (lane 0) is read
-      float x16[CL_NFEAT];
-      pair_features(s, ca, g, dmin, P, x16);
-      mlp::s4 h1[8], h2[8], h3[8];
-      const mlp::f4 y = mlp::forward(ca.net, mlp::input_frag(ca.net, x16, lane), h1, h2, h3, lane);
-      double ys[4];
-      for (int r = 0; r < 4; r++) ys[r] = (double)(__shfl(y[r], 0) / ca.net.out_scale[r]);
-      if (ys[3] > (double)ca.dz_max) mode = 2;
-      double b3[3];
-      pair_baseline(ca, g, P, b3);
-      const double Fn = (ys[0] + b3[0]) * unit, Ft = (ys[1] + b3[1]) * unit;
-      Fa[0] = Fn * g.n[0] - Ft * g.n[1];
-      Fa[1] = Fn * g.n[1] + Ft * g.n[0];
-      Ma = (ys[2] + b3[2]) * unit * g.ha;
+    if (near) {
+      mode = 2;
+      if (ca.use_net && fits && g.gap >= 0. && g.rc - dmin <= ca.pen_max_h * fmin(g.ha, g.hb)) {
+        pair_damage(s, lane, g);
+        if (g.za < 1.f && g.zb < 1.f) mode = 1;
+      }
+    }
+    if (mode == 0 && prev == 0) {  // both sides already hold mode 0
+      cnt[0]++;
+      continue;
     }
     const int sb = find_slot(s, g.b, g.a, lane);
-    if (lane == 0) {
-      s.cmode[ia] = (int8_t)mode;
-      if (sb >= 0) s.cmode[(int64_t)g.b * s.dcap + sb] = (int8_t)mode;
-      if (mode == 2 || (mode == 1 && sb >= 0)) {
-        const int bit = mode == 2 ? 1 : 2;
-        atomicOr(&s.pflags[g.a], bit);
-        atomicOr(&s.pflags[g.b], bit);
-      }
-      if (mode == 1 && sb >= 0) {
-        const double nb[2] = {-g.n[0], -g.n[1]};
-        const double Fb[2] = {-Fa[0], -Fa[1]};
-        // torque balance: M_b = -M_a + (c_b - c_a) x F_a
-        const double Mb = -Ma + ((g.cb[0] - g.ca[0]) * Fa[1] - (g.cb[1] - g.ca[1]) * Fa[0]);
-        const double ta = g.ext_a - g.rc, tb = g.ext_b - g.rc;
-        closure_record(s.cout + ia * CL_REC, g.n, ta, tb, Fa, Ma, P.W, P.sa, P.saa, g.rc, g.ha);
-        closure_record(s.cout + ((int64_t)g.b * s.dcap + sb) * CL_REC, nb, tb, ta, Fb, Mb, P.W, P.sb, P.sbb,
-                       g.rc, g.hb);
-      }
-      if (sb < 0) s.ctrl->overflow = 6;
-      atomicAdd(&ca.stats[mode], 1ull);
-    }
-    if (check && mode == 1) {
-      double Fp[2], Mp, zl;
-      plain_pair(s, g, lane, Fp, Mp, zl);
+    if (mode != 1) {
       if (lane == 0) {
-        const double e = sqrt((Fa[0] - Fp[0]) * (Fa[0] - Fp[0]) + (Fa[1] - Fp[1]) * (Fa[1] - Fp[1])) / unit;
-        const double fp = sqrt(Fp[0] * Fp[0] + Fp[1] * Fp[1]) / unit;
-        atomicAdd(&ca.err_sum[0], e);
-        atomicAdd(&ca.err_sum[1], fp);
-        atomicAdd(&ca.stats[3], 1ull);
-        atomicMax(&ca.stats[4], (unsigned long long)__double_as_longlong(e));
+        s.cmode[ia] = (int8_t)mode;
+        if (sb >= 0) s.cmode[(int64_t)g.b * s.dcap + sb] = (int8_t)mode;
+        if (mode == 2) {
+          atomicOr(&s.pflags[g.a], 1);
+          atomicOr(&s.pflags[g.b], 1);
+        }
+        if (sb < 0) s.ctrl->overflow = 6;
       }
+      cnt[mode]++;
+      continue;
+    }
+    double Fp[2] = {0., 0.}, Mp = 0., zl = 0.;
+    if (check) plain_pair(s, g, lane, Fp, Mp, zl);
+    if (lane == 0) {
+      const unsigned row = atomicAdd(&s.ctrl->npend, 1u);
+      double* r = s.cl_pend + (int64_t)row * CL_PEND;
+      float x16[CL_NFEAT];
+      pair_features(s, ca, g, dmin, P, x16);
+      double b3[3];
+      pair_baseline(ca, g, P, b3);
+      r[CLP_IA] = (double)ia;
+      r[CLP_IB] = sb >= 0 ? (double)((int64_t)g.b * s.dcap + sb) : -1.;
+      for (int f = 0; f < CL_NFEAT; f++) r[CLP_X + f] = (double)x16[f];
+      for (int q = 0; q < 3; q++) r[CLP_B3 + q] = b3[q];
+      r[CLP_UNIT] = pair_unit(ca, g);
+      for (int d = 0; d < 2; d++) {
+        r[CLP_N + d] = g.n[d];
+        r[CLP_D + d] = g.cb[d] - g.ca[d];
+        r[CLP_SA + d] = P.sa[d];
+        r[CLP_SB + d] = P.sb[d];
+        r[CLP_FP + d] = Fp[d];
+      }
+      r[CLP_TA] = g.ext_a - g.rc;
+      r[CLP_TB] = g.ext_b - g.rc;
+      r[CLP_RC] = g.rc;
+      r[CLP_HA] = g.ha;
+      r[CLP_HB] = g.hb;
+      r[CLP_W] = P.W;
+      r[CLP_SAA] = P.saa;
+      r[CLP_SBB] = P.sbb;
+      r[CLP_MP] = Mp;
+      r[CLP_CHK] = check ? 1. : 0.;
+    }
+  }
+  if (lane == 0)
+    for (int q = 0; q < 3; q++) counts[wave][q] = cnt[q];
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    unsigned long long t = 0;
+    for (int w = 0; w < CL_WAVES; w++) t += counts[w][threadIdx.x];
+    if (t) atomicAdd(&ca.stats[threadIdx.x], t);
+  }
+}
+
+// The network for the pending pairs, 16 per wavefront (one row each):
+// predicted local damage above dz_max sends a pair to the plain method,
+// otherwise the predicted force and moment (remainder over the baseline) go
+// to both sides' transfer records (mode 1).
+__global__ void __launch_bounds__(CL_WAVES * WAVE) k_cl_net(DS s, ClosureArgs ca) {
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE, pl = lane & 15;
+  const int npend = min((int)s.ctrl->npend, s.cpair_cap);
+  for (int base = (blockIdx.x * CL_WAVES + wave) * 16; base < npend; base += gridDim.x * CL_WAVES * 16) {
+    const int k = base + pl;
+    const bool valid = k < npend;
+    const double* r = s.cl_pend + (int64_t)(valid ? k : base) * CL_PEND;
+    float x16[CL_NFEAT];
+    for (int f = 0; f < CL_NFEAT; f++) x16[f] = (float)r[CLP_X + f];
+    mlp::s4 h1[8], h2[8], h3[8];
+    const mlp::f4 y = mlp::forward(ca.net, mlp::input_frag(ca.net, x16, lane), h1, h2, h3, lane);
+    double ys[4];
+    for (int q = 0; q < 4; q++) ys[q] = (double)(__shfl(y[q], pl) / ca.net.out_scale[q]);
+    if (!valid || lane >= 16) continue;
+    const int64_t ia = (int64_t)r[CLP_IA], ib = (int64_t)r[CLP_IB];
+    const int a = (int)(ia / s.dcap), b = s.dlist[ia];
+    const int mode = ys[3] > (double)ca.dz_max ? 2 : 1;
+    s.cmode[ia] = (int8_t)mode;
+    if (ib >= 0) s.cmode[ib] = (int8_t)mode;
+    if (mode == 2 || ib >= 0) {
+      const int bit = mode == 2 ? 1 : 2;
+      atomicOr(&s.pflags[a], bit);
+      atomicOr(&s.pflags[b], bit);
+    }
+    if (ib < 0) s.ctrl->overflow = 6;
+    atomicAdd(&ca.stats[mode], 1ull);
+    if (mode != 1) continue;
+    const double unit = r[CLP_UNIT], n[2] = {r[CLP_N], r[CLP_N + 1]};
+    const double Fn = (ys[0] + r[CLP_B3]) * unit, Ft = (ys[1] + r[CLP_B3 + 1]) * unit;
+    const double Fa[2] = {Fn * n[0] - Ft * n[1], Fn * n[1] + Ft * n[0]};
+    const double Ma = (ys[2] + r[CLP_B3 + 2]) * unit * r[CLP_HA];
+    if (ib >= 0) {
+      const double nb[2] = {-n[0], -n[1]};
+      const double Fb[2] = {-Fa[0], -Fa[1]};
+      // torque balance: M_b = -M_a + (c_b - c_a) x F_a
+      const double Mb = -Ma + (r[CLP_D] * Fa[1] - r[CLP_D + 1] * Fa[0]);
+      const double sa[2] = {r[CLP_SA], r[CLP_SA + 1]}, sb[2] = {r[CLP_SB], r[CLP_SB + 1]};
+      closure_record(s.cout + ia * CL_REC, n, r[CLP_TA], r[CLP_TB], Fa, Ma, r[CLP_W], sa, r[CLP_SAA], r[CLP_RC],
+                     r[CLP_HA]);
+      closure_record(s.cout + ib * CL_REC, nb, r[CLP_TB], r[CLP_TA], Fb, Mb, r[CLP_W], sb, r[CLP_SBB], r[CLP_RC],
+                     r[CLP_HB]);
+    }
+    if (r[CLP_CHK] != 0.) {
+      const double e = sqrt((Fa[0] - r[CLP_FP]) * (Fa[0] - r[CLP_FP]) +
+                            (Fa[1] - r[CLP_FP + 1]) * (Fa[1] - r[CLP_FP + 1])) / unit;
+      const double fp = sqrt(r[CLP_FP] * r[CLP_FP] + r[CLP_FP + 1] * r[CLP_FP + 1]) / unit;
+      atomicAdd(&ca.err_sum[0], e);
+      atomicAdd(&ca.err_sum[1], fp);
+      atomicAdd(&ca.stats[3], 1ull);
+      atomicMax(&ca.stats[4], (unsigned long long)__double_as_longlong(e));
     }
   }
 }
@@ -458,7 +541,7 @@ __global__ void __launch_bounds__(CL_WAVES * WAVE) k_pair_log(DS s, ClosureArgs 
   const int npairs = min((int)s.ctrl->npairs, s.cpair_cap);
   for (int k = blockIdx.x * CL_WAVES + wave; k < npairs; k += gridDim.x * CL_WAVES) {
     PairGeo g;
-    pair_geometry(s, k, lane, g);
+    pair_geometry(s, s.cpairs[2 * (int64_t)k], s.cpairs[2 * (int64_t)k + 1], lane, g);
     if (!(g.gap < g.rc + band_h * fmin(g.ha, g.hb))) continue;
     PatchSums P{};
     if (g.gap >= 0. && !patch_contact(s, g, lane, stage[wave], P)) continue;
@@ -485,12 +568,13 @@ __global__ void __launch_bounds__(CL_WAVES * WAVE) k_pair_log(DS s, ClosureArgs 
   }
 }
 
-// Pair connections at a rebuild (every step when `force`: set-up): each
-// candidate node pair (i in non-wall particle a, j in non-wall particle b)
-// stamps the partner slot of max(a, b) in min(a, b)'s list with the rebuild
-// count.  One lane per node; consecutive candidates of one particle share
-// the search.
+// Pair connections at a rebuild (whatever the step when `force`: set-up):
+// each candidate node pair (i in non-wall particle a, j in non-wall particle
+// b) stamps the partner slot of max(a, b) in min(a, b)'s list with the
+// rebuild count, and the first stamp appends the slot to conn_list.  One lane
+// per node; consecutive candidates of one particle share the search.
 __global__ void k_cl_connect(DS s, int force) {
+  if (gid() == 0) s.ctrl->npend = 0;  // k_closure fills the pending list of this step
   if (!force && !s.ctrl->rebuild) return;
   const int ep = (int)s.ctrl->n_rebuilds;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -509,7 +593,12 @@ __global__ void k_cl_connect(DS s, int force) {
         const int mid = (l + h) >> 1;
         const int v = dl[mid];
         if (v == hi) {
-          s.cconn[(int64_t)lo * s.dcap + mid] = ep;
+          const int64_t slot = (int64_t)lo * s.dcap + mid;
+          // the first stamp of this rebuild lists the pair
+          if (atomicExch(&s.cconn[slot], ep) != ep) {
+            const unsigned r = atomicAdd(&s.ctrl->nconn, 1u);
+            if ((int)r < s.cpair_cap) s.conn_list[r] = (int)slot;
+          }
           break;
         }
         if (v < hi)

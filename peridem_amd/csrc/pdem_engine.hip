@@ -201,6 +201,12 @@ unsigned closure_blocks(pdem_ctx* c) {
   const int64_t waves = std::min<int64_t>(std::max(c->ds.cpair_cap, 1), 16384);
   return (unsigned)((waves + CL_WAVES - 1) / CL_WAVES);
 }
+// grid of the network kernel: 16 pending pairs per wavefront, grid-stride
+// beyond 1024 wavefronts
+unsigned net_blocks(pdem_ctx* c) {
+  const int64_t waves = std::min<int64_t>((std::max(c->ds.cpair_cap, 1) + 15) / 16, 1024);
+  return (unsigned)((waves + CL_WAVES - 1) / CL_WAVES);
+}
 
 template <int DIM>
 struct Launch {
@@ -426,6 +432,8 @@ struct Launch {
       hipLaunchKernelGGL(k_cl_connect, dim3((unsigned)std::min<int64_t>(blocks(c->N), CL_CONNECT_BLOCKS)), dim3(TPB),
                          0, c->stream, c->ds, 0);
       hipLaunchKernelGGL(k_closure, dim3(closure_blocks(c)), dim3(CL_WAVES * WAVE), 0, c->stream, c->ds, c->ca);
+      if (c->ca.use_net)
+        hipLaunchKernelGGL(k_cl_net, dim3(net_blocks(c)), dim3(CL_WAVES * WAVE), 0, c->stream, c->ds, c->ca);
     }
   }
   static void contact_forces(pdem_ctx* c, bool apply_damping) {
@@ -1809,6 +1817,8 @@ void closure_lists(pdem_ctx* c) {
   DS t = s;
   t.cl_on = 1;
   HIPCHK(hipMemsetAsync(s.cmode, 0, (size_t)nn * s.dcap, c->stream));
+  HIPCHK(hipMemsetAsync(s.cconn, 0xff, (size_t)nn * s.dcap * sizeof(int), c->stream));
+  HIPCHK(hipMemcpyAsync(&s.ctrl->nconn, &zero, 4, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_cl_wall_near, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, t);
   hipLaunchKernelGGL(k_cl_pairs, dim3(blocks(nn)), dim3(TPB), 0, c->stream, t);
   hipLaunchKernelGGL(k_cl_connect, dim3((unsigned)std::min<int64_t>(blocks(c->N), CL_CONNECT_BLOCKS)), dim3(TPB), 0,
@@ -1882,12 +1892,15 @@ void closure_setup(pdem_ctx* c, const float* particle_code) {
   s.cpair_cap = (int)std::min<int64_t>((int64_t)nn * 16, (int64_t)nn * s.dcap);
   s.cpair_cap = std::max(s.cpair_cap, 1);
   s.cpairs = c->dalloc<int>(2 * (size_t)s.cpair_cap);
+  s.conn_list = c->dalloc<int>((size_t)s.cpair_cap);
+  s.cl_pend = c->dalloc<double>((size_t)s.cpair_cap * CL_PEND, false);
   c->ca.stats = c->dalloc<unsigned long long>(8);
   c->ca.err_sum = c->dalloc<double>(2);
   c->persist.push_back({s.cpairs, 2 * (size_t)s.cpair_cap * 4});
   c->persist.push_back({s.wall_near, (size_t)nn * 4});
   c->persist.push_back({s.cmode, (size_t)nn * s.dcap});
   c->persist.push_back({s.cconn, (size_t)nn * s.dcap * 4});
+  c->persist.push_back({s.conn_list, (size_t)s.cpair_cap * 4});
   c->cl_ready = true;
 }
 

@@ -60,6 +60,8 @@ struct Ctrl {
   uint32_t nactive;  // nodes with more than LIGHT_MAX candidates (wavefront per node)
   uint32_t nlight;   // nodes with 1..LIGHT_MAX candidates (lane per node)
   uint32_t npairs;   // closure: particle pairs listed at the last rebuild
+  uint32_t nconn;    // closure: listed pairs with a candidate node pair at the last rebuild
+  uint32_t npend;    // closure: pairs of this step waiting for the network
   uint32_t pad_;
   unsigned long long vmax_acc;  // bit pattern of max |v| (non-negative doubles order as integers)
   double vmax_last;
@@ -178,6 +180,8 @@ struct DS {
   int cpair_cap;
   int8_t* cmode;        // [n_nonwall][dcap] 0: no contact possible, 1: closure, 2: plain
   int* cconn;           // [n_nonwall][dcap] rebuild count at which a candidate node pair joined a and dlist[a][slot]
+  int* conn_list;       // [cpair_cap] a * dcap + slot of the connected pairs (a < dlist[a][slot])
+  double* cl_pend;      // [cpair_cap][CL_PEND] pairs of this step waiting for the network
   double* cout;         // [n_nonwall][dcap][CL_REC] closure force records
   int* pflags;          // [n_nonwall] this step: bit 0 some pair is plain, bit 1 some pair is closure
   int* wall_near;       // [n_nonwall] rebuild count at which a node had a wall candidate
@@ -400,6 +404,7 @@ __device__ void ctrl_decide(Ctrl* cg) {
     c->nactive = 0u;
     c->nlight = 0u;
     c->npairs = 0u;
+    c->nconn = 0u;
   }
   *cg = h;
 }
