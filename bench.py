@@ -19,8 +19,10 @@ value = (nodes integrated per step) x steps / seconds of the timed region
 (HIP-graph replay of exactly --steps steps).  "steady_state" repeats the
 measurement over a whole number of contact-rebuild intervals.  The roofline
 entry is for the peridynamic bond passes: the algorithm's bytes per launch
-divided by the average duration of the passes, timed with HIP events on the
-engine's stream over a second, eagerly launched region of the same step.
+divided by the average duration of the passes, timed with HIP events over a
+second, eagerly launched region of the same step (around the kernel on the
+engine stream and around the tiles that run beside it on the side stream;
+per step the longer of the two).
 Before the warm-up the launch shape of the bond passes is autotuned (outside
 the timed regions).
 """
@@ -186,8 +188,9 @@ def measure(spec, prec, steps, warmup, prof_steps, autotune=True, state=None, st
             "bytes_per_launch": bytes_canon, "bytes_per_launch_layout": bytes_layout,
             "achieved_layout": round(achieved_layout, 2), "frac_layout": round(achieved_layout / HBM_PEAK_GBS, 4),
             "mean_neighbors": round(mean_nbr, 2),
-            "timing": f"HIP events on the engine stream around the bond passes (dilation and force) of "
-                      f"{prof_steps} eager steps ({t_prof:.3f} s)"}
+            "timing": f"HIP events around the bond passes (dilation and force) of {prof_steps} eager steps "
+                      f"({t_prof:.3f} s): around the kernel on the engine stream and, where tiles run beside it "
+                      f"on the side stream, around those; per step the longer of the two"}
     n, t = eng.time()
     st = eng.contact_state()
     broken, total = eng.broken_bonds()

@@ -129,6 +129,8 @@ struct pdem_ctx {
   hipGraphExec_t gexec = nullptr;
   bool profiling = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
+  std::vector<int> ev_group;  // per used pair: the timed region it belongs to (regions on two streams: the longer)
+  int ev_next_group = 0;
   double prof_ms = 0.;
   int64_t prof_n = 0;
   std::string err;
@@ -210,8 +212,10 @@ unsigned net_blocks(pdem_ctx* c) {
 
 template <int DIM>
 struct Launch {
-  // HIP events around the bond-force kernel when profiling
-  static std::pair<hipEvent_t, hipEvent_t> ev_begin(pdem_ctx* c) {
+  // HIP events around the bond-force kernels when profiling: a timed region
+  // is one or two event pairs (a kernel on the engine stream and the tiles
+  // run beside it on the side stream); its time is the longer pair
+  static std::pair<hipEvent_t, hipEvent_t> ev_begin(pdem_ctx* c, hipStream_t st) {
     std::pair<hipEvent_t, hipEvent_t> ev{};
     if (!c->profiling) return ev;
     if (c->ev_free.empty()) {
@@ -222,13 +226,14 @@ struct Launch {
     }
     ev = c->ev_free.back();
     c->ev_free.pop_back();
-    HIPCHK(hipEventRecord(ev.first, c->stream));
+    HIPCHK(hipEventRecord(ev.first, st));
     return ev;
   }
-  static void ev_end(pdem_ctx* c, std::pair<hipEvent_t, hipEvent_t> ev) {
+  static void ev_end(pdem_ctx* c, std::pair<hipEvent_t, hipEvent_t> ev, hipStream_t st, int group) {
     if (!c->profiling) return;
-    HIPCHK(hipEventRecord(ev.second, c->stream));
+    HIPCHK(hipEventRecord(ev.second, st));
     c->ev_used.push_back(ev);
+    c->ev_group.push_back(group);
   }
   // the side stream starts after everything enqueued so far on the engine stream
   static void fork(pdem_ctx* c) {
@@ -283,14 +288,28 @@ struct Launch {
   template <int INF, int T>
   static void mixed_state(pdem_ctx* c) {
     const MixedArgs& ma = c->mixed;
+    const int grp = c->ev_next_group++;
     if (ma.ntiles_closed == 0) {
+      auto ev = ev_begin(c, c->stream);
       mixed_dilation<INF, T>(c);
       mixed_force<INF, T>(c);
+      ev_end(c, ev, c->stream, grp);
       return;
     }
     const bool two_pass = ma.ntiles_lds || ma.ntiles_hbm;
+    // the fused kernel is enqueued first (right after the fork point), so a
+    // host that enqueues behind the GPU does not delay it by the side-stream
+    // launches
+    if (two_pass) fork(c);
+    auto ev = ev_begin(c, c->stream);
+    if (ma.small)
+      hipLaunchKernelGGL((k_mixed_state<DIM, INF, T, LDS_SMALL>), dim3(ma.ntiles_closed), dim3(T), 0, c->stream,
+                         c->ds, ma);
+    else
+      hipLaunchKernelGGL((k_mixed_state<DIM, INF, T>), dim3(ma.ntiles_closed), dim3(T), 0, c->stream, c->ds, ma);
+    ev_end(c, ev, c->stream, grp);
     if (two_pass) {
-      fork(c);
+      auto es = ev_begin(c, c->side);
       mixed_umax(c, c->side);
       if (ma.ntiles_lds && ma.small)
         hipLaunchKernelGGL((k_mixed_dilation<DIM, INF, true, T, LDS_SMALL>), dim3(ma.ntiles_lds), dim3(T), 0,
@@ -309,14 +328,10 @@ struct Launch {
       if (ma.ntiles_hbm)
         hipLaunchKernelGGL((k_mixed_force<DIM, INF, false, T>), dim3(ma.ntiles_hbm), dim3(T), 0, c->side, c->ds,
                            ma);
+      ev_end(c, es, c->side, grp);
       HIPCHK(hipEventRecord(c->ev_join, c->side));
+      HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     }
-    if (ma.small)
-      hipLaunchKernelGGL((k_mixed_state<DIM, INF, T, LDS_SMALL>), dim3(ma.ntiles_closed), dim3(T), 0, c->stream,
-                         c->ds, ma);
-    else
-      hipLaunchKernelGGL((k_mixed_state<DIM, INF, T>), dim3(ma.ntiles_closed), dim3(T), 0, c->stream, c->ds, ma);
-    if (two_pass) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   }
   template <int INF>
   static void pd_inf(pdem_ctx* c) {
@@ -328,18 +343,16 @@ struct Launch {
       if (c->rnp_any) hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(nb), dim3(TPB), 0, st, c->ds);
       if (c->pmb_any) hipLaunchKernelGGL((k_pd_bond<DIM, INF, 2>), dim3(nb), dim3(TPB), 0, st, c->ds);
       if (c->elastic_any) hipLaunchKernelGGL((k_pd_bond<DIM, INF, 3>), dim3(nb), dim3(TPB), 0, st, c->ds);
-      // timed: both state-based passes (dilation and force), all tiles
-      auto ev = ev_begin(c);
+      // timed inside: both state-based passes (dilation and force), all tiles
       if (c->mixed_tpb == 256)
         mixed_state<INF, 256>(c);
       else
         mixed_state<INF, 1024>(c);
-      ev_end(c, ev);
       return;
     }
     const unsigned pb = (unsigned)c->pd_tpb, pnb = blocks(c->N, c->pd_tpb);
     if (c->ds.state_global) hipLaunchKernelGGL((k_pd_dilation<DIM, INF>), dim3(pnb), dim3(pb), 0, st, c->ds);
-    auto ev = ev_begin(c);
+    auto ev = ev_begin(c, st);
     if (c->rnp_tile)
       hipLaunchKernelGGL((k_pd_rnp_tile<DIM, INF>), dim3(c->rt.ntiles), dim3(RT_THREADS), c->rt.lds_bytes, st,
                          c->ds, c->rt);
@@ -348,7 +361,7 @@ struct Launch {
     if (c->pmb_any) hipLaunchKernelGGL((k_pd_bond<DIM, INF, 2>), dim3(pnb), dim3(pb), 0, st, c->ds);
     if (c->elastic_any) hipLaunchKernelGGL((k_pd_bond<DIM, INF, 3>), dim3(pnb), dim3(pb), 0, st, c->ds);
     if (c->state_any) hipLaunchKernelGGL((k_pd_state<DIM, INF>), dim3(pnb), dim3(pb), 0, st, c->ds);
-    ev_end(c, ev);
+    ev_end(c, ev, st, c->ev_next_group++);
   }
   static void pd_forces(pdem_ctx* c) {
     if (c->ds.inf_type == 0)
@@ -1780,14 +1793,23 @@ int pdem_profile_enable(pdem_ctx* c, int32_t on) {
 int pdem_profile_get(pdem_ctx* c, double* ms, int64_t* n) {
   return guarded(c, [&] {
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (auto& e : c->ev_used) {
+    if (c->side) HIPCHK(hipStreamSynchronize(c->side));
+    // a region's time is the longer of its pairs (they run side by side)
+    for (size_t k = 0; k < c->ev_used.size();) {
       float t = 0.f;
-      HIPCHK(hipEventElapsedTime(&t, e.first, e.second));
+      size_t m = k;
+      for (; m < c->ev_used.size() && c->ev_group[m] == c->ev_group[k]; m++) {
+        float tm = 0.f;
+        HIPCHK(hipEventElapsedTime(&tm, c->ev_used[m].first, c->ev_used[m].second));
+        t = std::max(t, tm);
+        c->ev_free.push_back(c->ev_used[m]);
+      }
       c->prof_ms += t;
       c->prof_n++;
-      c->ev_free.push_back(e);
+      k = m;
     }
     c->ev_used.clear();
+    c->ev_group.clear();
     *ms = c->prof_n ? c->prof_ms / (double)c->prof_n : 0.;
     *n = c->prof_n;
   });
