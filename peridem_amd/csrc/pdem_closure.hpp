@@ -286,7 +286,7 @@ __device__ __forceinline__ void plain_pair(const DS& s, const PairGeo& g, int la
     double fi[2] = {0., 0.};
     bool any = false;
     for (int k = 0; k < nc; k++) {
-      const int j = s.cand[(int64_t)k * s.N + i];
+      const int j = s.cand[cand_idx(s.N, i, k)];
       if (s.ptid[j] != g.pb) continue;
       double yji[2] = {s.x[0][j] - yi[0], s.x[1][j] - yi[1]};
       const double R = len3<2>(yji);
@@ -583,7 +583,7 @@ __global__ void k_cl_connect(DS s, int force) {
     const int cnt = qa >= 0 ? s.ccount[i] : 0;
     int prev = -1;
     for (int k = 0; k < cnt; k++) {
-      const int qb = s.P[s.ptid[s.cand[(int64_t)k * s.N + i]]].nonwall_index;
+      const int qb = s.P[s.ptid[s.cand[cand_idx(s.N, i, k)]]].nonwall_index;
       if (qb < 0 || qb == prev) continue;
       prev = qb;
       const int lo = qa < qb ? qa : qb, hi = qa < qb ? qb : qa;
@@ -622,7 +622,7 @@ __global__ void k_cl_wall_near(DS s) {
   if (P.nonwall_index < 0) return;
   const int cnt = s.ccount[i];
   for (int k = 0; k < cnt; k++)
-    if (s.pmeta[s.ptid[s.cand[(int64_t)k * s.N + i]]] >> 16) {
+    if (s.pmeta[s.ptid[s.cand[cand_idx(s.N, i, k)]]] >> 16) {
       s.wall_near[P.nonwall_index] = (int)s.ctrl->n_rebuilds;
       return;
     }

@@ -1562,7 +1562,7 @@ int pdem_get_neighbors(pdem_ctx* c, int32_t kind, int64_t* rowptr, int64_t* cols
     HIPCHK(hipMemcpyAsync(cand.data(), c->ds.cand, cand.size() * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int64_t i = 0; i < N; i++) {
-      for (int k = 0; k < cnt[i]; k++) cols[rowptr[i] + k] = cand[(size_t)k * N + i];
+      for (int k = 0; k < cnt[i]; k++) cols[rowptr[i] + k] = cand[(size_t)cand_idx(N, i, k)];
       rowptr[i + 1] = rowptr[i] + cnt[i];
     }
   });

@@ -9,7 +9,12 @@
 //   so every load of a wavefront's k-th neighbor id or bond word is one
 //   contiguous 256-byte (ids) / 256-byte (words) access, and a lane owns its
 //   bond words (no atomics on the parity path).
-//   contact candidates: column-major ELL cand[k*N + i], count ccount[i].
+//   contact candidates: blocked ELL, count ccount[i]: candidate k of node i
+//   at cand[cand_idx(N, i, k)] = cand[((k / CAND_BLOCK) N + i) CAND_BLOCK +
+//   k % CAND_BLOCK], so a node's candidates come in 32-byte runs (a
+//   wavefront taking one node's candidates reads 8 runs per 64 candidates)
+//   and the runs of consecutive nodes lie side by side (a wavefront taking
+//   one candidate rank of consecutive nodes reads 2 KB in one piece).
 //
 // FP64 kernels are compiled with -ffp-contract=off and evaluate every
 // expression in the reference's operation order, so they reproduce the
@@ -21,6 +26,11 @@
 #include <climits>
 
 namespace pdem {
+
+constexpr int CAND_BLOCK = 8;  // contact candidates per run of the blocked layout
+__host__ __device__ inline int64_t cand_idx(int64_t N, int64_t i, int k) {
+  return ((int64_t)(k / CAND_BLOCK) * N + i) * CAND_BLOCK + (k % CAND_BLOCK);
+}
 
 constexpr int WAVE = 64;
 constexpr int LIGHT_MAX_C = 16;  // candidate count up to which one lane takes a node
@@ -1221,7 +1231,7 @@ __device__ int query_node(const DS& s, int64_t i, int pi) {
             d2 += t * t;
           }
           if (d2 < r2) {
-            if (cnt < cap) s.cand[(int64_t)cnt * s.N + i] = j;
+            if (cnt < cap) s.cand[cand_idx(s.N, i, cnt)] = j;
             cnt++;
           }
         }
@@ -1310,7 +1320,7 @@ __device__ __forceinline__ void query_one(const DS& s, int64_t i) {
   const int cnt = query_node<DIM>(s, i, pid);
   if (s.cl_on && pi.nonwall_index >= 0)
     for (int k = 0; k < cnt; k++)
-      if (s.pmeta[s.ptid[s.cand[(int64_t)k * s.N + i]]] >> 16) {
+      if (s.pmeta[s.ptid[s.cand[cand_idx(s.N, i, k)]]] >> 16) {
         s.wall_near[pi.nonwall_index] = (int)s.ctrl->n_rebuilds;
         break;
       }
@@ -1376,7 +1386,7 @@ __global__ void __launch_bounds__(256) k_contact_light(DS s) {
     unsigned pass = 0;
 #pragma unroll 4
     for (int k = 0; k < n; k++) {
-      const int j = s.cand[(int64_t)k * s.N + i];
+      const int j = s.cand[cand_idx(s.N, i, k)];
       double yji[3];
       int pj;
       cand_pos<DIM>(s, rec, j, yi, yji, pj);
@@ -1385,7 +1395,7 @@ __global__ void __launch_bounds__(256) k_contact_light(DS s) {
     while (pass) {
       const int k = __builtin_ctz(pass);
       pass &= pass - 1u;
-      const int j = s.cand[(int64_t)k * s.N + i];
+      const int j = s.cand[cand_idx(s.N, i, k)];
       double yji[3];
       int pjid;
       cand_pos<DIM>(s, rec, j, yi, yji, pjid);
@@ -1475,7 +1485,7 @@ __global__ void __launch_bounds__(CWAVES * WAVE) k_contact_force(DS s) {
       int j = 0;
       double term[6] = {0., 0., 0., 0., 0., 0.};
       if (k < n) {
-        j = s.cand[(int64_t)k * s.N + i];
+        j = s.cand[cand_idx(s.N, i, k)];
         // distance prefilter (largest contact radius) before any particle data is loaded
         double yji[3];
         int pj;
