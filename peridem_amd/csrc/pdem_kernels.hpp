@@ -1527,25 +1527,30 @@ __global__ void __launch_bounds__(CWAVES * WAVE) k_contact_force(DS s) {
     // wavefront complete in order: no workgroup barrier is needed (the waves
     // of a workgroup run different trip counts of this loop)
     __builtin_amdgcn_wave_barrier();
+    if (cnt > MAXACT) {
+      if (lane == 0) s.ctrl->overflow = 3;
+      cnt = MAXACT;
+    }
+    if (cnt > 1) {
+      // ascending node order: each lane moves one active pair to its rank
+      // (a node's candidates are distinct nodes); the wavefront issues all
+      // its reads of the slot before the writes
+      int kj = INT_MAX;
+      double kt[6] = {0., 0., 0., 0., 0., 0.};
+      if (lane < cnt) {
+        kj = sj[wv][lane];
+        for (int d = 0; d < 6; d++) kt[d] = st[wv][lane][d];
+      }
+      int rank = 0;
+      for (int q = 0; q < cnt; q++) rank += sj[wv][q] < kj ? 1 : 0;
+      __builtin_amdgcn_wave_barrier();
+      if (lane < cnt) {
+        sj[wv][rank] = kj;
+        for (int d = 0; d < 6; d++) st[wv][rank][d] = kt[d];
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
     if (lane == 0 && cnt > 0) {
-      if (cnt > MAXACT) {
-        s.ctrl->overflow = 3;
-        cnt = MAXACT;
-      }
-      // ascending node order: insertion sort of the (few) active pairs
-      for (int q = 1; q < cnt; q++) {
-        const int kj = sj[wv][q];
-        double kt[6];
-        for (int d = 0; d < 6; d++) kt[d] = st[wv][q][d];
-        int r = q - 1;
-        while (r >= 0 && sj[wv][r] > kj) {
-          sj[wv][r + 1] = sj[wv][r];
-          for (int d = 0; d < 6; d++) st[wv][r + 1][d] = st[wv][r][d];
-          r--;
-        }
-        sj[wv][r + 1] = kj;
-        for (int d = 0; d < 6; d++) st[wv][r + 1][d] = kt[d];
-      }
       double force[3] = {0., 0., 0.};
       for (int q = 0; q < cnt; q++) {
         for (int d = 0; d < DIM; d++) force[d] += st[wv][q][d];
