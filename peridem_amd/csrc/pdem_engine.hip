@@ -412,13 +412,25 @@ struct Launch {
     hipLaunchKernelGGL(k_contact_force<DIM>, dim3((unsigned)((waves + CWAVES - 1) / CWAVES)),
                        dim3(CWAVES * WAVE), 0, c->stream, c->ds);
   }
-  static void damping(pdem_ctx* c, bool fin = false) {
+  // fin 1: the damping pass finalises the nodes; 2: it leaves the
+  // finalisation to the next node update (f_pending)
+  static void damping(pdem_ctx* c, int fin = 0) {
     if (c->ds.n_nonwall > 0) {
-      // with fin: blocks after the particles add gravity to the wall nodes
-      const int64_t wb = fin && c->ds.has_gravity ? std::min<int64_t>((c->ds.n_wall_nodes + 255) / 256, 256) : 0;
-      hipLaunchKernelGGL(k_damping<DIM>, dim3((unsigned)(c->ds.n_nonwall + wb)), dim3(fin ? 256 : 64), 0, c->stream,
-                         c->ds, fin ? 1 : 0);
+      // with fin 1: blocks after the particles add gravity to the wall nodes
+      const int64_t wb =
+          fin == 1 && c->ds.has_gravity ? std::min<int64_t>((c->ds.n_wall_nodes + 255) / 256, 256) : 0;
+      hipLaunchKernelGGL(k_damping<DIM>, dim3((unsigned)(c->ds.n_nonwall + wb)), dim3(fin == 1 ? 256 : 64), 0,
+                         c->stream, c->ds, fin);
     }
+  }
+  // Large models (unfused step) leave damping and gravity to the next node
+  // update, which adds them to the forces it reads (one pass over f less per
+  // step); a host read of f first finalises them (finalize_pending).  Not
+  // with force-BC sets (their terms follow the finalisation), the closure
+  // (its transfer runs in the finalisation) or Verlet steps (the second half
+  // kick reads f).
+  static bool defer_finalize(pdem_ctx* c, bool verlet) {
+    return c->multi && !verlet && !fuse_finalize(c) && c->ds.n_nonwall > 0 && !c->ds.cl_on && c->fbc.empty();
   }
   // Fused step (node update with BC and centre data, one-workgroup
   // controller, damping pass finalising the nodes): fewer launches for
@@ -460,8 +472,7 @@ struct Launch {
   static void external_forces(pdem_ctx* c, bool with_damping, bool finalized = false) {
     hipStream_t st = c->stream;
     if (!finalized && (c->ds.has_gravity || with_damping))
-      hipLaunchKernelGGL(k_finalize<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, st, c->ds,
-                         with_damping ? 1 : 0);
+      hipLaunchKernelGGL(k_finalize<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, st, c->ds, with_damping ? 1 : 0, 0);
     for (auto& b : c->fbc)
       if (b.n > 0) hipLaunchKernelGGL(k_fbc<DIM>, dim3(blocks(b.n)), dim3(TPB), 0, st, c->ds, b);
   }
@@ -477,20 +488,27 @@ struct Launch {
   }
   // forces after the controller has run: PD (binning on a rebuild), contact
   // (query on a rebuild, then forces), damping, external forces
-  static void forces_after_ctrl(pdem_ctx* c) {
+  static void forces_after_ctrl(pdem_ctx* c, bool verlet = false) {
     pd_forces_and_cells(c);
     const bool fin = fuse_finalize(c);
+    const bool defer = defer_finalize(c, verlet);
     if (c->multi) {
       rebuild_lists(c);
       closure(c);
       contact_force_kernel(c);
-      damping(c, fin);
+      damping(c, fin ? 1 : (defer ? 2 : 0));
     }
-    external_forces(c, c->multi != 0, fin);
+    external_forces(c, c->multi != 0, fin || defer);
   }
   static void compute_forces(pdem_ctx* c) {
     if (c->multi) hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, c->stream, c->ds, 0);
     forces_after_ctrl(c);
+  }
+  // forces left pending by a deferred step, finalised in place
+  static void finalize_pending(pdem_ctx* c) {
+    if (!c->multi) return;
+    hipLaunchKernelGGL(k_finalize<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds, 1, 1);
+    hipLaunchKernelGGL(k_clear_pending, dim3(1), dim3(64), 0, c->stream, c->ds);
   }
   // integrateCD, or integrateVerlet when `verlet` (half kick and drift,
   // forces, second half kick)
@@ -508,7 +526,7 @@ struct Launch {
       displacement_bc(c);
       if (c->multi) hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, c->stream, c->ds, 0);
     }
-    forces_after_ctrl(c);
+    forces_after_ctrl(c, verlet);
     if (verlet) hipLaunchKernelGGL(k_kick<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds);
   }
 };
@@ -528,6 +546,16 @@ int set_err(pdem_ctx* c, const std::string& m) {
 }
 
 void check_launch(pdem_ctx* c) { HIPCHK(hipGetLastError()); }
+
+// forces a deferred step left pending, finalised before a host reads f or
+// a call adds to it
+void finalize_pending(pdem_ctx* c) {
+  dispatch(c, [&] { Launch<2>::finalize_pending(c); }, [&] { Launch<3>::finalize_pending(c); });
+}
+// f set without pending terms
+void clear_pending(pdem_ctx* c) {
+  if (c->multi) hipLaunchKernelGGL(k_clear_pending, dim3(1), dim3(64), 0, c->stream, c->ds);
+}
 
 // every node's candidate-position record from the node arrays (after the
 // positions were set from the host or restored)
@@ -1376,6 +1404,7 @@ int pdem_compute_forces(pdem_ctx* c) {
 int pdem_compute_peridynamic_forces(pdem_ctx* c) {
   return guarded(c, [&] {
     dispatch(c, [&] { Launch<2>::pd_forces(c); }, [&] { Launch<3>::pd_forces(c); });
+    clear_pending(c);
     check_launch(c);
   });
 }
@@ -1383,6 +1412,7 @@ int pdem_compute_peridynamic_forces(pdem_ctx* c) {
 int pdem_update_contact_neighborlist(pdem_ctx* c) {
   return guarded(c, [&] {
     if (!c->multi) return;
+    finalize_pending(c);  // the controller kernel below clears the pending flag
     dispatch(c, [&] { Launch<2>::contact_search(c); }, [&] { Launch<3>::contact_search(c); });
     check_launch(c);
     check_overflow(c);
@@ -1392,6 +1422,7 @@ int pdem_update_contact_neighborlist(pdem_ctx* c) {
 int pdem_compute_contact_forces(pdem_ctx* c) {
   return guarded(c, [&] {
     if (!c->multi) return;
+    finalize_pending(c);
     dispatch(c, [&] { Launch<2>::contact_forces(c, true); }, [&] { Launch<3>::contact_forces(c, true); });
     check_launch(c);
   });
@@ -1399,6 +1430,7 @@ int pdem_compute_contact_forces(pdem_ctx* c) {
 
 int pdem_compute_external_forces(pdem_ctx* c) {
   return guarded(c, [&] {
+    finalize_pending(c);
     dispatch(c, [&] { Launch<2>::external_forces(c, false); }, [&] { Launch<3>::external_forces(c, false); });
     check_launch(c);
   });
@@ -1419,6 +1451,7 @@ int pdem_get_field(pdem_ctx* c, int32_t field, void* host) {
   return guarded(c, [&] {
     const int64_t N = c->N;
     DS& s = c->ds;
+    if (field == PDEM_FIELD_F) finalize_pending(c);
     auto getp = [&](double* const* comp) {
       std::vector<double> t(N);
       double* o = (double*)host;
@@ -1486,7 +1519,11 @@ int pdem_set_field(pdem_ctx* c, int32_t field, const void* host) {
         break;
       case PDEM_FIELD_U: setp(s.u); break;
       case PDEM_FIELD_V: setp(s.v); break;
-      case PDEM_FIELD_F: setp(s.f); break;
+      case PDEM_FIELD_F:
+        setp(s.f);
+        clear_pending(c);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        break;
       case PDEM_FIELD_THETA:
         HIPCHK(hipMemcpyAsync(s.theta, host, N * 8, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -1586,6 +1623,7 @@ int pdem_pp_compressive_test(pdem_ctx* c, int32_t wall_id, int32_t f_dir, double
   return guarded(c, [&] {
     if (wall_id < 0 || wall_id >= c->np || f_dir < 0 || f_dir >= c->dim)
       throw HipError{"bad wall id or direction"};
+    finalize_pending(c);
     dispatch(
         c, [&] { hipLaunchKernelGGL(k_pp_compressive<2>, dim3(1), dim3(64), 0, c->stream, c->ds, wall_id, f_dir); },
         [&] { hipLaunchKernelGGL(k_pp_compressive<3>, dim3(1), dim3(64), 0, c->stream, c->ds, wall_id, f_dir); });
@@ -1943,6 +1981,7 @@ int pdem_closure_enable(pdem_ctx* c, const pdem_mlp* net, double pen_max_h, doub
       if (dev != net->device) throw HipError{"closure network lives on another device"};
     }
     if (check_every < 0) throw HipError{"check_every must be >= 0"};
+    finalize_pending(c);  // the closure's transfer runs in the finalisation of later steps
     HIPCHK(hipStreamSynchronize(c->stream));  // steps in flight still read the previous network
     c->ca.use_net = (net && pen_max_h > 0.) ? 1 : 0;
     c->ca.net = net ? mlp::make_net(net->wb, net->wtb, net->p, net->norm) : mlp::Net{};
@@ -2082,6 +2121,7 @@ int pdem_checkpoint_size(pdem_ctx* c, int64_t* bytes) {
 int pdem_checkpoint_save(pdem_ctx* c, void* buf, int64_t bytes) {
   return guarded(c, [&] {
     if (bytes < ckp_bytes(c)) throw HipError{"checkpoint buffer too small"};
+    finalize_pending(c);  // the checkpoint holds final forces
     char* o = (char*)buf;
     std::memcpy(o, "PDEMCKP1", 8);
     o += 8;

@@ -80,7 +80,10 @@ struct Ctrl {
   unsigned long long broken;
   int64_t restart_step;   // DEMModel::restart: the restart step (restart_active)
   int32_t restart_active;
-  int32_t pad2_;
+  // the force pass left f without the damping and gravity terms: the next
+  // node update adds them (as k_finalize would, in its order), and a host
+  // read of f first runs k_finalize over the pending forces
+  int32_t f_pending;
 };
 
 // displacement-BC set parameters and the per-node entries of the sets that
@@ -429,6 +432,9 @@ __global__ void __launch_bounds__(1024) k_ctrl(DS s, int advance) {
   __shared__ unsigned long long red[1024 / WAVE];
   Ctrl* c = s.ctrl;
   const int t = threadIdx.x;
+  // the node update before this kernel consumed any pending forces, and the
+  // force pass after it writes new ones
+  if (blockIdx.x == 0 && t == 0) c->f_pending = 0;
   if (blockIdx.x == 0) {
     if (advance) {
       // the fused node update left the time advance to this kernel
@@ -566,6 +572,7 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
   int64_t i = gid();
   Ctrl* ct = s.ctrl;
   const double dt = ct->dt;
+  const bool pend = ct->f_pending != 0;  // k_finalize's terms still to add (same order)
   double vm = 0.;
   if (i < s.N) {
     const int pid = s.ptid[i];
@@ -589,6 +596,10 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
         if (!(fx >> d & 1)) {
           fd[d] = s.f[d][i];
           ud[d] = s.u[d][i];
+          if (pend) {
+            if (p.type_index == 0) fd[d] += s.fdamp[3 * pid + d];
+            if (s.has_gravity) fd[d] += p.rho * s.gravity[d];
+          }
         }
       }
       for (int d = 0; d < DIM; d++)
@@ -1731,9 +1742,11 @@ template <int DIM>
 __global__ void __launch_bounds__(256) k_damping(DS s, int fin) {
   const int q = blockIdx.x;
   const int nn = s.n_nonwall;
+  // fin 2: the node update of the next step finalises the forces
+  if (fin == 2 && q == 0 && threadIdx.x == 0) s.ctrl->f_pending = 1;
   if (q >= nn) {
     // finalisation of wall nodes: gravity only (k_finalize)
-    if (fin && s.has_gravity)
+    if (fin == 1 && s.has_gravity)
       for (int64_t k = (int64_t)(q - nn) * blockDim.x + threadIdx.x; k < s.n_wall_nodes;
            k += (int64_t)(gridDim.x - nn) * blockDim.x) {
         const int64_t i = s.wall_nodes[k];
@@ -1744,7 +1757,7 @@ __global__ void __launch_bounds__(256) k_damping(DS s, int fin) {
   }
   __shared__ double fsh[3];
   if (threadIdx.x < WAVE) damping_wave<DIM>(s, q, fsh);
-  if (!fin) return;
+  if (fin != 1) return;
   __syncthreads();
   const DevParticle& p = s.P[s.nonwall_pid[q]];
   const bool cl = DIM == 2 && s.cl_on && (s.pflags[q] & 2);
@@ -1797,9 +1810,9 @@ __device__ __forceinline__ void closure_node_force(const DS& s, int a, int64_t i
 // closure's pair forces on surface nodes (closure mode only), then f += rho g
 // (computeExternalForces, demModel.cpp:791-812).
 template <int DIM>
-__global__ void k_finalize(DS s, int with_damping) {
+__global__ void k_finalize(DS s, int with_damping, int pending_only) {
   int64_t i = gid();
-  if (i >= s.N) return;
+  if (i >= s.N || (pending_only && !s.ctrl->f_pending)) return;
   const int pid = s.ptid[i];
   const DevParticle& p = s.P[pid];
   if (with_damping && p.type_index == 0)
@@ -1812,6 +1825,12 @@ __global__ void k_finalize(DS s, int with_damping) {
   }
   if (s.has_gravity)
     for (int d = 0; d < DIM; d++) s.f[d][i] += p.rho * s.gravity[d];
+}
+
+// f holds its final forces (after k_finalize over pending forces, or forces
+// set without the damping and gravity terms pending)
+__global__ void k_clear_pending(DS s) {
+  if (gid() == 0) s.ctrl->f_pending = 0;
 }
 
 template <int DIM>

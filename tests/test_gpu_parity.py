@@ -169,3 +169,52 @@ def test_check_stop_quantities():
         a, b = g.check_stop(crit, param), r.check_stop(crit, param)
         print(f"[stop] criterion {crit}: gpu {a} oracle {b}", flush=True)
         assert a == b
+
+
+def test_deferred_finalisation_is_bit_exact():
+    """Unfused steps leave damping and gravity to the next node update (forces
+    pending); reads of f, checkpoints, partial force calls and Verlet steps
+    finalise them first.  Against the fused step (damping pass finalising the
+    nodes), bit for bit: fields after eager and replayed steps, a checkpoint
+    taken with forces pending and resumed in another context, and Verlet
+    steps after a deferred start."""
+    spec = configs.packing_3d(n_spheres=6, nodes_per_sphere=200, L=5.0e-3, seed=3, gap_factor=1.0, init_speed=2.0)
+    fields = (capi.FIELD_F, capi.FIELD_U, capi.FIELD_V)
+
+    def run(mode, verlet=False):
+        g = capi.GpuEngine(spec, capi.PREC_MIXED)
+        g.set_step_fusion(mode)
+        g.integrate_begin()
+        out = []
+        if verlet:
+            g.integrate_verlet(40)
+            out.append([g.field(f) for f in fields])
+            g.close()
+            return out, None
+        g.integrate_cd(30)
+        out.append([g.field(f) for f in fields])
+        g.integrate_cd_graph(30)
+        blob = g.checkpoint()
+        g.integrate_cd_graph(30)
+        out.append([g.field(f) for f in fields])
+        pg = g.pp_compressive_test(0, 2)
+        g.close()
+        return out, (blob, pg)
+
+    fused, (_, pg_f) = run(True)
+    unfused, (blob, pg_u) = run(False)
+    for a, b in zip(fused, unfused):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+    assert pg_f == pg_u
+    r = capi.GpuEngine(spec, capi.PREC_MIXED)
+    r.set_step_fusion(False)
+    r.restore(blob)
+    r.integrate_cd_graph(30)
+    for x, y in zip([r.field(f) for f in fields], unfused[1]):
+        np.testing.assert_array_equal(x, y)
+    r.close()
+    vf, _ = run(True, verlet=True)
+    vu, _ = run(False, verlet=True)
+    for x, y in zip(vf[0], vu[0]):
+        np.testing.assert_array_equal(x, y)
