@@ -341,7 +341,8 @@ __device__ __forceinline__ float2 dilation_node(const DS& s, const MixedArgs& m,
         const unsigned bit = 1u << (k & 31);
         bool fs = (word & bit) != 0u;
         const float sa = fabsf(cl) * inv_r;  // |S|
-        Zs = fmaxf(Zs, sa);
+        // maximum of non-negative floats: their bit patterns order alike
+        Zs = __uint_as_float(max(__float_as_uint(Zs), __float_as_uint(sa)));
         if (!fs && sa > c.thr_rel) {
           fs = true;
           wnew |= bit;
@@ -387,6 +388,7 @@ __device__ __forceinline__ void force_node(const DS& s, const MixedArgs& m, cons
   uint4 blk = n > 0 ? cb[0] : make_uint4(0, 0, 0, 0);
   for (int kb = 0; kb < n; kb += KB) {
     if ((kb & 31) == 0) word = bw[(int64_t)(kb >> 5) * WAVE];
+    const unsigned wblk = word >> (kb & 31);  // this block's 8 bond bits from bit 0
     const uint4 cur = blk;
     if (kb + KB < n) blk = cb[(int64_t)(kb / KB + 1) * WAVE];
     float fblk[3] = {0.f, 0.f, 0.f};
@@ -414,7 +416,7 @@ __device__ __forceinline__ void force_node(const DS& s, const MixedArgs& m, cons
         const float inv_r = rsqf_hw(xx), inv_ly = rsqf_hw(yy);
         const float r = xx * inv_r, ly = yy * inv_ly;
         const float cl = t2 * rcpf(ly + r);
-        const bool fs = (word & (1u << (k & 31))) != 0u;
+        const bool fs = (wblk & (1u << tt)) != 0u;
         float sf;
         if (!fs) {
           if (r > c.low_rel) volj *= c.up_h - r * c.inv_h;
