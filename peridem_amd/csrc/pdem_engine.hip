@@ -443,10 +443,12 @@ struct Launch {
     return true;
   }
   static bool fuse_finalize(pdem_ctx* c) { return fuse_step(c) && c->ds.n_nonwall > 0; }
-  // rebuild steps: candidate query and damping partner lists in one launch
-  // (grid-stride, capped grids keep the no-op launch of other steps short)
+  // rebuild steps: candidate query (a lane per node: the nodes' scans vary
+  // in length, and more wavefronts in flight hide their latency; the launch
+  // of other steps, which returns at once, costs under 1 us more than a
+  // capped grid) and damping partner lists in one launch
   static void rebuild_lists(pdem_ctx* c) {
-    const int qb = (int)std::min<int64_t>(blocks(c->N), 4096);
+    const int qb = (int)blocks(c->N);
     const int db = (int)std::min<int64_t>((std::max(c->ds.n_nonwall, 1) + 3) / 4, 1024);
     hipLaunchKernelGGL(k_rebuild_lists<DIM>, dim3((unsigned)(qb + db)), dim3(TPB), 0, c->stream, c->ds, qb);
   }

@@ -1308,25 +1308,16 @@ __device__ __forceinline__ bool pair_plain(const DS& s, int a, int b) {
 
 // ------------------------------------------------------- contact forces
 // Candidate query on a rebuild: one lane per searching node; nodes that end
-// with candidates (and compute forces) are appended to the active list.
-// append slot for the lanes of a wavefront whose `pred` holds: one atomic
-// per wavefront (the lists are consumed per node, so their order is free)
-__device__ __forceinline__ int wave_append(unsigned* counter, bool pred) {
-  const unsigned long long m = __ballot(pred);
-  if (!m) return -1;
-  const int leader = __ffsll((long long)m) - 1;
-  const int lane = (int)(threadIdx.x & (WAVE - 1));
-  unsigned base = 0;
-  if (lane == leader) base = atomicAdd(counter, (unsigned)__popcll(m));
-  base = __shfl(base, leader);
-  return pred ? (int)(base + __popcll(m & ((1ull << lane) - 1ull))) : -1;
-}
+// with candidates (and compute forces) join the light or the active list
+// (the lists are consumed per node, so their order is free).
 
+// the candidate query of node i; returns the contact list the node joins:
+// 1 light (lane per node), 2 active (wavefront per node), 0 none
 template <int DIM>
-__device__ __forceinline__ void query_one(const DS& s, int64_t i) {
+__device__ __forceinline__ int query_one(const DS& s, int64_t i) {
   const int pid = s.ptid[i];
   const DevParticle& pi = s.P[pid];
-  if (!pi.searches) return;
+  if (!pi.searches) return 0;
   const uint8_t ep = (uint8_t)s.ctrl->n_rebuilds;
   const int cnt = query_node<DIM>(s, i, pid);
   if (s.cl_on && pi.nonwall_index >= 0)
@@ -1335,18 +1326,13 @@ __device__ __forceinline__ void query_one(const DS& s, int64_t i) {
         s.wall_near[pi.nonwall_index] = (int)s.ctrl->n_rebuilds;
         break;
       }
-  const bool want = cnt > 0 && pi.compute_force;
-  const bool lt = want && cnt <= s.light_max;
-  const bool ac = want && cnt > s.light_max;
-  const int pl = wave_append(&s.ctrl->nlight, lt);
-  const int pa = wave_append(&s.ctrl->nactive, ac);
-  if (lt) s.light[pl] = (int)i;
-  if (ac) s.active[pa] = (int)i;
   // node i is a candidate of each of its searching candidates (the FP64 test
   // is symmetric): its record is kept current until the next rebuild.  The
   // byte store comes last: a store through uint8_t may alias any earlier
   // load, which would otherwise have to be repeated after it.
   if (cnt > 0) s.xa_epoch[i] = ep;
+  const bool want = cnt > 0 && pi.compute_force;
+  return want ? (cnt <= s.light_max ? 1 : 2) : 0;
 }
 
 // candidate j seen from node position yi: yji = x_j - yi, and j's particle id
@@ -1634,8 +1620,36 @@ template <int DIM>
 __global__ void __launch_bounds__(256) k_rebuild_lists(DS s, int qblocks) {
   if (!s.ctrl->rebuild) return;
   if ((int)blockIdx.x < qblocks) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < s.N; i += (int64_t)qblocks * blockDim.x)
-      query_one<DIM>(s, i);
+    // the workgroup's nodes join the light and active lists with one atomic
+    // per list and workgroup (ballot counts per wavefront, offsets in LDS)
+    __shared__ unsigned wcnt[2][256 / WAVE];
+    __shared__ unsigned wbase[2];
+    const int lane = (int)(threadIdx.x & (WAVE - 1)), w = (int)(threadIdx.x / WAVE);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < s.N; b0 += (int64_t)qblocks * blockDim.x) {
+      const int64_t i = b0 + threadIdx.x;
+      const int list = i < s.N ? query_one<DIM>(s, i) : 0;
+      const unsigned long long ml = __ballot(list == 1), ma = __ballot(list == 2);
+      if (lane == 0) {
+        wcnt[0][w] = (unsigned)__popcll(ml);
+        wcnt[1][w] = (unsigned)__popcll(ma);
+      }
+      __syncthreads();
+      if (threadIdx.x < 2) {
+        unsigned t = 0;
+        for (int v = 0; v < (int)(blockDim.x / WAVE); v++) t += wcnt[threadIdx.x][v];
+        wbase[threadIdx.x] = t ? atomicAdd(threadIdx.x == 0 ? &s.ctrl->nlight : &s.ctrl->nactive, t) : 0u;
+      }
+      __syncthreads();
+      unsigned ol = wbase[0] + (unsigned)__popcll(ml & below), oa = wbase[1] + (unsigned)__popcll(ma & below);
+      for (int v = 0; v < w; v++) {
+        ol += wcnt[0][v];
+        oa += wcnt[1][v];
+      }
+      if (list == 1) s.light[ol] = (int)i;
+      if (list == 2) s.active[oa] = (int)i;
+      __syncthreads();  // wcnt and wbase are rewritten by the next round
+    }
     return;
   }
   const int lane = threadIdx.x & (WAVE - 1);
