@@ -118,6 +118,7 @@ struct pdem_ctx {
   // bond-based RNP pair-tile kernel (FP64, all force-computing particles RNP)
   RnpTileArgs rt{};
   bool rnp_tile = false;  // use it (set up by default, autotuned)
+  bool nf_zeroed = false;  // inside a step whose node update zeroed f of the non-force nodes
   int step_fusion = 1;  // fused step kernels: 0 never, 1 models up to FUSE_MAX_NODES, 2 any size
   bool state_fusion = true;  // closed state-based tiles run both passes in one kernel
   bool light_used = false;  // the lane-per-node contact list may hold nodes
@@ -338,7 +339,7 @@ struct Launch {
     hipStream_t st = c->stream;
     const unsigned nb = blocks(c->N);
     if (c->precision == PDEM_PREC_MIXED && c->ntiles > 0) {
-      if (c->ds.n_noforce > 0)
+      if (c->ds.n_noforce > 0 && !c->nf_zeroed)
         hipLaunchKernelGGL(k_zero_noforce<DIM>, dim3(std::min<unsigned>(nb, BZ_BLOCKS)), dim3(TPB), 0, st, c->ds);
       if (c->rnp_any) hipLaunchKernelGGL((k_pd_rnp<DIM, INF>), dim3(nb), dim3(TPB), 0, st, c->ds);
       if (c->pmb_any) hipLaunchKernelGGL((k_pd_bond<DIM, INF, 2>), dim3(nb), dim3(TPB), 0, st, c->ds);
@@ -528,7 +529,9 @@ struct Launch {
       displacement_bc(c);
       if (c->multi) hipLaunchKernelGGL(k_ctrl, dim3(CTRL_BLOCKS), dim3(1024), 0, c->stream, c->ds, 0);
     }
+    c->nf_zeroed = c->ds.nf_zero != 0;  // the node update above zeroed f of the non-force nodes
     forces_after_ctrl(c, verlet);
+    c->nf_zeroed = false;
     if (verlet) hipLaunchKernelGGL(k_kick<DIM>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds);
   }
 };
@@ -1191,6 +1194,9 @@ void build(pdem_ctx* c, const pdem_desc* d) {
         for (int64_t i = c->hp[p].glob_start; i < c->hp[p].glob_end; i++) nf.push_back(i);
     s.n_noforce = (int64_t)nf.size();
     s.noforce_nodes = c->upload(nf.data(), nf.size());
+    // the mixed bond kernels leave f of these nodes alone: the node update
+    // of a step zeroes it (other force passes launch k_zero_noforce)
+    s.nf_zero = (c->precision == PDEM_PREC_MIXED && c->ntiles > 0 && s.n_noforce > 0) ? 1 : 0;
   }
   // state carried between steps: node fields, bond bits, damage, controller,
   // candidate and partner lists of the last rebuild (everything else is

@@ -209,6 +209,7 @@ struct DS {
   int64_t n_wall_nodes;
   const int64_t* noforce_nodes;  // nodes of particles that compute no peridynamic force
   int64_t n_noforce;
+  int nf_zero;  // the node update zeroes f of nodes without peridynamic force (mixed bond kernels)
   // per-node displacement-BC entries (CSR over nodes, set order)
   const int* bc_ptr;
   const BcEntry* bc_ent;
@@ -613,6 +614,10 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
       xs_ok = true;
       vm = len3<DIM>(vd);
       s.vmag[i] = vm;
+    } else if (s.nf_zero) {
+      // the step's force pass starts from zero here (its bond kernels skip
+      // these nodes); nothing reads their f before it
+      for (int d = 0; d < DIM; d++) s.f[d][i] = 0.;
     }
     if (fused) {
       const double t_new = ct->time + dt;
