@@ -218,7 +218,9 @@ int pdem_integrate_cd(pdem_ctx *ctx, int64_t nsteps);
 
 /* DEMModel::computeForces (demModel.cpp:462-600). */
 int pdem_compute_forces(pdem_ctx *ctx);
-/* DEMModel::computePeridynamicForces (demModel.cpp:602-789); overwrites f. */
+/* DEMModel::computePeridynamicForces (demModel.cpp:602-789): overwrites f of
+ * the nodes of force-computing particles; f of the other nodes stays as it
+ * is (the last step's forces). */
 int pdem_compute_peridynamic_forces(pdem_ctx *ctx);
 /* DEMModel::updateContactNeighborlist (demModel.cpp:1696-1806), including the
  * cadence decision of updateContactNeighborSearchParameters (:1808-1955). */

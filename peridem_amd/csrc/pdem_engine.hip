@@ -121,6 +121,7 @@ struct pdem_ctx {
   bool nf_zeroed = false;  // inside a step whose node update zeroed f of the non-force nodes
   int step_fusion = 1;  // fused step kernels: 0 never, 1 models up to FUSE_MAX_NODES, 2 any size
   bool state_fusion = true;  // closed state-based tiles run both passes in one kernel
+  double* d_nfsave = nullptr;  // f of the non-force nodes around a standalone PD pass
   bool light_used = false;  // the lane-per-node contact list may hold nodes
   double* pp_out = nullptr;  // two-particle test results
   // strain / stress post-processing (pdem_fe_setup)
@@ -1411,8 +1412,22 @@ int pdem_compute_forces(pdem_ctx* c) {
 
 int pdem_compute_peridynamic_forces(pdem_ctx* c) {
   return guarded(c, [&] {
+    // the PD passes set f of the nodes that take no PD force to zero (the
+    // force reset of a step); computePeridynamicForces alone leaves their f
+    // as it is: it is saved, final (pending terms applied), and put back
+    finalize_pending(c);
+    const int64_t nnf = c->ds.n_noforce;
+    if (nnf > 0 && !c->d_nfsave) c->d_nfsave = c->dalloc<double>(3 * nnf, false);
+    const unsigned nb = (unsigned)std::min<int64_t>(blocks(std::max<int64_t>(nnf, 1)), BZ_BLOCKS);
+    auto copy = [&](int save) {
+      if (nnf == 0) return;
+      dispatch(
+          c, [&] { hipLaunchKernelGGL(k_noforce_copy<2>, dim3(nb), dim3(TPB), 0, c->stream, c->ds, c->d_nfsave, save); },
+          [&] { hipLaunchKernelGGL(k_noforce_copy<3>, dim3(nb), dim3(TPB), 0, c->stream, c->ds, c->d_nfsave, save); });
+    };
+    copy(1);
     dispatch(c, [&] { Launch<2>::pd_forces(c); }, [&] { Launch<3>::pd_forces(c); });
-    clear_pending(c);
+    copy(0);
     check_launch(c);
   });
 }

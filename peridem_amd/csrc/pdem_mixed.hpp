@@ -524,4 +524,20 @@ __global__ void k_zero_noforce(DS s) {
   }
 }
 
+// copy of f of the nodes that take no PD force (save != 0: into buf, else
+// back from it), around a standalone PD force pass; grid-stride
+template <int DIM>
+__global__ void k_noforce_copy(DS s, double* buf, int save) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = gid(); k < s.n_noforce; k += stride) {
+    const int64_t i = s.noforce_nodes[k];
+    for (int d = 0; d < DIM; d++) {
+      if (save)
+        buf[d * s.n_noforce + k] = s.f[d][i];
+      else
+        s.f[d][i] = buf[d * s.n_noforce + k];
+    }
+  }
+}
+
 }  // namespace pdem

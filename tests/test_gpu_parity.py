@@ -218,3 +218,33 @@ def test_deferred_finalisation_is_bit_exact():
     vu, _ = run(False, verlet=True)
     for x, y in zip(vf[0], vu[0]):
         np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("prec", [capi.PREC_FP64, capi.PREC_MIXED])
+def test_peridynamic_forces_leave_other_nodes(prec):
+    """computePeridynamicForces alone (demModel.cpp:602) writes f of the nodes
+    of force-computing particles only: the walls keep the forces of the last
+    step (contact, damping, gravity), also when the engine holds them pending
+    after unfused steps."""
+    spec = configs.packing_3d(n_spheres=6, nodes_per_sphere=150, L=5.0e-3, seed=1, gap_factor=1.0,
+                              init_speed=5.0)
+    g = capi.GpuEngine(spec, prec)
+    g.set_step_fusion(False)  # forces pending after every step
+    r = oracle.OracleEngine(spec, 8)
+    for e in (g, r):
+        e.integrate_begin()
+        e.integrate_cd(30)
+    walls = np.concatenate([np.arange(p["glob_start"], p["glob_end"]) for p in spec.particles
+                            if not p["compute_force"]])
+    assert walls.size > 0
+    f_before = r.field(capi.FIELD_F)[walls]
+    for e in (g, r):
+        e.compute_peridynamic_forces()
+    fg, fr = g.field(capi.FIELD_F), r.field(capi.FIELD_F)
+    np.testing.assert_array_equal(fr[walls], f_before)
+    print(f"[pd-only] prec {prec}: max |f| walls {np.abs(fr[walls]).max():.3e} all {np.abs(fr).max():.3e}",
+          flush=True)
+    assert np.abs(fr[walls]).max() > 0.0
+    bar = 1e-12 if prec == capi.PREC_FP64 else 1e-4
+    assert np.abs(fg - fr).max() <= bar * np.abs(fr).max()
+    assert np.abs(fg[walls] - fr[walls]).max() <= 1e-12 * np.abs(fr).max()
