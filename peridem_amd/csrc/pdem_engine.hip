@@ -383,17 +383,20 @@ struct Launch {
                        dim3(CELL_TPB), 0, st, c->ds);
   }
   // PD forces and, on a rebuild step, the cell list on the second side stream
-  // beside them (joined before the candidate query)
+  // beside them (joined before the candidate query).  The fork point is
+  // recorded first and the PD kernels are enqueued before the cell kernels:
+  // the same dependencies, and a captured graph keeps the PD chain on the
+  // engine's queue.
   static void pd_forces_and_cells(pdem_ctx* c) {
     if (!c->multi) {
       pd_forces(c);
       return;
     }
     HIPCHK(hipEventRecord(c->ev_fork2, c->stream));
+    pd_forces(c);
     HIPCHK(hipStreamWaitEvent(c->side2, c->ev_fork2, 0));
     cells_build(c, c->side2);
     HIPCHK(hipEventRecord(c->ev_join2, c->side2));
-    pd_forces(c);
     HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join2, 0));
   }
   // standalone search: controller, cell list, candidate query

@@ -306,6 +306,10 @@ __device__ __forceinline__ float2 dilation_node(const DS& s, const MixedArgs& m,
   const float xi[3] = {xvi.x, xvi.y, xvi.z};
   const auto own = w.own(ip);
   const float us = w.us;
+  // |xi + eta|^2 - |xi|^2 = us^2 ee + 2 us xe in one multiply and one FMA
+  // (us is a power of two: the same value as us (us ee + 2 xe) while us^2
+  // is a normal float, i.e. for displacements above ~1e-14 of the unit)
+  const float us2 = us * us, us2x = 2.f * us;
   double theta = 0.;
   float Zs = 0.f;  // maximum of |S| over all bonds, broken ones included; Z = Zs / s0
   unsigned word = 0, wnew = 0;
@@ -316,11 +320,12 @@ __device__ __forceinline__ float2 dilation_node(const DS& s, const MixedArgs& m,
       word = bw[(int64_t)(kb >> 5) * WAVE];
       wnew = word;
     }
+    const unsigned wblk = word >> (kb & 31);  // this block's 8 bond bits from bit 0
+    unsigned brk = 0;                         // bonds of the block that break now
     const uint4 cur = blk;
     if (kb + KB < n) blk = cb[(int64_t)(kb / KB + 1) * WAVE];
     auto bond = [&](const int tt) {
-      const int k = kb + tt;
-      if (k < n) {
+      if (kb + tt < n) {
         const int jp = block_id(cur, tt);
         const float4 xvj = w.xv(jp);
         const float xj[3] = {xvj.x, xvj.y, xvj.z};
@@ -334,18 +339,17 @@ __device__ __forceinline__ float2 dilation_node(const DS& s, const MixedArgs& m,
           xe += dx[d] * du[d];
           ee += du[d] * du[d];
         }
-        const float t2 = us * fmaf(us, ee, 2.f * xe);  // |xi + eta|^2 - |xi|^2
+        const float t2 = fmaf(us2, ee, us2x * xe);  // |xi + eta|^2 - |xi|^2
         const float inv_r = rsqf_hw(xx);
         const float r = xx * inv_r;
         const float cl = t2 * rcpf(sqrtf_hw(xx + t2) + r);
-        const unsigned bit = 1u << (k & 31);
-        bool fs = (word & bit) != 0u;
+        bool fs = ((wblk >> tt) & 1u) != 0u;
         const float sa = fabsf(cl) * inv_r;  // |S|
         // maximum of non-negative floats: their bit patterns order alike
         Zs = __uint_as_float(max(__float_as_uint(Zs), __float_as_uint(sa)));
         if (!fs && sa > c.thr_rel) {
           fs = true;
-          wnew |= bit;
+          brk |= 1u << tt;
         }
         if (!fs) {
           if (r > c.low_rel) volj *= c.up_h - r * c.inv_h;
@@ -355,6 +359,7 @@ __device__ __forceinline__ float2 dilation_node(const DS& s, const MixedArgs& m,
     };
 #pragma unroll
     for (int tt = 0; tt < KB; tt++) bond(tt);
+    wnew |= brk << (kb & 31);
   }
   if (n > 0 && wnew != word) bw[(int64_t)((n - 1) >> 5) * WAVE] = wnew;
   const double mi = s.mx[i];
@@ -383,6 +388,7 @@ __device__ __forceinline__ void force_node(const DS& s, const MixedArgs& m, cons
   const float us = w.us;
   const float2 abi = w.ab(ip, own.b);
   const float ai = abi.x, bi = abi.y;
+  const float us2 = us * us, us2x = 2.f * us;
   double force[3] = {0., 0., 0.};
   unsigned word = 0;
   uint4 blk = n > 0 ? cb[0] : make_uint4(0, 0, 0, 0);
@@ -393,8 +399,7 @@ __device__ __forceinline__ void force_node(const DS& s, const MixedArgs& m, cons
     if (kb + KB < n) blk = cb[(int64_t)(kb / KB + 1) * WAVE];
     float fblk[3] = {0.f, 0.f, 0.f};
     auto bond = [&](const int tt) {
-      const int k = kb + tt;
-      if (k < n) {
+      if (kb + tt < n) {
         const int jp = block_id(cur, tt);
         const float4 xvj = w.xv(jp);
         const float xj[3] = {xvj.x, xvj.y, xvj.z};
@@ -411,7 +416,7 @@ __device__ __forceinline__ void force_node(const DS& s, const MixedArgs& m, cons
           xe += dx[d] * du[d];
           ee += du[d] * du[d];
         }
-        const float t2 = us * fmaf(us, ee, 2.f * xe);  // |y|^2 - |xi|^2
+        const float t2 = fmaf(us2, ee, us2x * xe);  // |y|^2 - |xi|^2
         const float yy = xx + t2;
         const float inv_r = rsqf_hw(xx), inv_ly = rsqf_hw(yy);
         const float r = xx * inv_r, ly = yy * inv_ly;
