@@ -1455,8 +1455,10 @@ __global__ void __launch_bounds__(256) k_contact_light(DS s) {
 constexpr int CWAVES = 4;      // wavefronts per workgroup
 constexpr int MAXACT = 64;     // active pairs per node held in LDS
 
+// (at most 128 VGPRs: 4 wavefronts per SIMD; the kernel waits on chains of
+// dependent gathers, and 3 wavefronts per SIMD took 28 % longer)
 template <int DIM>
-__global__ void __launch_bounds__(CWAVES * WAVE) k_contact_force(DS s) {
+__global__ void __launch_bounds__(CWAVES * WAVE, 4) k_contact_force(DS s) {
   __shared__ int sj[CWAVES][MAXACT];
   __shared__ double st[CWAVES][MAXACT][6];
   // this rebuild's query is done: leave the cell table empty for the next
