@@ -122,7 +122,7 @@ def kernel_names(spec, eng, prec):
     state = any(p["material_type"] == capi.MAT_PDSTATE and p["compute_force"] for p in spec.particles)
     if prec == capi.PREC_MIXED and state:
         return ("k_mixed_state (+ k_mixed_dilation, k_mixed_force of open tiles)", "mixed_state",
-                "FP32 VALU issue (about one wave instruction per CU cycle, 59 per bond and pass)")
+                "FP32 VALU issue (about one wave instruction per CU cycle, 57 per bond and pass)")
     if state:
         return "k_pd_dilation + k_pd_state", "pd_state", "FP64 VALU"
     if eng.launch_config()[0] == 512:
