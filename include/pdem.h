@@ -342,8 +342,11 @@ int pdem_set_contact_light_max(pdem_ctx *ctx, int32_t light_max);
 /* Step fusion for multi-particle models: the node update also applies the
  * displacement BC and packs the particle centres, the controller runs as one
  * workgroup and the damping pass finalises the nodes, which saves launches
- * where the step is launch-bound.  on: 0 never, 1 (default) models of up to
- * 2^20 nodes, 2 any size.  Results are the same bit for bit in every mode.
+ * where the step is launch-bound.  on: 0 never; 1 (default) all of it for
+ * models of up to 2^20 nodes, and for larger models the node update and
+ * controller only, where no displacement-BC set uses the sine or rotation
+ * time function (the finalisation stays with the next node update); 2 all
+ * of it at any size.  Results are the same bit for bit in every mode.
  * No counterpart in the reference. */
 int pdem_set_step_fusion(pdem_ctx *ctx, int32_t on);
 /* Fused state-based passes (mixed precision, on by default): a particle

@@ -551,8 +551,10 @@ class GpuEngine(Engine):
         return tuple(x.value for x in v)
 
     def set_step_fusion(self, on=True):
-        """Fused step kernels: False never, True (default) for multi-particle
-        models of up to 2^20 nodes, 2 at any size (results unchanged)."""
+        """Fused step kernels: False never; True (default) for multi-particle
+        models of up to 2^20 nodes, and the fused node update and controller
+        for larger models without sine or rotation displacement BCs; 2 all
+        of it at any size (results unchanged)."""
         self._check(self.lib.pdem_set_step_fusion(self.ctx, int(on)), "set_step_fusion")
 
     def set_state_fusion(self, on=True):

@@ -89,6 +89,7 @@ struct pdem_ctx {
   std::vector<pdem_particle> hp;
   std::vector<DevBc> ubc, fbc;
   std::vector<int> ubc_zero;
+  bool bc_trig = false;  // a displacement-BC set uses the sine or rotation time function
   std::vector<char> zero_done;
   // device arrays that carry state from step to step (checkpoint / restore)
   std::vector<std::pair<void*, size_t>> persist;
@@ -439,15 +440,19 @@ struct Launch {
   }
   // Fused step (node update with BC and centre data, one-workgroup
   // controller, damping pass finalising the nodes): fewer launches for
-  // models small enough that the step is launch-bound; the unfused kernels
-  // stream large models at full occupancy.
+  // models small enough that the step is launch-bound.  Larger models fuse
+  // the node update and the controller only where no BC set needs the sine
+  // or rotation time function (whose FP64 sin/cos cost the fused update half
+  // its occupancy), and keep the finalisation deferred to the next node
+  // update (a finalising damping pass would add a pass over f).
+  static bool large(pdem_ctx* c) { return c->step_fusion == 1 && c->N > FUSE_MAX_NODES; }
   static bool fuse_step(pdem_ctx* c) {
-    if (!c->multi || c->step_fusion == 0 || (c->step_fusion == 1 && c->N > FUSE_MAX_NODES)) return false;
+    if (!c->multi || c->step_fusion == 0 || (large(c) && c->bc_trig)) return false;
     for (size_t k = 0; k < c->ubc.size(); k++)
       if (c->ubc_zero[k] && !c->zero_done[k]) return false;  // must apply in set order first
     return true;
   }
-  static bool fuse_finalize(pdem_ctx* c) { return fuse_step(c) && c->ds.n_nonwall > 0; }
+  static bool fuse_finalize(pdem_ctx* c) { return fuse_step(c) && !large(c) && c->ds.n_nonwall > 0; }
   // rebuild steps: candidate query (a lane per node: the nodes' scans vary
   // in length, and more wavefronts in flight hide their latency; the launch
   // of other steps, which returns at once, costs under 1 us more than a
@@ -521,9 +526,12 @@ struct Launch {
   // forces, second half kick)
   static void integrate_step(pdem_ctx* c, bool verlet = false) {
     const bool fused = fuse_step(c);
-    if (fused)
-      hipLaunchKernelGGL((k_integrate<DIM, true>), dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds, c->multi,
-                         verlet ? 1 : 0);
+    if (fused && c->bc_trig)
+      hipLaunchKernelGGL((k_integrate<DIM, true, true>), dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds,
+                         c->multi, verlet ? 1 : 0);
+    else if (fused)
+      hipLaunchKernelGGL((k_integrate<DIM, true, false>), dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds,
+                         c->multi, verlet ? 1 : 0);
     else
       hipLaunchKernelGGL((k_integrate<DIM, false>), dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds, c->multi,
                          verlet ? 1 : 0);
@@ -940,6 +948,7 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     o.amp = c->upload(b.amp, b.n_entries);
     c->ubc.push_back(o);
     c->ubc_zero.push_back(b.is_zero ? 1 : 0);
+    if (b.time_fn == 3 || b.time_fn == 4) c->bc_trig = true;
   }
   c->zero_done.assign(c->ubc.size(), 0);
   {

@@ -497,7 +497,9 @@ __device__ __forceinline__ void clear_cells(const DS& s, int64_t first, int64_t 
 __global__ void k_cells_init(DS s) { clear_cells(s, gid(), (int64_t)gridDim.x * blockDim.x); }
 
 // ParticleULoading::apply (particleULoading.cpp:115-230) for one node of a set
-template <int DIM>
+// (TRIG false: only the time functions 0-2, without the sine and rotation
+// code, whose FP64 sin/cos raise a kernel's register count)
+template <int DIM, bool TRIG = true>
 __device__ __forceinline__ void apply_ubc(const DS& s, int time_fn, int dir_mask, const double* tp, double umax,
                                           int64_t i, double time) {
   double du = 0., dv = 0.;
@@ -509,7 +511,7 @@ __device__ __forceinline__ void apply_ubc(const DS& s, int time_fn, int dir_mask
   } else if (time_fn == 2) {
     du = umax * time + tp[1] * time * time;
     dv = umax + tp[1] * time;
-  } else if (time_fn == 3) {
+  } else if (TRIG && time_fn == 3) {
     double a = M_PI * tp[1];
     du = umax * sin(a * time);
     dv = umax * a * cos(a * time);
@@ -520,7 +522,7 @@ __device__ __forceinline__ void apply_ubc(const DS& s, int time_fn, int dir_mask
       ui[d] = du;
       vi[d] = dv;
     }
-  if (time_fn == 4) {
+  if (TRIG && time_fn == 4) {
     double dx0 = s.xr[0][i] - tp[1], dx1 = s.xr[1][i] - tp[2];
     double dx2 = (DIM == 3 ? s.xr[2][i] : 0.) - tp[3];
     double th = tp[0] * time;
@@ -567,8 +569,9 @@ __global__ void k_xa_all(DS s) {
 // displacement-BC sets of the step (computeExternalDisplacementBC, per node
 // in set order, at the advanced time t + dt) and packs the non-wall
 // particles' centre data for the damping pass; the time advance is then left
-// to the controller kernel that follows (k_ctrl with `advance`).
-template <int DIM, bool fused>
+// to the controller kernel that follows (k_ctrl with `advance`).  TRIG false:
+// no set uses the sine or rotation time function.
+template <int DIM, bool fused, bool TRIG = true>
 __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
   int64_t i = gid();
   Ctrl* ct = s.ctrl;
@@ -626,7 +629,7 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
       for (int e = s.bc_ptr[i]; e < e1; e++) {
         const BcEntry be = s.bc_ent[e];
         const BcSetParams& bs = s.bc_sets[be.set];
-        apply_ubc<DIM>(s, bs.time_fn, bs.dir_mask, bs.tp, be.amp, i, t_new);
+        apply_ubc<DIM, TRIG>(s, bs.time_fn, bs.dir_mask, bs.tp, be.amp, i, t_new);
       }
       if (p.type_index == 0 && p.center == i) {
         const int nn = s.n_nonwall, q = p.nonwall_index;

@@ -109,16 +109,21 @@ def test_graph_replay_matches_eager():
     assert a.time() == b.time()
 
 
-@pytest.mark.parametrize("which", ["compressive", "packing"])
+@pytest.mark.parametrize("which", ["compressive", "packing", "packing_2m"])
 def test_step_fusion_is_bit_exact(which):
     """Fused step kernels (node update with BC and centre data, one-workgroup
     controller, damping pass finalising the nodes) against the unfused
     launches, graph and eager: identical fields after contact, damping,
-    gravity and the moving wall (compressive) or a 3D mixed-precision run."""
+    gravity and the moving wall (compressive) or a 3D mixed-precision run.
+    Above 2^20 nodes (the 2.08M-node packing) the fused node update and
+    controller run with the finalisation deferred to the next node update."""
     if which == "compressive":
         spec = configs.compressive_2d(n_particles=12, nodes_per_particle=120, seed=5, interval=10,
                                       wall_speed=2.0, init_speed=1.0, friction=True)
         prec = capi.PREC_FP64
+    elif which == "packing_2m":
+        spec = configs.packing_3d(init_speed=2.0)
+        prec = capi.PREC_MIXED
     else:
         spec = configs.packing_3d(n_spheres=6, nodes_per_sphere=200, L=5.0e-3, seed=3, gap_factor=1.0,
                                   init_speed=2.0)
