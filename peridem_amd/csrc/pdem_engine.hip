@@ -577,6 +577,7 @@ void clear_pending(pdem_ctx* c) {
 // every node's candidate-position record from the node arrays (after the
 // positions were set from the host or restored)
 void refresh_xa(pdem_ctx* c) {
+  if (c->ds.ctrl) hipLaunchKernelGGL(k_skin_invalidate, dim3(1), dim3(1), 0, c->stream, c->ds);
   if (!c->ds.xa) return;
   dispatch(
       c, [&] { hipLaunchKernelGGL(k_xa_all<2>, dim3(blocks(c->N)), dim3(TPB), 0, c->stream, c->ds); },
@@ -998,6 +999,7 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     for (auto& q : dc)
       if (maxR < q.contact_r) maxR = q.contact_r;
   h.max_contact_r = maxR;
+  h.dsum = INFINITY;  // no rebuild yet: the contact skin bound is unknown
   s.ctrl = c->upload(&h, 1);
 
   // ---- PD neighbor list, sliced ELL built on device
@@ -1184,6 +1186,9 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     double rc = 0.;
     for (int k = 0; k < c->nz * c->nz && c->multi; k++) rc = std::max(rc, d->contact[k].contact_r);
     s.rc2_max = rc * rc * (1.0 + 1e-12);
+    s.rc_skip = rc * (1.0 + 1e-9);
+    s.cdmin = c->dalloc<float>(N, false);
+    s.n_ubc_sets = (int)c->ubc.size();
     // a lane per node for nodes with few candidates pays off where many nodes
     // have candidates (3D, and large 2D models such as batched ensembles:
     // 0.66 -> 0.60 ms per step for 512 instances); a small 2D model with few
@@ -1193,6 +1198,7 @@ void build(pdem_ctx* c, const pdem_desc* d) {
     c->light_used = s.light_max > 0;
     s.nblockmax = (int)blocks(N);
     s.blockmax = c->dalloc<unsigned long long>(s.nblockmax);
+    s.blockmv = c->dalloc<unsigned long long>(s.nblockmax);
     std::vector<int64_t> wn;
     for (int p = 0; p < c->np; p++)
       if (c->hp[p].type_index != 0)
@@ -1474,6 +1480,9 @@ int pdem_compute_external_forces(pdem_ctx* c) {
 int pdem_compute_external_displacement_bc(pdem_ctx* c) {
   return guarded(c, [&] {
     dispatch(c, [&] { Launch<2>::displacement_bc(c); }, [&] { Launch<3>::displacement_bc(c); });
+    // moved outside a step: contact calls may follow before any controller
+    // step adds the move to the contact skin bound
+    hipLaunchKernelGGL(k_skin_invalidate, dim3(1), dim3(1), 0, c->stream, c->ds);
     check_launch(c);
   });
 }

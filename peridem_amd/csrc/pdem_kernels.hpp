@@ -84,6 +84,12 @@ struct Ctrl {
   // node update adds them (as k_finalize would, in its order), and a host
   // read of f first runs k_finalize over the pending forces
   int32_t f_pending;
+  int32_t pad2_;
+  // contact skin: an upper bound of how far any node has moved since the
+  // last rebuild's query (infinite when unknown), and the squared moves of
+  // this step's displacement-BC kernels (bit pattern, atomic maximum)
+  double dsum;
+  unsigned long long mv_bc_acc;
 };
 
 // displacement-BC set parameters and the per-node entries of the sets that
@@ -186,6 +192,14 @@ struct DS {
   // per particle: zone | wall << 16, for the contact loops
   const int* pmeta;
   double rc2_max;  // largest contact radius squared: distance prefilter
+  // contact skin: a node whose nearest candidate at the last rebuild
+  // (cdmin, rounded down) minus twice the bound of node moves since then
+  // (Ctrl::dsum) exceeds rc_skip (the largest contact radius and a margin)
+  // has no candidate within contact distance, and the contact loops skip it
+  double rc_skip;
+  int n_ubc_sets;  // displacement-BC sets (launches of k_ubc per step at most)
+  float* cdmin;  // [N]
+  unsigned long long* blockmv;  // per node-update workgroup: max squared move (bit pattern)
   int light_max;   // nodes with at most this many candidates take one lane
   // learned pair-contact closure (cl_on == 0: none of these is read)
   int cl_on;
@@ -430,7 +444,7 @@ __device__ void ctrl_decide(Ctrl* cg) {
 constexpr int CTRL_BLOCKS = 64;
 
 __global__ void __launch_bounds__(1024) k_ctrl(DS s, int advance) {
-  __shared__ unsigned long long red[1024 / WAVE];
+  __shared__ unsigned long long red[1024 / WAVE], redm[1024 / WAVE];
   Ctrl* c = s.ctrl;
   const int t = threadIdx.x;
   // the node update before this kernel consumed any pending forces, and the
@@ -447,33 +461,57 @@ __global__ void __launch_bounds__(1024) k_ctrl(DS s, int advance) {
       }
       __syncthreads();
     }
-    if (c->vdirty) {
+    const bool moved = c->vdirty != 0;  // a node update ran since the last controller step
+    unsigned long long mvb = 0;           // its largest squared node move (bit pattern)
+    if (moved) {
       // eight loads in flight per thread (one workgroup reads every block maximum)
-      unsigned long long m = 0;
+      unsigned long long m = 0, mv = 0;
       const int nb = s.nblockmax, stride = (int)blockDim.x;
       for (int k0 = t; k0 < nb; k0 += 8 * stride) {
-        unsigned long long v[8];
+        unsigned long long v[8], w[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
           const int k = k0 + u * stride;
           v[u] = k < nb ? s.blockmax[k] : 0ull;
+          w[u] = k < nb ? s.blockmv[k] : 0ull;
         }
 #pragma unroll
-        for (int u = 0; u < 8; u++) m = v[u] > m ? v[u] : m;
+        for (int u = 0; u < 8; u++) {
+          m = v[u] > m ? v[u] : m;
+          mv = w[u] > mv ? w[u] : mv;
+        }
       }
       for (int off = WAVE / 2; off > 0; off >>= 1) {
         unsigned long long o = __shfl_xor(m, off);
         m = o > m ? o : m;
+        o = __shfl_xor(mv, off);
+        mv = o > mv ? o : mv;
       }
-      if ((t & (WAVE - 1)) == 0) red[t / WAVE] = m;
+      if ((t & (WAVE - 1)) == 0) {
+        red[t / WAVE] = m;
+        redm[t / WAVE] = mv;
+      }
       __syncthreads();
       if (t == 0) {
         unsigned long long mm = 0;
-        for (int k = 0; k < (int)(blockDim.x / WAVE); k++) mm = red[k] > mm ? red[k] : mm;
+        for (int k = 0; k < (int)(blockDim.x / WAVE); k++) {
+          mm = red[k] > mm ? red[k] : mm;
+          mvb = redm[k] > mvb ? redm[k] : mvb;
+        }
         c->vmax_acc = mm;
       }
     }
-    if (t == 0) ctrl_decide(c);
+    if (t == 0) {
+      ctrl_decide(c);
+      // contact skin: a rebuild's query sees the positions of now; otherwise
+      // every node has moved at most this step's largest node-update move
+      // plus, per displacement-BC set, that set's largest move (a relative
+      // margin covers the rounding of the bound)
+      double step = (double)s.n_ubc_sets * sqrt(__longlong_as_double((long long)c->mv_bc_acc));
+      if (moved) step += sqrt(__longlong_as_double((long long)mvb));
+      c->dsum = c->rebuild ? 0. : c->dsum + step * (1. + 1e-9);
+      c->mv_bc_acc = 0ull;
+    }
   }
   if (advance) return;  // centre data packed by the fused node update
   const int nn = s.n_nonwall;
@@ -499,9 +537,10 @@ __global__ void k_cells_init(DS s) { clear_cells(s, gid(), (int64_t)gridDim.x * 
 // ParticleULoading::apply (particleULoading.cpp:115-230) for one node of a set
 // (TRIG false: only the time functions 0-2, without the sine and rotation
 // code, whose FP64 sin/cos raise a kernel's register count)
+// Returns the squared distance the node moved.
 template <int DIM, bool TRIG = true>
-__device__ __forceinline__ void apply_ubc(const DS& s, int time_fn, int dir_mask, const double* tp, double umax,
-                                          int64_t i, double time) {
+__device__ __forceinline__ double apply_ubc(const DS& s, int time_fn, int dir_mask, const double* tp, double umax,
+                                            int64_t i, double time) {
   double du = 0., dv = 0.;
   if (time_fn == 0)
     du = umax;
@@ -535,12 +574,17 @@ __device__ __forceinline__ void apply_ubc(const DS& s, int time_fn, int dir_mask
     vi[1] += tp[0] * d1;
     vi[2] += tp[0] * 0.0;
   }
+  double m2 = 0.;
   for (int d = 0; d < DIM; d++)
     if (dir_mask >> d & 1) {
       s.u[d][i] = ui[d];
       s.v[d][i] = vi[d];
-      s.x[d][i] = ui[d] + s.xr[d][i];
+      const double xn = ui[d] + s.xr[d][i];
+      const double dd = xn - s.x[d][i];
+      m2 += dd * dd;
+      s.x[d][i] = xn;
     }
+  return m2;
 }
 
 // the candidate-position record of node i from the node arrays
@@ -561,6 +605,10 @@ __global__ void k_xa_all(DS s) {
   if (i < s.N) xa_store<DIM>(s, i, s.ptid[i]);
 }
 
+// positions set from the host or restored: the contact skin bound is
+// unknown until the next rebuild
+__global__ void k_skin_invalidate(DS s) { s.ctrl->dsum = INFINITY; }
+
 // ------------------------------------------------------------ node update
 // integrateCD (demModel.cpp:349-390): semi-implicit Euler on free dofs;
 // with `half`, the first half kick and drift of integrateVerlet
@@ -578,6 +626,7 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
   const double dt = ct->dt;
   const bool pend = ct->f_pending != 0;  // k_finalize's terms still to add (same order)
   double vm = 0.;
+  double mv2 = 0.;  // squared distance the node moves in this launch (a bound for BC nodes)
   if (i < s.N) {
     const int pid = s.ptid[i];
     const DevParticle& p = s.P[pid];
@@ -611,7 +660,10 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
           vd[d] += a * fd[d];
           s.v[d][i] = vd[d];
           s.u[d][i] = ud[d] + dt * vd[d];
-          xs[d] = xs[d] + dt * vd[d];
+          const double xn = xs[d] + dt * vd[d];
+          const double dd = xn - xs[d];
+          mv2 += dd * dd;
+          xs[d] = xn;
           s.x[d][i] = xs[d];
         }
       xs_ok = true;
@@ -624,12 +676,18 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
     }
     if (fused) {
       const double t_new = ct->time + dt;
-      const int e1 = s.bc_ptr[i + 1];
-      if (e1 > s.bc_ptr[i]) xs_ok = false;  // moved by the displacement BC below
-      for (int e = s.bc_ptr[i]; e < e1; e++) {
-        const BcEntry be = s.bc_ent[e];
-        const BcSetParams& bs = s.bc_sets[be.set];
-        apply_ubc<DIM, TRIG>(s, bs.time_fn, bs.dir_mask, bs.tp, be.amp, i, t_new);
+      const int e0 = s.bc_ptr[i], e1 = s.bc_ptr[i + 1];
+      if (e1 > e0) {
+        xs_ok = false;  // moved by the displacement BC below
+        // the node's move is at most the sum of its k + 1 partial moves:
+        // squared, at most (k + 1) times the sum of their squares
+        double m2 = mv2;
+        for (int e = e0; e < e1; e++) {
+          const BcEntry be = s.bc_ent[e];
+          const BcSetParams& bs = s.bc_sets[be.set];
+          m2 += apply_ubc<DIM, TRIG>(s, bs.time_fn, bs.dir_mask, bs.tp, be.amp, i, t_new);
+        }
+        mv2 = (double)(e1 - e0 + 1) * m2;
       }
       if (p.type_index == 0 && p.center == i) {
         const int nn = s.n_nonwall, q = p.nonwall_index;
@@ -654,21 +712,31 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
     }
     return;
   }
-  // block maximum of |v| (non-negative doubles order like their bit patterns),
-  // reduced over blocks by the controller
-  __shared__ unsigned long long wmax[256 / WAVE];
+  // block maxima of |v| and of the squared move (non-negative doubles order
+  // like their bit patterns), reduced over blocks by the controller
+  __shared__ unsigned long long wmax[256 / WAVE], wmv[256 / WAVE];
   unsigned long long b = (unsigned long long)__double_as_longlong(vm);
+  unsigned long long bm = (unsigned long long)__double_as_longlong(mv2);
   for (int off = WAVE / 2; off > 0; off >>= 1) {
     unsigned long long o = __shfl_xor(b, off);
     b = o > b ? o : b;
+    o = __shfl_xor(bm, off);
+    bm = o > bm ? o : bm;
   }
   const int w = threadIdx.x / WAVE;
-  if ((threadIdx.x & (WAVE - 1)) == 0) wmax[w] = b;
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    wmax[w] = b;
+    wmv[w] = bm;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long m = 0;
-    for (int k = 0; k < (int)(blockDim.x / WAVE); k++) m = wmax[k] > m ? wmax[k] : m;
+    unsigned long long m = 0, mm = 0;
+    for (int k = 0; k < (int)(blockDim.x / WAVE); k++) {
+      m = wmax[k] > m ? wmax[k] : m;
+      mm = wmv[k] > mm ? wmv[k] : mm;
+    }
     s.blockmax[blockIdx.x] = m;
+    s.blockmv[blockIdx.x] = mm;
     if (!fused && blockIdx.x == 0) {
       ct->current_dt = dt;
       ct->n += 1;
@@ -709,15 +777,33 @@ struct DevBc {
   const double* amp;
 };
 
+// (the largest squared move goes to Ctrl::mv_bc_acc for the contact skin: a
+// node in several sets moves at most the sum of their moves, which the
+// controller bounds by the number of sets times the largest)
 template <int DIM>
-__global__ void k_ubc(DS s, DevBc bc) {
+__global__ void __launch_bounds__(256) k_ubc(DS s, DevBc bc) {
+  __shared__ unsigned long long wm[256 / WAVE];
   int64_t k = gid();
-  if (k >= bc.n) return;
-  const int64_t i = bc.nodes[k];
-  apply_ubc<DIM>(s, bc.time_fn, bc.dir_mask, bc.tp, bc.amp[k], i, s.ctrl->time);
-  if (s.xa) {
-    const int pid = s.ptid[i];
-    xa_refresh<DIM>(s, i, pid, s.P[pid].searches != 0);
+  double m2 = 0.;
+  if (k < bc.n) {
+    const int64_t i = bc.nodes[k];
+    m2 = apply_ubc<DIM>(s, bc.time_fn, bc.dir_mask, bc.tp, bc.amp[k], i, s.ctrl->time);
+    if (s.xa) {
+      const int pid = s.ptid[i];
+      xa_refresh<DIM>(s, i, pid, s.P[pid].searches != 0);
+    }
+  }
+  unsigned long long b = (unsigned long long)__double_as_longlong(m2);
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(b, off);
+    b = o > b ? o : b;
+  }
+  if ((threadIdx.x & (WAVE - 1)) == 0) wm[threadIdx.x / WAVE] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = 0;
+    for (int w = 0; w < (int)(blockDim.x / WAVE); w++) m = wm[w] > m ? wm[w] : m;
+    if (m > 0) atomicMax(&s.ctrl->mv_bc_acc, m);
   }
 }
 
@@ -1215,6 +1301,7 @@ __device__ int query_node(const DS& s, int64_t i, int pi) {
     hi[d] = cc0[d] + s.csub < s.gn[d] - 1 ? cc0[d] + s.csub : s.gn[d] - 1;
   }
   int cnt = 0;
+  double near2 = r2;  // nearest candidate (squared distance)
   const int cap = s.cap;
   for (int z = lo[2]; z <= hi[2]; z++)
     for (int y = lo[1]; y <= hi[1]; y++)
@@ -1252,6 +1339,7 @@ __device__ int query_node(const DS& s, int64_t i, int pi) {
           if (d2 < r2) {
             if (cnt < cap) s.cand[cand_idx(s.N, i, cnt)] = j;
             cnt++;
+            near2 = d2 < near2 ? d2 : near2;
           }
         }
       }
@@ -1260,7 +1348,17 @@ __device__ int query_node(const DS& s, int64_t i, int pi) {
     cnt = cap;
   }
   s.ccount[i] = cnt;
+  if (cnt > 0) s.cdmin[i] = __double2float_rd(sqrt(near2));
   return cnt;
+}
+
+// contact skin test: node i's candidates of the last rebuild were all at
+// least cdmin[i] away then, and every node has moved at most Ctrl::dsum
+// since, so none is within contact distance when cdmin[i] - 2 dsum exceeds
+// the largest contact radius (with a margin): the contact loops would find
+// no active pair for the node and skip it
+__device__ __forceinline__ bool skin_far(const DS& s, int64_t i, double two_dsum) {
+  return (double)s.cdmin[i] - two_dsum > s.rc_skip;
 }
 
 // one active contact pair's two force terms on node i (demModel.cpp:868-904):
@@ -1368,8 +1466,10 @@ template <int DIM>
 __global__ void __launch_bounds__(256) k_contact_light(DS s) {
   const int nl = (int)s.ctrl->nlight;
   const bool rec = !s.ctrl->rebuild;
+  const double two_dsum = 2. * s.ctrl->dsum;
   for (int a = (int)gid(); a < nl; a += gridDim.x * blockDim.x) {
     const int i = s.light[a];
+    if (skin_far(s, i, two_dsum)) continue;
     const int n = s.ccount[i];
     const int pid = s.ptid[i];
     const int mi = s.pmeta[pid];
@@ -1471,8 +1571,10 @@ __global__ void __launch_bounds__(CWAVES * WAVE, 4) k_contact_force(DS s) {
   const int nwaves = gridDim.x * CWAVES;
   const int nact_nodes = (int)s.ctrl->nactive;
   const bool rec = !s.ctrl->rebuild;
+  const double two_dsum = 2. * s.ctrl->dsum;
   for (int a = blockIdx.x * CWAVES + wv; a < nact_nodes; a += nwaves) {
     const int i = s.active[a];
+    if (skin_far(s, i, two_dsum)) continue;
     const int n = s.ccount[i];
     const int pid = s.ptid[i];
     const int mi = s.pmeta[pid];
