@@ -1572,102 +1572,115 @@ __global__ void __launch_bounds__(CWAVES * WAVE, 4) k_contact_force(DS s) {
   const int nact_nodes = (int)s.ctrl->nactive;
   const bool rec = !s.ctrl->rebuild;
   const double two_dsum = 2. * s.ctrl->dsum;
-  for (int a = blockIdx.x * CWAVES + wv; a < nact_nodes; a += nwaves) {
-    const int i = s.active[a];
-    if (skin_far(s, i, two_dsum)) continue;
-    const int n = s.ccount[i];
-    const int pid = s.ptid[i];
-    const int mi = s.pmeta[pid];
-    const int zi = mi & 0xffff;
-    const bool wi = (mi >> 16) != 0;
-    const int nwi = s.P[pid].nonwall_index;
-    if (!node_needs_contact(s, nwi)) continue;
-    double yi[3], vi[3];
-    for (int d = 0; d < DIM; d++) {
-      yi[d] = s.x[d][i];
-      vi[d] = s.v[d][i];
+  // the skin test for the wavefront's next 64 nodes at once (a lane each),
+  // then the nodes that pass it one after the other, in the same order
+  for (int a0 = blockIdx.x * CWAVES + wv; a0 < nact_nodes; a0 += WAVE * nwaves) {
+    const int al = a0 + lane * nwaves;
+    int il = 0;
+    bool near = false;
+    if (al < nact_nodes) {
+      il = s.active[al];
+      near = !skin_far(s, il, two_dsum);
     }
-    int cnt = 0;
-    for (int base = 0; base < n; base += WAVE) {
-      const int k = base + lane;
-      bool on = false;
-      int j = 0;
-      double term[6] = {0., 0., 0., 0., 0., 0.};
-      if (k < n) {
-        j = s.cand[cand_idx(s.N, i, k)];
-        // distance prefilter (largest contact radius) before any particle data is loaded
-        double yji[3];
-        int pj;
-        cand_pos<DIM>(s, rec, j, yi, yji, pj);
-        const double d2 = dot3<DIM>(yji, yji);
-        const int pjid = d2 < s.rc2_max ? (pj >= 0 ? pj : s.ptid[j]) : pid;
-        const int mj = s.pmeta[pjid];
-        const bool wj = (mj >> 16) != 0;
-        if (!(j == i || pjid == pid || (wi && wj))) {
-          const DevContact ct = s.CT[zi * s.nz + (mj & 0xffff)];
-          // d^2 >= rc^2 implies R >= rc (1 - 2^-53): isLess(R, rc) is false
-          if (d2 < ct.contact_r * ct.contact_r) {
-            const double Rji = len3<DIM>(yji);
-            if (isLess(Rji, ct.contact_r) &&
-                !(s.cl_on && !wj && !pair_plain(s, nwi, s.P[pjid].nonwall_index))) {
-              on = true;
-              if (wj && nwi >= 0) {
-                const int w = s.widx[j];
-                atomicOr(&s.wbits[(int64_t)nwi * s.nwords + (w >> 5)], 1u << (w & 31));
-                atomicMin(&s.wlo[nwi], w >> 5);
-                atomicMax(&s.whi[nwi], w >> 5);
+    unsigned long long todo = __ballot(near);
+    while (todo) {
+      const int l = __builtin_ctzll(todo);
+      todo &= todo - 1ull;
+      const int i = __shfl(il, l);
+      const int n = s.ccount[i];
+      const int pid = s.ptid[i];
+      const int mi = s.pmeta[pid];
+      const int zi = mi & 0xffff;
+      const bool wi = (mi >> 16) != 0;
+      const int nwi = s.P[pid].nonwall_index;
+      if (!node_needs_contact(s, nwi)) continue;
+      double yi[3], vi[3];
+      for (int d = 0; d < DIM; d++) {
+        yi[d] = s.x[d][i];
+        vi[d] = s.v[d][i];
+      }
+      int cnt = 0;
+      for (int base = 0; base < n; base += WAVE) {
+        const int k = base + lane;
+        bool on = false;
+        int j = 0;
+        double term[6] = {0., 0., 0., 0., 0., 0.};
+        if (k < n) {
+          j = s.cand[cand_idx(s.N, i, k)];
+          // distance prefilter (largest contact radius) before any particle data is loaded
+          double yji[3];
+          int pj;
+          cand_pos<DIM>(s, rec, j, yi, yji, pj);
+          const double d2 = dot3<DIM>(yji, yji);
+          const int pjid = d2 < s.rc2_max ? (pj >= 0 ? pj : s.ptid[j]) : pid;
+          const int mj = s.pmeta[pjid];
+          const bool wj = (mj >> 16) != 0;
+          if (!(j == i || pjid == pid || (wi && wj))) {
+            const DevContact ct = s.CT[zi * s.nz + (mj & 0xffff)];
+            // d^2 >= rc^2 implies R >= rc (1 - 2^-53): isLess(R, rc) is false
+            if (d2 < ct.contact_r * ct.contact_r) {
+              const double Rji = len3<DIM>(yji);
+              if (isLess(Rji, ct.contact_r) &&
+                  !(s.cl_on && !wj && !pair_plain(s, nwi, s.P[pjid].nonwall_index))) {
+                on = true;
+                if (wj && nwi >= 0) {
+                  const int w = s.widx[j];
+                  atomicOr(&s.wbits[(int64_t)nwi * s.nwords + (w >> 5)], 1u << (w & 31));
+                  atomicMin(&s.wlo[nwi], w >> 5);
+                  atomicMax(&s.whi[nwi], w >> 5);
+                }
+                contact_pair<DIM>(s, vi, j, ct, Rji, yji, term, term + 3);
               }
-              contact_pair<DIM>(s, vi, j, ct, Rji, yji, term, term + 3);
             }
           }
         }
-      }
-      const unsigned long long m = __ballot(on);
-      if (on) {
-        const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
-        if (pos < MAXACT) {
-          sj[wv][pos] = j;
-          for (int d = 0; d < 6; d++) st[wv][pos][d] = term[d];
+        const unsigned long long m = __ballot(on);
+        if (on) {
+          const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
+          if (pos < MAXACT) {
+            sj[wv][pos] = j;
+            for (int d = 0; d < 6; d++) st[wv][pos][d] = term[d];
+          }
         }
+        cnt += __popcll(m);
       }
-      cnt += __popcll(m);
-    }
-    // LDS is private to this wavefront's slot and LDS instructions of one
-    // wavefront complete in order: no workgroup barrier is needed (the waves
-    // of a workgroup run different trip counts of this loop)
-    __builtin_amdgcn_wave_barrier();
-    if (cnt > MAXACT) {
-      if (lane == 0) s.ctrl->overflow = 3;
-      cnt = MAXACT;
-    }
-    if (cnt > 1) {
-      // ascending node order: each lane moves one active pair to its rank
-      // (a node's candidates are distinct nodes); the wavefront issues all
-      // its reads of the slot before the writes
-      int kj = INT_MAX;
-      double kt[6] = {0., 0., 0., 0., 0., 0.};
-      if (lane < cnt) {
-        kj = sj[wv][lane];
-        for (int d = 0; d < 6; d++) kt[d] = st[wv][lane][d];
-      }
-      int rank = 0;
-      for (int q = 0; q < cnt; q++) rank += sj[wv][q] < kj ? 1 : 0;
+      // LDS is private to this wavefront's slot and LDS instructions of one
+      // wavefront complete in order: no workgroup barrier is needed (the waves
+      // of a workgroup run different trip counts of this loop)
       __builtin_amdgcn_wave_barrier();
-      if (lane < cnt) {
-        sj[wv][rank] = kj;
-        for (int d = 0; d < 6; d++) st[wv][rank][d] = kt[d];
+      if (cnt > MAXACT) {
+        if (lane == 0) s.ctrl->overflow = 3;
+        cnt = MAXACT;
+      }
+      if (cnt > 1) {
+        // ascending node order: each lane moves one active pair to its rank
+        // (a node's candidates are distinct nodes); the wavefront issues all
+        // its reads of the slot before the writes
+        int kj = INT_MAX;
+        double kt[6] = {0., 0., 0., 0., 0., 0.};
+        if (lane < cnt) {
+          kj = sj[wv][lane];
+          for (int d = 0; d < 6; d++) kt[d] = st[wv][lane][d];
+        }
+        int rank = 0;
+        for (int q = 0; q < cnt; q++) rank += sj[wv][q] < kj ? 1 : 0;
+        __builtin_amdgcn_wave_barrier();
+        if (lane < cnt) {
+          sj[wv][rank] = kj;
+          for (int d = 0; d < 6; d++) st[wv][rank][d] = kt[d];
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+      if (lane == 0 && cnt > 0) {
+        double force[3] = {0., 0., 0.};
+        for (int q = 0; q < cnt; q++) {
+          for (int d = 0; d < DIM; d++) force[d] += st[wv][q][d];
+          for (int d = 0; d < DIM; d++) force[d] += st[wv][q][3 + d];
+        }
+        for (int d = 0; d < DIM; d++) s.f[d][i] += force[d];
       }
       __builtin_amdgcn_wave_barrier();
     }
-    if (lane == 0 && cnt > 0) {
-      double force[3] = {0., 0., 0.};
-      for (int q = 0; q < cnt; q++) {
-        for (int d = 0; d < DIM; d++) force[d] += st[wv][q][d];
-        for (int d = 0; d < DIM; d++) force[d] += st[wv][q][3 + d];
-      }
-      for (int d = 0; d < DIM; d++) s.f[d][i] += force[d];
-    }
-    __builtin_amdgcn_wave_barrier();
   }
 }
 
