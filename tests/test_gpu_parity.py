@@ -253,3 +253,64 @@ def test_peridynamic_forces_leave_other_nodes(prec):
     bar = 1e-12 if prec == capi.PREC_FP64 else 1e-4
     assert np.abs(fg - fr).max() <= bar * np.abs(fr).max()
     assert np.abs(fg[walls] - fr[walls]).max() <= 1e-12 * np.abs(fr).max()
+
+
+def test_contact_skin_after_positions_set():
+    """The contact loops skip nodes whose candidates cannot be in contact,
+    from each node's nearest candidate at the last rebuild and a bound of the
+    node moves since.  Positions set from the host between rebuilds void
+    that bound: a particle moved onto a candidate particle mid-interval (no
+    rebuild in between) must get the same contact forces as the oracle's
+    from the same candidate lists."""
+    spec = configs.compressive_2d(n_particles=12, nodes_per_particle=120, seed=3, interval=10,
+                                  wall_speed=2.0, R_lo=1e-3, R_hi=1e-3, pitch_factor=1.03,
+                                  init_speed=0.5, dt=2e-8, friction=True)
+    g, r = _pair(spec)
+    for e in (g, r):
+        e.integrate_begin()
+        e.integrate_cd(14)
+
+    def quiet(st):
+        # the controller's next call neither rebuilds nor revisits its radius
+        # (demModel.cpp:1808-1955): counter off both the interval and its
+        # fifth
+        iv, c = st["interval"], st["counter"]
+        upi = int(0.2 * iv) if iv > 5 else 1
+        return iv > 5 and c % iv != 0 and c % upi != 0
+
+    for _ in range(40):
+        if quiet(g.contact_state()):
+            break
+        for e in (g, r):
+            e.integrate_cd(1)
+    assert quiet(g.contact_state())
+    rp, col = r.neighbors(capi.NEIGH_CONTACT)
+    x = r.field(capi.FIELD_X)
+    rc = r.contact_state()["max_contact_r"]
+    ii = np.repeat(np.arange(spec.n_nodes), np.diff(rp))
+    d = np.linalg.norm(x[ii] - x[col], axis=1)
+    pid = spec.pt_id
+    far = (d > 1.5 * rc) & (pid[ii] != pid[col])
+    far &= np.array([spec.particles[p]["type_index"] == 0 for p in pid[col]])
+    assert far.any()
+    k = int(np.argmin(np.where(far, d, np.inf)))
+    i, j = int(ii[k]), int(col[k])
+    # move j's particle straight towards node i until j sits 0.5 rc from it
+    shift = (x[i] - x[j]) * (1.0 - 0.5 * rc / d[k])
+    q = spec.particles[pid[j]]
+    sl = slice(q["glob_start"], q["glob_end"])
+    u = r.field(capi.FIELD_U)
+    x2, u2 = x.copy(), u.copy()
+    x2[sl] += shift
+    u2[sl] += shift
+    before = g.contact_state()["n_rebuilds"]
+    for e in (g, r):
+        e.set_field(capi.FIELD_X, x2)
+        e.set_field(capi.FIELD_U, u2)
+        e.compute_forces()
+    assert g.contact_state()["n_rebuilds"] == before  # the lists of the last rebuild
+    errs = compare_state(g, r, tag="skin")
+    print(f"[skin] node {i} particle {pid[i]}, candidate {j} particle {pid[j]} moved by "
+          f"{np.linalg.norm(shift):.3e} (contact radius {rc:.3e}): {errs}", flush=True)
+    fi = r.field(capi.FIELD_F)[i]
+    assert np.abs(fi).max() > 0.0
