@@ -374,7 +374,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU (particle-sharded) "
-              "form is a design in DESIGN.md and is not part of this benchmark.", file=sys.stderr)
+              "form is a design, not part of this benchmark.", file=sys.stderr)
         raise SystemExit(2)
 
     from peridem_amd import capi, configs
