@@ -634,6 +634,16 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
     // the node's candidate-position record is rewritten below (read before
     // any store: a store could alias the flag)
     const bool rec = multi && s.xa && (!p.searches || s.xa_epoch[i] == (uint8_t)ct->n_rebuilds);
+    // the fused update's BC entries and centre flag, also read before any store
+    int e0 = 0, e1 = 0;
+    bool centre = false;
+    double t_new = 0.;
+    if (fused) {
+      e0 = s.bc_ptr[i];
+      e1 = s.bc_ptr[i + 1];
+      centre = p.type_index == 0 && p.center == i;
+      t_new = ct->time + dt;
+    }
     double xs[3] = {0., 0., 0.};
     bool xs_ok = false;
     if (p.compute_force) {
@@ -675,8 +685,6 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
       for (int d = 0; d < DIM; d++) s.f[d][i] = 0.;
     }
     if (fused) {
-      const double t_new = ct->time + dt;
-      const int e0 = s.bc_ptr[i], e1 = s.bc_ptr[i + 1];
       if (e1 > e0) {
         xs_ok = false;  // moved by the displacement BC below
         // the node's move is at most the sum of its k + 1 partial moves:
@@ -689,7 +697,7 @@ __global__ void __launch_bounds__(256) k_integrate(DS s, int multi, int half) {
         }
         mv2 = (double)(e1 - e0 + 1) * m2;
       }
-      if (p.type_index == 0 && p.center == i) {
+      if (centre) {
         const int nn = s.n_nonwall, q = p.nonwall_index;
         for (int d = 0; d < 3; d++) {
           s.pcx[d * nn + q] = d < DIM ? s.x[d][i] : 0.;
